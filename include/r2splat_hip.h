@@ -1,0 +1,325 @@
+/*
+ * r2splat_hip.h — C ABI of the MI355X (gfx950) CT splatting library.
+ *
+ * Every entry point takes raw device pointers, element counts and a
+ * hipStream_t, and returns an R2S_* status code (message: r2s_last_error()).
+ * Nothing here allocates device memory: outputs and workspaces belong to the
+ * caller, so a sequence of calls can be captured into a hipGraph. Outputs that
+ * are accumulated inside a kernel are cleared by the library itself with a fill
+ * kernel (never hipMemsetAsync), so callers may pass uninitialised buffers.
+ *
+ * `dtype` selects the floating-point type of every floating-point pointer of a
+ * call (R2S_F32 = float, R2S_F64 = double). Integer buffers have fixed types as
+ * declared. Array layouts are the reference operator layouts (row-major,
+ * contiguous); each declaration cites the reference interface it replaces, in
+ * the reference repository (MedAPT-group/R2Splat), by file:line.
+ *
+ * Naming of image/volume dimensions follows the reference: for images
+ * `width` = dim1 (x, columns), `height` = dim0 (y, rows); for volumes
+ * `dim_x` = dim2 (fastest), `dim_y` = dim1, `dim_z` = dim0 (slowest).
+ */
+#ifndef R2SPLAT_HIP_H
+#define R2SPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* r2s_stream_t; /* identical to hipStream_t */
+
+#define R2S_OK 0
+#define R2S_ERR_INVALID 1 /* bad argument (shape, size, flag) */
+#define R2S_ERR_HIP 2     /* HIP runtime error (launch, device) */
+
+#define R2S_F32 0
+#define R2S_F64 1
+
+/* Library identification and error reporting. */
+int r2s_abi_version(void);
+const char* r2s_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Per-primitive geometry                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* quats [N,4] (w,x,y,z, unnormalised), scales [N,3] -> covars [N,6] packed
+ * (00,01,02,11,12,22), max_scales [N].
+ * Replaces parameter_transform_fwd_tensor (r2splat/cuda/include/Ops.h:421-424,
+ * bound at r2splat/cuda/csrc/ext.cpp:174). */
+int r2s_parameter_transform_fwd(int dtype, const void* quats, const void* scales,
+                                void* covars, void* max_scales, uint32_t N,
+                                r2s_stream_t stream);
+
+/* VJP to raw quats [N,4] and scales [N,3] from v_covars [N,6].
+ * Replaces parameter_transform_bwd_tensor (Ops.h:426-430, ext.cpp:176). */
+int r2s_parameter_transform_bwd(int dtype, const void* quats, const void* scales,
+                                const void* v_covars, void* v_quats, void* v_scales,
+                                uint32_t N, r2s_stream_t stream);
+
+/* means [N,3], covars [N,6], viewmats [C,4,4], Ks [C,3,3] ->
+ * means2d [C,N,2], cov2d [C,N,6] (projected 3x3, packed), radii int32 [C,N,2].
+ * eps and radius_clip are float for every dtype, as in the reference.
+ * Replaces world_to_image_projection_fwd_tensor (Ops.h:398-407, ext.cpp:170). */
+int r2s_world_to_image_projection_fwd(int dtype, const void* means, const void* covars,
+                                      const void* viewmats, const void* Ks,
+                                      uint32_t width, uint32_t height, float eps,
+                                      float radius_clip, void* means2d, void* cov2d,
+                                      int32_t* radii, uint32_t C, uint32_t N,
+                                      r2s_stream_t stream);
+
+/* VJP: v_means2d [C,N,2], v_cov2d [C,N,6] -> v_means [N,3], v_covars [N,6]
+ * (summed over cameras deterministically; outputs are overwritten).
+ * Replaces world_to_image_projection_bwd_tensor (Ops.h:409-418, ext.cpp:172). */
+int r2s_world_to_image_projection_bwd(int dtype, const void* means, const void* covars,
+                                      const void* viewmats, const void* Ks,
+                                      uint32_t width, uint32_t height,
+                                      const void* v_means2d, const void* v_cov2d,
+                                      void* v_means, void* v_covars, uint32_t C,
+                                      uint32_t N, r2s_stream_t stream);
+
+/* cov2d [T,6] -> conics [T,3] = (Sigma_2x2 + eps I)^-1 packed (00,01,11) and
+ * mus [T] = sqrt(2 pi (f + eps - [c,e] P [c,e]^T)); the 2x2 inverse and the
+ * Schur complement are evaluated in double precision for every dtype.
+ * Replaces conics_mu_fwd_tensor (Ops.h:214-217, ext.cpp:166). */
+int r2s_conics_mu_fwd(int dtype, const void* cov2d, double eps, void* conics, void* mus,
+                      uint32_t total, r2s_stream_t stream);
+
+/* Replaces conics_mu_bwd_tensor (Ops.h:219-225, ext.cpp:168). */
+int r2s_conics_mu_bwd(int dtype, const void* cov2d, double eps, const void* conics,
+                      const void* mus, const void* v_conics, const void* v_mus,
+                      void* v_cov2d, uint32_t total, r2s_stream_t stream);
+
+/* means [N,3], covars [N,6], max_scales [N] (unused metadata), ranges [3],
+ * centers [3] -> means3d_v [N,3] voxel coordinates, conics [N,6] (inverse voxel
+ * covariance, packed), radii int32 [N,3]. The 3x3 inversion runs in double.
+ * Replaces world_to_volume_mapping_fwd_tensor (Ops.h:432-442, ext.cpp:178). */
+int r2s_world_to_volume_mapping_fwd(int dtype, const void* means, const void* covars,
+                                    const void* max_scales, const void* ranges,
+                                    const void* centers, uint32_t dim_x, uint32_t dim_y,
+                                    uint32_t dim_z, double eps3d, double radius_clip,
+                                    void* means3d_v, void* conics, int32_t* radii,
+                                    uint32_t N, r2s_stream_t stream);
+
+/* Replaces world_to_volume_mapping_bwd_tensor (Ops.h:444-454, ext.cpp:180). */
+int r2s_world_to_volume_mapping_bwd(int dtype, const void* means, const void* covars,
+                                    const void* ranges, const void* centers,
+                                    uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+                                    double eps3d, const void* v_means3d_v,
+                                    const void* v_conics, void* v_means, void* v_covars,
+                                    uint32_t N, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Tile binning (2D and 3D)                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace sizes (bytes) for the device scan over `n` counts and for the
+ * stable radix sort of `n_isects` (key, value) pairs over `end_bit` key bits. */
+size_t r2s_scan_workspace_bytes(uint64_t n);
+size_t r2s_sort_workspace_bytes(uint64_t n_isects, uint32_t end_bit);
+
+/* Pass 1 of isect_tiles: per (camera, primitive) tile count (int32 [C,N]) and
+ * its exclusive prefix sum `cum` (uint32 [C*N+1]; cum[C*N] = n_isects, also
+ * written to *d_total, a device uint32). radii are [C,N,2] if radii_is_rect
+ * else [C,N]. Replaces the count/cumsum half of isect_tiles_tensor
+ * (r2splat/cuda/csrc/IsectTiles.cpp:31-107, Ops.h:49-55). */
+int r2s_isect_tiles_count(int dtype, const void* means2d, const int32_t* radii,
+                          int radii_is_rect, uint32_t C, uint32_t N, uint32_t tile_size,
+                          uint32_t n_tiles_x, uint32_t n_tiles_y, int32_t* tiles_per_gauss,
+                          uint32_t* cum, void* scan_ws, uint32_t* d_total,
+                          r2s_stream_t stream);
+
+/* Pass 2: write (key = cam*n_tiles + ty*n_tiles_x + tx, value = cam*N + g)
+ * records and sort them stably by key (ties keep ascending value). Outputs
+ * isect_ids int64 [n_isects] and flatten_ids int32 [n_isects].
+ * Replaces the generate + CUB sort half of isect_tiles_tensor
+ * (IsectTiles.cpp:109-170, IsectTilesKernel.cu:137-217,398-445). */
+int r2s_isect_tiles_emit(int dtype, const void* means2d, const int32_t* radii,
+                         int radii_is_rect, uint32_t C, uint32_t N, uint32_t tile_size,
+                         uint32_t n_tiles_x, uint32_t n_tiles_y, const uint32_t* cum,
+                         uint32_t n_isects, int64_t* isect_ids, int32_t* flatten_ids,
+                         void* sort_ws, size_t sort_ws_bytes, r2s_stream_t stream);
+
+/* 3D analogues over a (z,y,x) tile grid; key = tz*ny*nx + ty*nx + tx, value = g.
+ * Replaces isect_tiles_3d_tensor (r2splat/cuda/csrc/IsectTiles3D.cpp:32-206,
+ * Ops.h:61-68). */
+int r2s_isect_tiles_3d_count(int dtype, const void* means3d, const int32_t* radii,
+                             uint32_t N, uint32_t tile_size, uint32_t n_tiles_x,
+                             uint32_t n_tiles_y, uint32_t n_tiles_z,
+                             int32_t* tiles_per_gauss, uint32_t* cum, void* scan_ws,
+                             uint32_t* d_total, r2s_stream_t stream);
+int r2s_isect_tiles_3d_emit(int dtype, const void* means3d, const int32_t* radii,
+                            uint32_t N, uint32_t tile_size, uint32_t n_tiles_x,
+                            uint32_t n_tiles_y, uint32_t n_tiles_z, const uint32_t* cum,
+                            uint32_t n_isects, int64_t* isect_ids, int32_t* flatten_ids,
+                            void* sort_ws, size_t sort_ws_bytes, r2s_stream_t stream);
+
+/* offsets int32 [n_keys]: offsets[k] = first index i with isect_ids[i] >= k
+ * (n_isects if none). n_keys = C*n_tiles (2D) or n_tiles (3D).
+ * Replaces isect_offset_encode_tensor / isect_offset_encode_3d_tensor
+ * (IsectTiles.cpp:176-215, IsectTiles3D.cpp:~150-206, Ops.h:57-74). */
+int r2s_isect_offset_encode(const int64_t* isect_ids, uint32_t n_isects, uint32_t n_keys,
+                            int32_t* offsets, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 2D rasterization (CT line integrals, additive, no alpha compositing)      */
+/* ------------------------------------------------------------------------ */
+/* Common arguments: means2d [C,N,2], conics [C,N,3], opacities [C,N],
+ * mus [C,N], betas [C,N] (projected exponent, Beta ops only), uncertainties
+ * [N], tile_offsets int32 [C,n_tiles_y,n_tiles_x], flatten_ids int32
+ * [n_isects], isect_ids int64 [n_isects]; images [C,height,width]. */
+
+/* Replaces rasterize_to_pixels_fwd_tensor (Ops.h:227-236, ext.cpp:96). */
+int r2s_rasterize_to_pixels_fwd(int dtype, const void* means2d, const void* conics,
+                                const void* opacities, const void* mus,
+                                const int32_t* tile_offsets, const int32_t* flatten_ids,
+                                uint32_t n_isects, uint32_t C, uint32_t N, uint32_t width,
+                                uint32_t height, uint32_t tile_size, uint32_t n_tiles_x,
+                                uint32_t n_tiles_y, void* render, r2s_stream_t stream);
+
+/* Replaces rasterize_to_pixels_mean_var_fwd_tensor (Ops.h:238-248, ext.cpp:98). */
+int r2s_rasterize_to_pixels_mean_var_fwd(int dtype, const void* means2d, const void* conics,
+                                         const void* opacities, const void* mus,
+                                         const void* uncertainties,
+                                         const int32_t* tile_offsets,
+                                         const int32_t* flatten_ids, uint32_t n_isects,
+                                         uint32_t C, uint32_t N, uint32_t width,
+                                         uint32_t height, uint32_t tile_size,
+                                         uint32_t n_tiles_x, uint32_t n_tiles_y,
+                                         void* render, void* render_var,
+                                         r2s_stream_t stream);
+
+/* Replaces beta_rasterize_to_pixels_fwd_tensor (Ops.h:318-328, ext.cpp:133). */
+int r2s_beta_rasterize_to_pixels_fwd(int dtype, const void* means2d, const void* conics,
+                                     const void* opacities, const void* mus,
+                                     const void* betas, const int32_t* tile_offsets,
+                                     const int32_t* flatten_ids, uint32_t n_isects,
+                                     uint32_t C, uint32_t N, uint32_t width,
+                                     uint32_t height, uint32_t tile_size,
+                                     uint32_t n_tiles_x, uint32_t n_tiles_y, void* render,
+                                     r2s_stream_t stream);
+
+/* Replaces beta_rasterize_to_pixels_mean_var_fwd_tensor (Ops.h:330-341, ext.cpp:135). */
+int r2s_beta_rasterize_to_pixels_mean_var_fwd(
+    int dtype, const void* means2d, const void* conics, const void* opacities,
+    const void* mus, const void* betas, const void* uncertainties,
+    const int32_t* tile_offsets, const int32_t* flatten_ids, uint32_t n_isects, uint32_t C,
+    uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, uint32_t n_tiles_x,
+    uint32_t n_tiles_y, void* render, void* render_var, r2s_stream_t stream);
+
+/* Gradient of sum(v_render_var * var) w.r.t. uncertainties [N] (overwritten).
+ * betas == NULL selects the Gaussian profile.
+ * Replaces rasterize_to_pixels_uncertainty_bwd_tensor and its Beta variant
+ * (Ops.h:250-260,343-354; ext.cpp:100,137). */
+int r2s_rasterize_to_pixels_uncertainty_bwd(
+    int dtype, const void* means2d, const void* conics, const void* opacities,
+    const void* mus, const void* betas, const int32_t* tile_offsets,
+    const int32_t* flatten_ids, uint32_t n_isects, uint32_t C, uint32_t N, uint32_t width,
+    uint32_t height, uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y,
+    const void* v_render_var, void* v_uncertainties, r2s_stream_t stream);
+
+/* Tile-parallel ("pixel" mode) VJP. betas/v_betas NULL selects the Gaussian
+ * profile; v_means2d_abs may be NULL. All outputs are overwritten.
+ * Replaces rasterize_to_pixels_bwd_tensor / beta_rasterize_to_pixels_bwd_tensor
+ * (Ops.h:262-274,356-369; ext.cpp:102,139). */
+int r2s_rasterize_to_pixels_bwd(
+    int dtype, const void* means2d, const void* conics, const void* opacities,
+    const void* mus, const void* betas, const int32_t* tile_offsets,
+    const int32_t* flatten_ids, uint32_t n_isects, uint32_t C, uint32_t N, uint32_t width,
+    uint32_t height, uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y,
+    const void* v_render, void* v_means2d_abs, void* v_means2d, void* v_conics,
+    void* v_opacities, void* v_mus, void* v_betas, r2s_stream_t stream);
+
+/* Intersection-parallel ("primitive" mode) VJP: one wave64 per intersection,
+ * lanes stride the tile's pixels, wave reduction, then one atomic per output
+ * field per wave. v_render_var == NULL: mean-only VJP. With v_render_var and
+ * uncertainties != NULL the variance path also feeds the mean adjoint
+ * (v += 2 q alpha v_var); uncertainties == NULL is the detached form. The
+ * uncertainty gradient is written to v_uncertainties when it is non-NULL.
+ * Replaces rasterize_to_pixels_bwd_per_gaussian_tensor,
+ * rasterize_to_pixels_mean_var_bwd_per_gaussian_tensor and the Beta variants
+ * (Ops.h:276-316,371-395; ext.cpp:104-128,141-162). */
+int r2s_rasterize_to_pixels_bwd_per_gaussian(
+    int dtype, const void* means2d, const void* conics, const void* opacities,
+    const void* mus, const void* betas, const int64_t* isect_ids,
+    const int32_t* flatten_ids, uint32_t n_isects, uint32_t C, uint32_t N, uint32_t width,
+    uint32_t height, uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y,
+    const void* v_render, const void* v_render_var, const void* uncertainties,
+    void* v_means2d_abs, void* v_means2d, void* v_conics, void* v_opacities, void* v_mus,
+    void* v_betas, void* v_uncertainties, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 3D voxelization                                                           */
+/* ------------------------------------------------------------------------ */
+/* means3d [N,3] voxel coordinates, conics [N,6], opacities [N], betas [N]
+ * (physical exponent; Beta ops only), uncertainties [N], tile_offsets int32
+ * [n_tiles_z,n_tiles_y,n_tiles_x]; volumes [dim_z,dim_y,dim_x].
+ * render_var == NULL -> mean only; betas == NULL -> Gaussian profile.
+ * Replaces voxelize_to_voxels_fwd_tensor, voxelize_to_voxels_mean_var_fwd_tensor
+ * and the Beta variants (Ops.h:76-99,138-161; ext.cpp:26-29,66-69). */
+int r2s_voxelize_to_voxels_fwd(int dtype, const void* means3d, const void* conics,
+                               const void* opacities, const void* betas,
+                               const void* uncertainties, const int32_t* tile_offsets,
+                               const int32_t* flatten_ids, uint32_t n_isects, uint32_t N,
+                               uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+                               uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y,
+                               uint32_t n_tiles_z, void* render, void* render_var,
+                               r2s_stream_t stream);
+
+/* Tile-parallel VJP (mean only). Outputs overwritten; v_betas used iff betas.
+ * Replaces voxelize_to_voxels_bwd_tensor / beta_voxelize_to_voxels_bwd_tensor
+ * (Ops.h:101-112,163-175; ext.cpp:30,70). */
+int r2s_voxelize_to_voxels_bwd(int dtype, const void* means3d, const void* conics,
+                               const void* opacities, const void* betas,
+                               const int32_t* tile_offsets, const int32_t* flatten_ids,
+                               uint32_t n_isects, uint32_t N, uint32_t dim_x,
+                               uint32_t dim_y, uint32_t dim_z, uint32_t tile_size,
+                               uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                               const void* v_render, void* v_means3d, void* v_conics,
+                               void* v_opacities, void* v_betas, r2s_stream_t stream);
+
+/* Intersection-parallel VJP with optional variance path (see the 2D form).
+ * Replaces voxelize_to_voxels_bwd_per_gaussian_tensor,
+ * voxelize_to_voxels_mean_var_bwd_per_gaussian_tensor and the Beta variants
+ * (Ops.h:114-136,177-212; ext.cpp:32-54,72-94). */
+int r2s_voxelize_to_voxels_bwd_per_gaussian(
+    int dtype, const void* means3d, const void* conics, const void* opacities,
+    const void* betas, const int64_t* isect_ids, const int32_t* flatten_ids,
+    uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+    uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+    const void* v_render, const void* v_render_var, const void* uncertainties,
+    void* v_means3d, void* v_conics, void* v_opacities, void* v_betas,
+    void* v_uncertainties, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Losses                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Anisotropic TV over volume [B,D,H,W]: tv_map (per-voxel sum of |forward
+ * differences|) and tv_sum[0] = sum of tv_map (overwritten).
+ * Replaces the kernel half of volume_loss_fwd (r2splat/cuda/csrc/Loss.cpp:115-153). */
+int r2s_volume_tv_fwd(int dtype, const void* volume, uint32_t B, uint32_t D, uint32_t H,
+                      uint32_t W, void* tv_map, void* tv_sum, r2s_stream_t stream);
+
+/* grad_volume = lambda * grad_output[0] / edge_count * sign-sum of adjacent
+ * differences. Replaces volume_loss_bwd (Loss.cpp:156-188). */
+int r2s_volume_tv_bwd(int dtype, const void* volume, const void* grad_output, uint32_t B,
+                      uint32_t D, uint32_t H, uint32_t W, double lambda_tv,
+                      void* grad_volume, r2s_stream_t stream);
+
+/* Mean-L1 image loss map |p - t| and its sum (overwritten); gradient
+ * lambda * g[0] * sign(p - t) / n. Replaces image_loss_l1_fwd/bwd
+ * (Ops.h:14-34, r2splat/cuda/csrc/ImageLossFwd.cu:20-31, ImageLossBwd.cu:14-30). */
+int r2s_image_l1_fwd(int dtype, const void* prediction, const void* target, uint64_t n,
+                     void* l1_map, void* l1_sum, r2s_stream_t stream);
+int r2s_image_l1_bwd(int dtype, const void* prediction, const void* target,
+                     const void* grad_output, uint64_t n, double lambda_l1,
+                     void* grad_prediction, r2s_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* R2SPLAT_HIP_H */

@@ -1,0 +1,339 @@
+"""Explicit forward / VJP calls for physical primitive parameters.
+
+Drop-in for r2splat/explicit.py:102-351: ``rasterize_fwd`` / ``voxelize_fwd``
+return ``(output, ctx)`` without building an autograd graph, and
+``rasterize_bwd`` / ``voxelize_bwd`` consume the context once and return
+``ParameterGradients`` without touching ``.grad``. Geometry (cameras, volume
+frame) is constant; activations and loss derivatives belong to the caller.
+
+Each stage is one HIP operator from ``r2splat_amd.hip`` (the reference's op
+sequence: parameter transform -> projection -> conics/mu -> tile binning ->
+rasterization, and the VJPs in reverse).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional, Sequence, Union
+
+import torch
+
+from . import hip as ops
+from ._beta import beta_path_log_grad, beta_path_ratio
+from ._validation import _constant, _options, _primitives, _shape, _tensor, _vector
+
+Tensor = torch.Tensor
+Output = Union[Tensor, tuple]
+
+RASTER_TILE = 16
+VOXEL_TILE = 8
+_FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
+
+
+@dataclass
+class ParameterGradients:
+    """VJPs matching the physical inputs; absent optional inputs give None."""
+
+    means: Tensor
+    quats: Tensor
+    scales: Tensor
+    densities: Tensor
+    beta: Optional[Tensor] = None
+    uncertainty: Optional[Tensor] = None
+
+
+@dataclass
+class _Context:
+    kind: str
+    shape: tuple
+    primitives: tuple
+    buffers: dict
+    versions: tuple
+    eps: float
+    backward_mode: str
+    detach_variance: bool
+    used: bool = False
+
+
+def _mode(value) -> str:
+    value = str(value).lower()
+    if value not in ("pixel", "primitive"):
+        raise ValueError("backward_mode must be 'pixel' or 'primitive'")
+    return value
+
+
+def _make_context(kind, shape, primitives, buffers, originals, eps, mode, detach):
+    """Snapshots the version counter of every tensor the backward depends on
+    (original inputs, their validated copies and the saved buffers)."""
+    watched, seen = [], set()
+    named = list(originals) + list(zip(_FIELDS, primitives)) + list(buffers.items())
+    for name, value in named:
+        if isinstance(value, Tensor) and id(value) not in seen:
+            try:
+                version = value._version
+            except RuntimeError as error:
+                raise ValueError("explicit contexts require tensors with version counters") from error
+            watched.append((name, value, version))
+            seen.add(id(value))
+    return _Context(kind, shape, primitives, buffers, tuple(watched), eps, mode, detach)
+
+
+def _check_context(ctx, kind):
+    if not isinstance(ctx, _Context) or ctx.kind != kind:
+        raise TypeError(f"expected a context returned by {kind}_fwd")
+    if ctx.used:
+        raise RuntimeError("this context has already been consumed by backward")
+    for name, value, version in ctx.versions:
+        if value._version != version:
+            raise RuntimeError(f"saved input {name} was modified in-place after forward")
+
+
+def _adjoint(name, value, ctx):
+    return None if value is None else _tensor(name, value, ctx.shape, ctx.primitives[0])
+
+
+def _zeros(ctx):
+    return ParameterGradients(*(None if v is None else torch.zeros_like(v) for v in ctx.primitives))
+
+
+def _consume(ctx, gradients):
+    ctx.used = True
+    ctx.primitives = ()
+    ctx.buffers.clear()
+    ctx.versions = ()
+    return gradients
+
+
+# ---------------------------------------------------------------------------
+# projection
+# ---------------------------------------------------------------------------
+@torch.no_grad()
+def rasterize_fwd(
+    means: Tensor,
+    quats: Tensor,
+    scales: Tensor,
+    densities: Tensor,
+    viewmats: Tensor,
+    Ks: Tensor,
+    image_size: Sequence[int],
+    *,
+    beta: Optional[Tensor] = None,
+    uncertainty: Optional[Tensor] = None,
+    eps: float = 1.0e-8,
+    radius_clip: Optional[float] = None,
+    detach_variance: bool = False,
+    backward_mode: str = "primitive",
+):
+    """Projects primitives to CT images [C, H, W] (or (mean, variance)) and a VJP context."""
+    originals = list(zip(("means", "quats", "scales", "densities", "viewmats", "Ks", "beta", "uncertainty"),
+                         (means, quats, scales, densities, viewmats, Ks, beta, uncertainty)))
+    height, width = _shape("image_size", image_size, 2)
+    prims = _primitives(means, quats, scales, densities, beta, uncertainty)
+    means, quats, scales, densities, beta, uncertainty = prims
+    eps, radius_clip = _options(beta, eps, radius_clip, detach_variance)
+    mode = _mode(backward_mode)
+    for name, value in (("viewmats", viewmats), ("Ks", Ks)):
+        if isinstance(value, Tensor):
+            _constant(name, value)
+    viewmats = _constant("viewmats", _tensor("viewmats", viewmats, (None, 4, 4), means))
+    C = viewmats.shape[0]
+    if C == 0:
+        raise ValueError("viewmats must contain at least one camera")
+    if C * max(height * width, means.shape[0]) > 2**31 - 1 or C > 65535:
+        raise ValueError("camera batch exceeds the kernels' indexing or grid limits")
+    if max(height, width) > 16 * 65535:
+        raise ValueError("image_size exceeds the kernels' grid limits")
+    Ks = _constant("Ks", _tensor("Ks", Ks, (C, 3, 3), means))
+    if bool((Ks[:, (0, 1), (0, 1)] <= 0).any()):
+        raise ValueError("Ks must have positive fx and fy")
+    if bool((Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any()) or bool((Ks[:, 2, 2] != 1).any()):
+        raise ValueError("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]")
+    if not torch.equal(viewmats[:, 3], means.new_tensor([0, 0, 0, 1]).expand(C, -1)):
+        raise ValueError("viewmats must have homogeneous last row [0, 0, 0, 1]")
+    shape = (C, height, width)
+    buffers = {"viewmats": viewmats, "Ks": Ks}
+    N = means.shape[0]
+    if N == 0:
+        image = means.new_zeros(shape)
+        output = image if uncertainty is None else (image, image.clone())
+    else:
+        T = RASTER_TILE
+        covars, _ = ops.parameter_transform_fwd(quats, scales)
+        mapped, cov2d, radii = ops.world_to_image_projection_fwd(means, covars, viewmats, Ks, width, height,
+                                                                 eps, radius_clip)
+        conics, base_mus = ops.conics_mu_fwd(cov2d, eps)
+        opacity = densities.unsqueeze(0).expand(C, -1).contiguous()
+        mus, beta2d, ratio = base_mus, None, None
+        if beta is not None:
+            ratio = beta_path_ratio(beta)
+            mus = base_mus * ratio.unsqueeze(0)
+            beta2d = (beta + 0.5).unsqueeze(0).expand(C, -1).contiguous()
+        tiles_x, tiles_y = (width + T - 1) // T, (height + T - 1) // T
+        _, isect_ids, flatten_ids = ops.isect_tiles(mapped, radii, T, tiles_x, tiles_y)
+        offsets = ops.isect_offset_encode(isect_ids, C, tiles_x, tiles_y)
+        prefix = "beta_" if beta is not None else ""
+        args = (mapped, conics, opacity, mus) + (() if beta is None else (beta2d,))
+        if uncertainty is None:
+            output = getattr(ops, prefix + "rasterize_to_pixels_fwd")(*args, offsets, flatten_ids, width, height, T)
+        else:
+            output = getattr(ops, prefix + "rasterize_to_pixels_mean_var_fwd")(
+                *args, uncertainty, offsets, flatten_ids, width, height, T)
+        buffers.update(covars=covars, mapped=mapped, cov2d=cov2d, conics=conics, base_mus=base_mus, mus=mus,
+                       opacity=opacity, beta2d=beta2d, ratio=ratio, offsets=offsets, isect_ids=isect_ids,
+                       flatten_ids=flatten_ids)
+    ctx = _make_context("rasterize", shape, prims, buffers, originals, eps, mode, detach_variance)
+    return output, ctx
+
+
+@torch.no_grad()
+def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Optional[Tensor] = None):
+    """Consumes a projection context; returns physical-parameter gradients."""
+    _check_context(context, "rasterize")
+    means, quats, scales, densities, beta, uncertainty = context.primitives
+    if v_variance is not None and uncertainty is None:
+        raise ValueError("v_variance requires uncertainty in the forward call")
+    v_image = _adjoint("v_image", v_image, context)
+    v_variance = _adjoint("v_variance", v_variance, context)
+    if means.shape[0] == 0 or (v_image is None and v_variance is None):
+        return _consume(context, _zeros(context))
+    if v_image is None:
+        v_image = means.new_zeros(context.shape)
+    b = context.buffers
+    _, height, width = context.shape
+    T = RASTER_TILE
+    prefix = "beta_" if beta is not None else ""
+    args = (b["mapped"], b["conics"], b["opacity"], b["mus"]) + (() if beta is None else (b["beta2d"],))
+    if v_variance is not None:
+        grads = getattr(ops, prefix + "rasterize_to_pixels_mean_var_bwd_per_gaussian")(
+            *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, v_variance, uncertainty,
+            context.detach_variance, width, height, T, False)
+        v_uncertainty = grads[-1]
+    else:
+        if context.backward_mode == "primitive":
+            grads = getattr(ops, prefix + "rasterize_to_pixels_bwd_per_gaussian")(
+                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, width, height, T, False)
+        else:
+            grads = getattr(ops, prefix + "rasterize_to_pixels_bwd")(
+                *args, b["offsets"], b["flatten_ids"], v_image, width, height, T, False)
+        v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
+    v_mapped, v_conics, v_opacity, v_mus = grads[1:5]
+    v_beta = None
+    if beta is not None:
+        # mu_beta = mu_gauss * ratio(beta) and the projected exponent is beta + 1/2.
+        v_beta = grads[5].sum(dim=0) + (v_mus * b["mus"]).sum(dim=0) * beta_path_log_grad(beta)
+        v_mus = (v_mus * b["ratio"].unsqueeze(0)).contiguous()
+    v_cov2d = ops.conics_mu_bwd(b["cov2d"], context.eps, b["conics"], b["base_mus"], v_conics.contiguous(),
+                                v_mus.contiguous())
+    v_means, v_covars = ops.world_to_image_projection_bwd(means, b["covars"], b["viewmats"], b["Ks"], width, height,
+                                                          v_mapped.contiguous(), v_cov2d)
+    v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
+    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_opacity.sum(dim=0), v_beta,
+                                                v_uncertainty))
+
+
+# ---------------------------------------------------------------------------
+# voxelization
+# ---------------------------------------------------------------------------
+@torch.no_grad()
+def voxelize_fwd(
+    means: Tensor,
+    quats: Tensor,
+    scales: Tensor,
+    densities: Tensor,
+    volume_shape: Sequence[int],
+    *,
+    volume_extent,
+    volume_center=(0.0, 0.0, 0.0),
+    beta: Optional[Tensor] = None,
+    uncertainty: Optional[Tensor] = None,
+    eps: float = 1.0e-8,
+    radius_clip: Optional[float] = None,
+    detach_variance: bool = False,
+    backward_mode: str = "primitive",
+):
+    """Samples primitives at voxel centres: volume [D, H, W] (or (mean, variance)) and a VJP context."""
+    originals = list(zip(("means", "quats", "scales", "densities", "volume_extent", "volume_center", "beta",
+                          "uncertainty"),
+                         (means, quats, scales, densities, volume_extent, volume_center, beta, uncertainty)))
+    depth, height, width = _shape("volume_shape", volume_shape, 3)
+    if max(height, width) > 8 * 65535:
+        raise ValueError("volume_shape exceeds the kernels' grid limits")
+    prims = _primitives(means, quats, scales, densities, beta, uncertainty)
+    means, quats, scales, densities, beta, uncertainty = prims
+    eps, radius_clip = _options(beta, eps, radius_clip, detach_variance)
+    mode = _mode(backward_mode)
+    for name, value in (("volume_extent", volume_extent), ("volume_center", volume_center)):
+        if isinstance(value, Tensor):
+            _constant(name, value)
+    extent = _vector("volume_extent", volume_extent, means)
+    center = _vector("volume_center", volume_center, means)
+    if bool((extent <= 0).any()):
+        raise ValueError("volume_extent must contain positive physical lengths")
+    shape = (depth, height, width)
+    buffers = {"extent": extent, "center": center}
+    if means.shape[0] == 0:
+        volume = means.new_zeros(shape)
+        output = volume if uncertainty is None else (volume, volume.clone())
+    else:
+        T = VOXEL_TILE
+        covars, max_scales = ops.parameter_transform_fwd(quats, scales)
+        mapped, conics, radii = ops.world_to_volume_mapping_fwd(means, covars, max_scales, extent, center, width,
+                                                                height, depth, eps, radius_clip)
+        tx, ty, tz = (width + T - 1) // T, (height + T - 1) // T, (depth + T - 1) // T
+        _, isect_ids, flatten_ids = ops.isect_tiles_3d(mapped, radii, T, tx, ty, tz)
+        offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
+        prefix = "beta_" if beta is not None else ""
+        args = (mapped, conics, densities) + (() if beta is None else (beta,))
+        if uncertainty is None:
+            output = getattr(ops, prefix + "voxelize_to_voxels_fwd")(*args, offsets, flatten_ids, width, height,
+                                                                     depth, T)
+        else:
+            output = getattr(ops, prefix + "voxelize_to_voxels_mean_var_fwd")(
+                *args, uncertainty, offsets, flatten_ids, width, height, depth, T)
+        buffers.update(covars=covars, mapped=mapped, conics=conics, offsets=offsets, isect_ids=isect_ids,
+                       flatten_ids=flatten_ids)
+    ctx = _make_context("voxelize", shape, prims, buffers, originals, eps, mode, detach_variance)
+    return output, ctx
+
+
+@torch.no_grad()
+def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Optional[Tensor] = None):
+    """Consumes a volume context; returns physical-parameter gradients."""
+    _check_context(context, "voxelize")
+    means, quats, scales, densities, beta, uncertainty = context.primitives
+    if v_variance is not None and uncertainty is None:
+        raise ValueError("v_variance requires uncertainty in the forward call")
+    v_volume = _adjoint("v_volume", v_volume, context)
+    v_variance = _adjoint("v_variance", v_variance, context)
+    if means.shape[0] == 0 or (v_volume is None and v_variance is None):
+        return _consume(context, _zeros(context))
+    if v_volume is None:
+        v_volume = means.new_zeros(context.shape)
+    b = context.buffers
+    depth, height, width = context.shape
+    T = VOXEL_TILE
+    prefix = "beta_" if beta is not None else ""
+    args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
+    if v_variance is not None:
+        grads = getattr(ops, prefix + "voxelize_to_voxels_mean_var_bwd_per_gaussian")(
+            *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, v_variance, uncertainty,
+            context.detach_variance, width, height, depth, T)
+        v_uncertainty = grads[-1]
+    else:
+        if context.backward_mode == "primitive":
+            grads = getattr(ops, prefix + "voxelize_to_voxels_bwd_per_gaussian")(
+                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, width, height, depth, T)
+        else:
+            grads = getattr(ops, prefix + "voxelize_to_voxels_bwd")(
+                *args, b["offsets"], b["flatten_ids"], v_volume, width, height, depth, T)
+        v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
+    v_mapped, v_conics, v_densities = grads[:3]
+    v_beta = None if beta is None else grads[3]
+    v_means, v_covars = ops.world_to_volume_mapping_bwd(means, b["covars"], b["extent"], b["center"], width, height,
+                                                        depth, context.eps, v_mapped.contiguous(),
+                                                        v_conics.contiguous())
+    v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
+    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
+
+
+__all__ = ["ParameterGradients", "rasterize_fwd", "rasterize_bwd", "voxelize_fwd", "voxelize_bwd"]

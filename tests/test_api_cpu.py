@@ -1,0 +1,90 @@
+"""Host-side contracts that need no GPU: public surface, validation errors, and
+the C ABI library (loads, exports every declared symbol, matches the ctypes table)."""
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import pytest
+import torch
+
+import r2splat_amd
+from r2splat_amd import rasterize_fwd, voxelize_fwd
+from r2splat_amd._validation import _options, _shape
+from r2splat_amd.hip import _backend
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "r2splat_hip.h"
+
+
+def test_public_api_is_small():
+    assert r2splat_amd.__all__ == ["__version__", "rasterize_fwd", "rasterize_bwd", "voxelize_fwd",
+                                   "voxelize_bwd", "ParameterGradients"]
+    assert not hasattr(r2splat_amd, "rasterize")
+
+
+@pytest.mark.parametrize("shape", [(0, 3), (True, 3), (2.5, 3), (3,), (2**30, 4)])
+def test_invalid_output_shape_rejected(shape):
+    with pytest.raises(ValueError, match="image_size"):
+        _shape("image_size", shape, 2)
+
+
+@pytest.mark.parametrize("eps,radius,detach", [(-1, None, False), (math.nan, None, False), (0, 0, False),
+                                               (0, math.inf, False), (0, 1, "yes")])
+def test_invalid_numeric_options_rejected(eps, radius, detach):
+    with pytest.raises((ValueError, TypeError)):
+        _options(None, eps, radius, detach)
+
+
+def test_radius_clip_defaults():
+    assert _options(None, 0.0, None, False) == (0.0, 3.0)
+    assert _options(torch.ones(1), 0.0, None, False) == (0.0, 1.0)
+
+
+def test_cpu_inputs_fail_before_native_dispatch():
+    values = (torch.zeros(1, 3), torch.ones(1, 4), torch.ones(1, 3), torch.ones(1))
+    with pytest.raises(ValueError, match="means must be a CUDA tensor"):
+        voxelize_fwd(*values, (3, 4, 5), volume_extent=(1, 1, 1))
+    with pytest.raises(ValueError, match="means must be a CUDA tensor"):
+        rasterize_fwd(*values, torch.eye(4)[None], torch.eye(3)[None], (4, 4))
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.int32])
+def test_unsupported_precision_is_a_type_error(dtype):
+    with pytest.raises(TypeError, match="dtype"):
+        voxelize_fwd(torch.zeros(1, 3, dtype=dtype), None, None, None, (3, 4, 5), volume_extent=(1, 1, 1))
+
+
+def _header_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(r2s_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_backend.library_path()))
+    symbols = _header_symbols()
+    assert len(symbols) >= 30
+    missing = [s for s in symbols if not hasattr(lib, s)]
+    assert missing == []
+
+
+def test_ctypes_table_covers_the_header():
+    assert sorted(_backend.SIGNATURES) == _header_symbols()
+
+
+def test_library_version_and_workspace_queries_need_no_gpu():
+    lib = _backend.lib(require_gpu=False)
+    assert lib.r2s_abi_version() == 1
+    assert lib.r2s_scan_workspace_bytes(0) > 0
+    big = lib.r2s_sort_workspace_bytes(10**7, 15)
+    assert big >= 4 * 4 * 10**7
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
+def test_operators_raise_without_gpu():
+    from r2splat_amd import hip
+
+    assert not hip.is_cuda_extension_available()
+    with pytest.raises(RuntimeError):
+        hip.parameter_transform_fwd(torch.ones(1, 4), torch.ones(1, 3))
