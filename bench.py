@@ -1,0 +1,148 @@
+"""Benchmark: fwd+bwd iterations/s of the CT splatting path on one MI355X.
+
+Workload (one step): N = 100,000 Gaussians, 64 views of 256 x 256 (tile 16),
+float32; rasterize forward -> MSE adjoint 2 (img - target) / numel against a
+target rendered from a second Gaussian set -> rasterize backward to physical
+parameter gradients. Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (see DESIGN.md for the field definitions), including the
+roofline of the dominant kernel measured with HIP events on its launch stream
+during the timed region, and a CPU baseline (the C/OpenMP restatement of the
+same iteration timed on a sample of the views on the host cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd iters/s (100k Gauss, 64×256² views) vs plain path; p99 fit latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--path", default="explicit", choices=["explicit"])
+    p.add_argument("--cpu-views", type=int, default=16, help="views in the CPU-baseline sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md and not measured)"}))
+        sys.exit(2)
+    import torch
+
+    import r2splat_amd
+    from r2splat_amd import profiling, workloads
+
+    device = torch.device("cuda:0")
+    sc = workloads.config1(device=device, dtype=torch.float32, seed=0)
+    tgt_scene = workloads.config1(device=device, dtype=torch.float32, seed=1)
+    with torch.no_grad():
+        target, _ = r2splat_amd.rasterize_fwd(*tgt_scene.params(), tgt_scene.viewmats, tgt_scene.Ks, sc.image_size)
+    del tgt_scene
+
+    def step():
+        img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+        v = (2.0 / img.numel()) * (img - target)
+        return r2splat_amd.rasterize_bwd(ctx, v), ctx
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    profiling.reset()
+    profiling.enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grads, ctx = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    profiling.enable(False)
+    stages = profiling.collect()
+    ms = (t1 - t0) * 1e3 / args.steps
+
+    # intersections of this workload (fixed across steps: the parameters do not change)
+    from r2splat_amd import hip as ops
+
+    with torch.no_grad():
+        covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
+        m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width,
+                                                         sc.height, 1e-8, 3.0)
+        counts, _, _ = ops.isect_tiles(m2, radii, 16, 16, 16)
+        n_isects = int(counts.sum().item())
+    pairs = n_isects * 16 * 16
+    kernel_stages = {k: v for k, v in stages.items() if k in FLOP_PER_PAIR}
+    dominant = max(kernel_stages, key=lambda k: kernel_stages[k][0])
+    dom_ms = kernel_stages[dominant][0]
+    achieved = pairs * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
+                "avg_launch_ms": round(dom_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, target, args.cpu_views)
+
+    result = {
+        "metric": METRIC,
+        "value": round(1e3 / ms, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian sets, orbit cameras; target from a second seeded set)",
+        "config": {"workload": "config1: N=100000 Gaussians, C=64 views of 256x256, tile 16, MSE fwd+bwd "
+                               "through rasterize_fwd/rasterize_bwd",
+                   "path": args.path, "n_isects": n_isects},
+        "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(sc, target, views):
+    import numpy as np
+
+    from oracle import cpu_baseline as cb
+
+    C = sc.viewmats.shape[0]
+    views = max(1, min(views, C))
+    threads = min(cb.host_cores(), 16)
+    params = [t.detach().cpu().numpy() for t in sc.params()]
+    vm = sc.viewmats[:views].cpu().numpy()
+    K = sc.Ks[:views].cpu().numpy()
+    tg = target[:views].cpu().numpy().astype(np.float32)
+    t0 = time.perf_counter()
+    cb.iteration(*params, vm, K, sc.height, sc.width, tg, threads=threads)
+    dt = time.perf_counter() - t0
+    per_iter = dt * C / views
+    return {"value": round(1.0 / per_iter, 5), "unit": "iters/s", "cores": threads, "kind": "port",
+            "sample": f"{views} of {C} views x all {sc.means.shape[0]} Gaussians, one fwd+MSE+bwd iteration "
+                      f"({dt:.2f} s), scaled by {C}/{views}"}
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,8 @@ __device__ __forceinline__ void tile_range(const RasterArgs<S>& A, uint32_t blk,
     const uint64_t key = uint64_t(cam) * nt + tile;
     start = A.offsets[key];
     end = (key + 1 == uint64_t(A.C) * nt) ? int32_t(A.n_isects) : A.offsets[key + 1];
+    start = max(0, min(start, int32_t(A.n_isects)));
+    end = max(start, min(end, int32_t(A.n_isects)));
 }
 
 template <typename S, bool BETA, bool VAR>
@@ -63,14 +65,16 @@ __device__ __forceinline__ void stage_batch(const RasterArgs<S>& A, const PrimLD
                                             int32_t end, uint32_t batch) {
     const uint32_t t = threadIdx.x;
     if (t < batch && b0 + int32_t(t) < end) {
-        const int32_t g = A.flatten[b0 + t];
+        int32_t g = A.flatten[b0 + t];
+        const bool ok = g >= 0 && uint32_t(g) < A.C * A.N;
+        if (!ok) g = 0;  // out-of-range ids contribute nothing
         L.id[t] = g;
         L.mx[t] = A.means2d[2 * size_t(g)];
         L.my[t] = A.means2d[2 * size_t(g) + 1];
         L.a[t] = A.conics[3 * size_t(g)];
         L.b[t] = A.conics[3 * size_t(g) + 1];
         L.c[t] = A.conics[3 * size_t(g) + 2];
-        L.opac[t] = A.opac[g];
+        L.opac[t] = ok ? A.opac[g] : S(0);
         L.mu[t] = A.mus[g];
         if (BETA) L.beta[t] = A.betas[g];
         if (VAR) L.q[t] = A.unc[uint32_t(g) % A.N];

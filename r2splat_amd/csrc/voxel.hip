@@ -141,7 +141,9 @@ template <typename S, bool BETA, bool VAR>
 __device__ __forceinline__ void vox_stage(const VoxArgs<S>& A, const VoxLDS<S>& L, int32_t b0, int32_t end) {
     const uint32_t t = threadIdx.x;
     if (t < L.batch && b0 + int32_t(t) < end) {
-        const int32_t g = A.flatten[b0 + t];
+        int32_t g = A.flatten[b0 + t];
+        const bool ok = g >= 0 && uint32_t(g) < A.N;
+        if (!ok) g = 0;  // out-of-range ids contribute nothing
         const size_t G = size_t(g);
         L.id[t] = g;
         L.f[t] = A.means3d[3 * G];
@@ -149,7 +151,7 @@ __device__ __forceinline__ void vox_stage(const VoxArgs<S>& A, const VoxLDS<S>& 
         L.f[2 * L.batch + t] = A.means3d[3 * G + 2];
 #pragma unroll
         for (int k = 0; k < 6; ++k) L.f[(3 + k) * L.batch + t] = A.conics[6 * G + k];
-        L.f[9 * L.batch + t] = A.opac[g];
+        L.f[9 * L.batch + t] = ok ? A.opac[g] : S(0);
         if (BETA) L.f[10 * L.batch + t] = A.betas[g];
         if (VAR) L.q[t] = A.unc[g];
     }
@@ -169,6 +171,8 @@ __device__ __forceinline__ void vox_tile(const VoxArgs<S>& A, uint32_t blk, uint
     start = A.offsets[blk];
     const uint32_t nt = A.ntx * A.nty * A.ntz;
     end = (blk + 1 == nt) ? int32_t(A.n_isects) : A.offsets[blk + 1];
+    start = max(0, min(start, int32_t(A.n_isects)));
+    end = max(start, min(end, int32_t(A.n_isects)));
 }
 
 template <typename S, bool BETA, bool VAR>

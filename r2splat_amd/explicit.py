@@ -21,6 +21,7 @@ import torch
 from . import hip as ops
 from ._beta import beta_path_log_grad, beta_path_ratio
 from ._validation import _constant, _options, _primitives, _shape, _tensor, _vector
+from .profiling import stage
 
 Tensor = torch.Tensor
 Output = Union[Tensor, tuple]
@@ -158,10 +159,11 @@ def rasterize_fwd(
         output = image if uncertainty is None else (image, image.clone())
     else:
         T = RASTER_TILE
-        covars, _ = ops.parameter_transform_fwd(quats, scales)
-        mapped, cov2d, radii = ops.world_to_image_projection_fwd(means, covars, viewmats, Ks, width, height,
-                                                                 eps, radius_clip)
-        conics, base_mus = ops.conics_mu_fwd(cov2d, eps)
+        with stage("project"):
+            covars, _ = ops.parameter_transform_fwd(quats, scales)
+            mapped, cov2d, radii = ops.world_to_image_projection_fwd(means, covars, viewmats, Ks, width, height,
+                                                                     eps, radius_clip)
+            conics, base_mus = ops.conics_mu_fwd(cov2d, eps)
         opacity = densities.unsqueeze(0).expand(C, -1).contiguous()
         mus, beta2d, ratio = base_mus, None, None
         if beta is not None:
@@ -169,15 +171,18 @@ def rasterize_fwd(
             mus = base_mus * ratio.unsqueeze(0)
             beta2d = (beta + 0.5).unsqueeze(0).expand(C, -1).contiguous()
         tiles_x, tiles_y = (width + T - 1) // T, (height + T - 1) // T
-        _, isect_ids, flatten_ids = ops.isect_tiles(mapped, radii, T, tiles_x, tiles_y)
-        offsets = ops.isect_offset_encode(isect_ids, C, tiles_x, tiles_y)
+        with stage("bin"):
+            _, isect_ids, flatten_ids = ops.isect_tiles(mapped, radii, T, tiles_x, tiles_y)
+            offsets = ops.isect_offset_encode(isect_ids, C, tiles_x, tiles_y)
         prefix = "beta_" if beta is not None else ""
         args = (mapped, conics, opacity, mus) + (() if beta is None else (beta2d,))
-        if uncertainty is None:
-            output = getattr(ops, prefix + "rasterize_to_pixels_fwd")(*args, offsets, flatten_ids, width, height, T)
-        else:
-            output = getattr(ops, prefix + "rasterize_to_pixels_mean_var_fwd")(
-                *args, uncertainty, offsets, flatten_ids, width, height, T)
+        with stage("rasterize_fwd"):
+            if uncertainty is None:
+                output = getattr(ops, prefix + "rasterize_to_pixels_fwd")(*args, offsets, flatten_ids, width,
+                                                                         height, T)
+            else:
+                output = getattr(ops, prefix + "rasterize_to_pixels_mean_var_fwd")(
+                    *args, uncertainty, offsets, flatten_ids, width, height, T)
         buffers.update(covars=covars, mapped=mapped, cov2d=cov2d, conics=conics, base_mus=base_mus, mus=mus,
                        opacity=opacity, beta2d=beta2d, ratio=ratio, offsets=offsets, isect_ids=isect_ids,
                        flatten_ids=flatten_ids)
@@ -203,32 +208,34 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], b["opacity"], b["mus"]) + (() if beta is None else (b["beta2d"],))
-    if v_variance is not None:
-        grads = getattr(ops, prefix + "rasterize_to_pixels_mean_var_bwd_per_gaussian")(
-            *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, v_variance, uncertainty,
-            context.detach_variance, width, height, T, False)
-        v_uncertainty = grads[-1]
-    else:
-        if context.backward_mode == "primitive":
-            grads = getattr(ops, prefix + "rasterize_to_pixels_bwd_per_gaussian")(
-                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, width, height, T, False)
+    with stage("rasterize_bwd"):
+        if v_variance is not None:
+            grads = getattr(ops, prefix + "rasterize_to_pixels_mean_var_bwd_per_gaussian")(
+                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, v_variance, uncertainty,
+                context.detach_variance, width, height, T, False)
+            v_uncertainty = grads[-1]
         else:
-            grads = getattr(ops, prefix + "rasterize_to_pixels_bwd")(
-                *args, b["offsets"], b["flatten_ids"], v_image, width, height, T, False)
-        v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
+            if context.backward_mode == "primitive":
+                grads = getattr(ops, prefix + "rasterize_to_pixels_bwd_per_gaussian")(
+                    *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_image, width, height, T, False)
+            else:
+                grads = getattr(ops, prefix + "rasterize_to_pixels_bwd")(
+                    *args, b["offsets"], b["flatten_ids"], v_image, width, height, T, False)
+            v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
     v_mapped, v_conics, v_opacity, v_mus = grads[1:5]
     v_beta = None
     if beta is not None:
         # mu_beta = mu_gauss * ratio(beta) and the projected exponent is beta + 1/2.
         v_beta = grads[5].sum(dim=0) + (v_mus * b["mus"]).sum(dim=0) * beta_path_log_grad(beta)
         v_mus = (v_mus * b["ratio"].unsqueeze(0)).contiguous()
-    v_cov2d = ops.conics_mu_bwd(b["cov2d"], context.eps, b["conics"], b["base_mus"], v_conics.contiguous(),
-                                v_mus.contiguous())
-    v_means, v_covars = ops.world_to_image_projection_bwd(means, b["covars"], b["viewmats"], b["Ks"], width, height,
-                                                          v_mapped.contiguous(), v_cov2d)
-    v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
-    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_opacity.sum(dim=0), v_beta,
-                                                v_uncertainty))
+    with stage("project_bwd"):
+        v_cov2d = ops.conics_mu_bwd(b["cov2d"], context.eps, b["conics"], b["base_mus"], v_conics.contiguous(),
+                                    v_mus.contiguous())
+        v_means, v_covars = ops.world_to_image_projection_bwd(means, b["covars"], b["viewmats"], b["Ks"], width,
+                                                              height, v_mapped.contiguous(), v_cov2d)
+        v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
+        v_densities = v_opacity.sum(dim=0)
+    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
 
 
 # ---------------------------------------------------------------------------

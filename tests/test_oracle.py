@@ -17,7 +17,14 @@ from oracle import splat as O
 from r2splat_amd._beta import beta_path_log_grad, beta_path_ratio
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-torch.set_default_dtype(torch.float64)
+
+
+@pytest.fixture(autouse=True)
+def _float64_default():
+    previous = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(previous)
 
 
 # --------------------------------------------------------------------------- Beta
