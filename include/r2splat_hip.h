@@ -294,6 +294,42 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
     void* v_uncertainties, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Fused float32 Gaussian path (16x16 tiles)                                 */
+/* ------------------------------------------------------------------------ */
+/* One call runs the whole rasterize_fwd sequence of the reference
+ * (r2splat/explicit.py:160-183: parameter transform, projection, conics/mu,
+ * isect_tiles + sort, offsets, rasterize_to_pixels_fwd) from physical
+ * parameters, and one call the whole rasterize_bwd sequence
+ * (explicit.py:208-241), with identical semantics. Intersection-dependent
+ * sizes stay on the device: the sort covers min(counters[0], capacity)
+ * records, so the pair can be captured into a hipGraph.
+ *
+ * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
+ * packed = capacity records of r2s_fused_record_bytes() each, tile_offsets
+ * int32 [C, ceil(H/16), ceil(W/16)] and counters uint32 [2] persist from the
+ * forward to the backward. counters[0] = intersections, counters[1] = 1 when
+ * capacity was exceeded (that forward's outputs are then incomplete and the
+ * call must be repeated with a larger capacity). grad_scratch: float [C*N*6]. */
+size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity);
+size_t r2s_fused_record_bytes(void);
+
+int r2s_fused_forward(const float* means, const float* quats, const float* scales,
+                      const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                      uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
+                      uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                      int32_t* tile_offsets, uint32_t* counters, float* image,
+                      r2s_stream_t stream);
+
+/* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N]
+ * (overwritten; camera sums in a fixed order). */
+int r2s_fused_backward(const float* means, const float* quats, const float* scales,
+                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                       uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
+                       const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                       float* grad_scratch, const float* v_image, float* v_means, float* v_quats,
+                       float* v_scales, float* v_densities, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Losses                                                                    */
 /* ------------------------------------------------------------------------ */
 

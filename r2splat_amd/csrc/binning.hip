@@ -19,23 +19,6 @@ namespace r2s {
 // (mean -/+ radius) / tile_size, clamped to [0, n_tiles] before conversion.
 // ---------------------------------------------------------------------------
 template <typename S>
-__device__ __forceinline__ uint32_t clamp_bound(S v, uint32_t upper) {
-    if (!isfinite(v) || v <= S(0)) return 0u;
-    if (v >= S(upper)) return upper;
-    return static_cast<uint32_t>(v);
-}
-
-template <typename S>
-__device__ __forceinline__ bool tile_span(S mean, int32_t radius, uint32_t tile, uint32_t n_tiles,
-                                          uint32_t& lo, uint32_t& hi) {
-    const S t = S(tile);
-    const S c = mean / t, r = S(radius) / t;
-    lo = clamp_bound(floor(c - r), n_tiles);
-    hi = clamp_bound(ceil(c + r), n_tiles);
-    return hi > lo;
-}
-
-template <typename S>
 __device__ __forceinline__ bool rect_2d(const S* __restrict__ means2d, const int32_t* __restrict__ radii,
                                         bool rect, size_t idx, uint32_t tile, uint32_t ntx,
                                         uint32_t nty, uint32_t& x0, uint32_t& x1, uint32_t& y0,
@@ -252,10 +235,16 @@ constexpr uint32_t kSortTile = kSortThreads * kSortRounds;  // 4096 keys per blo
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 
+// `n` is the element count, or the capacity when `d_n` (device count) is given.
+__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* d_n) {
+    return d_n ? min(*d_n, n) : n;
+}
+
 __global__ void __launch_bounds__(kSortThreads)
-radix_upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift, uint32_t mask,
-                     uint32_t n_blocks, uint32_t* __restrict__ hist) {
+radix_upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ d_n,
+                     uint32_t shift, uint32_t mask, uint32_t n_blocks, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
+    n = live_count(n, d_n);
     h[threadIdx.x] = 0;
     __syncthreads();
     const size_t base = size_t(blockIdx.x) * kSortTile;
@@ -272,8 +261,10 @@ template <typename KOut>
 __global__ void __launch_bounds__(kSortThreads)
 radix_downsweep_kernel(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                        KOut* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
-                       uint32_t shift, uint32_t nbits, uint32_t n_blocks,
-                       const uint32_t* __restrict__ hist_scanned) {
+                       const uint32_t* __restrict__ d_n, uint32_t shift, uint32_t nbits,
+                       uint32_t n_blocks, const uint32_t* __restrict__ hist_scanned) {
+    n = live_count(n, d_n);
+    if (size_t(blockIdx.x) * kSortTile >= n) return;  // whole block beyond the live count
     __shared__ uint32_t wave_cnt[kSortWaves][kRadix];
     __shared__ uint32_t local_off[kRadix];
     __shared__ uint32_t global_base[kRadix];
@@ -375,10 +366,12 @@ uint32_t* sort_vals_buffer(void* ws, uint64_t n) {
     return reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + sort_layout(n).off_a_vals);
 }
 
-// Sorts the pairs held in the workspace's A buffers (written by an emit kernel)
-// and writes int64 keys / int32 values to the final outputs.
-int sort_pairs_to_output(void* ws, size_t ws_bytes, uint32_t n, uint32_t end_bit, int64_t* keys_out,
-                         int32_t* vals_out, hipStream_t stream) {
+// Sorts the pairs held in the workspace's A buffers (written by an emit kernel).
+// `n` is the element count, or a capacity when the live count is on the device
+// (`d_n`); the last pass writes keys as KOut and values to the final outputs.
+template <typename KOut>
+static int sort_pairs_impl(void* ws, size_t ws_bytes, uint32_t n, const uint32_t* d_n, uint32_t end_bit,
+                           KOut* keys_out, uint32_t* vals_out, hipStream_t stream) {
     const SortLayout L = sort_layout(n);
     R2S_REQUIRE(ws_bytes >= L.total, "sort workspace too small: %zu < %zu bytes", ws_bytes, L.total);
     if (n == 0) return R2S_OK;
@@ -395,22 +388,33 @@ int sort_pairs_to_output(void* ws, size_t ws_bytes, uint32_t n, uint32_t end_bit
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = p * kRadixBits;
         const uint32_t nb = std::min<uint32_t>(kRadixBits, bits - shift);
-        radix_upsweep_kernel<<<L.n_blocks, kSortThreads, 0, stream>>>(ka, n, shift, (1u << nb) - 1u,
+        radix_upsweep_kernel<<<L.n_blocks, kSortThreads, 0, stream>>>(ka, n, d_n, shift, (1u << nb) - 1u,
                                                                        L.n_blocks, hist);
         if (int rc = check_launch("radix_upsweep")) return rc;
         if (int rc = scan_exclusive(hist, hist_n, hist, nullptr, scan_ws, stream)) return rc;
         if (p + 1 == passes) {
-            radix_downsweep_kernel<int64_t><<<L.n_blocks, kSortThreads, 0, stream>>>(
-                ka, va, keys_out, reinterpret_cast<uint32_t*>(vals_out), n, shift, nb, L.n_blocks, hist);
+            radix_downsweep_kernel<KOut><<<L.n_blocks, kSortThreads, 0, stream>>>(
+                ka, va, keys_out, vals_out, n, d_n, shift, nb, L.n_blocks, hist);
         } else {
             radix_downsweep_kernel<uint32_t><<<L.n_blocks, kSortThreads, 0, stream>>>(
-                ka, va, kb, vb, n, shift, nb, L.n_blocks, hist);
+                ka, va, kb, vb, n, d_n, shift, nb, L.n_blocks, hist);
             uint32_t* t = ka; ka = kb; kb = t;
             t = va; va = vb; vb = t;
         }
         if (int rc = check_launch("radix_downsweep")) return rc;
     }
     return R2S_OK;
+}
+
+int sort_pairs_to_output(void* ws, size_t ws_bytes, uint32_t n, uint32_t end_bit, int64_t* keys_out,
+                         int32_t* vals_out, hipStream_t stream) {
+    return sort_pairs_impl<int64_t>(ws, ws_bytes, n, nullptr, end_bit, keys_out,
+                                    reinterpret_cast<uint32_t*>(vals_out), stream);
+}
+
+int sort_pairs_u32(void* ws, size_t ws_bytes, uint32_t capacity, const uint32_t* d_n, uint32_t end_bit,
+                   uint32_t* keys_out, uint32_t* vals_out, hipStream_t stream) {
+    return sort_pairs_impl<uint32_t>(ws, ws_bytes, capacity, d_n, end_bit, keys_out, vals_out, stream);
 }
 
 uint32_t key_end_bit(uint64_t max_key) {

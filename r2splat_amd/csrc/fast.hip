@@ -1,0 +1,451 @@
+// Fused float32 Gaussian pipeline for 16x16 tiles (the working-size path).
+//
+// Forward:  fused projection (parameter transform + projection + double
+//           precision conic/mu + radii + tile count, one 32-byte splat record
+//           per (camera, primitive)) -> device scan -> record emission ->
+//           capacity-bounded stable radix sort -> tile offsets -> per-
+//           intersection record packing -> rasterization.
+// Backward: rasterization VJP into per-(camera, primitive) accumulators ->
+//           one lane per primitive summing the conic/mu, projection and
+//           parameter VJPs over cameras in a fixed order.
+// All sizes that depend on the intersection count are read on the device, so
+// the whole sequence can be captured into a hipGraph; the only host-visible
+// condition is capacity overflow (reported in a device flag).
+//
+// Semantics are those of the operator kernels (raster.hip, geometry.hip) with
+// two precomputations per primitive: sigma is evaluated in log2 units
+// (A = a/2 log2 e, B = b log2 e, C = c/2 log2 e) and the contribution test
+// |w e^-sigma| >= 1e-5 becomes 0 <= sigma log2 e <= S with S = log2(|w| 1e5).
+#include "binning.hpp"
+#include "common.hpp"
+#include "geometry_common.hpp"
+#include "raster_common.hpp"
+
+namespace r2s {
+
+struct __align__(16) Splat {
+    float mx, my, A, B, C, w, S;
+    int32_t g;
+};
+static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
+constexpr uint32_t kT = 16;                          // tile edge of the fused path
+constexpr int kRasterWaves = 4;                      // waves (tiles) per workgroup
+
+// ---------------------------------------------------------------------------
+// Fused projection + tile count. grid (ceil(N/256), C).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+fused_project_kernel(const float* __restrict__ means, const float* __restrict__ quats,
+                     const float* __restrict__ scales, const float* __restrict__ dens,
+                     const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
+                     uint32_t H, float eps, float clip, uint32_t ntx, uint32_t nty, uint32_t N,
+                     Splat* __restrict__ recs, uint2* __restrict__ rects, uint32_t* __restrict__ counts) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.y;
+    if (n >= N) return;
+    const size_t idx = size_t(c) * N + n;
+    Camera<float> cam;
+    cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
+    float q[4], mag, norm, R[9], cov[6];
+    unit_quat(quats + 4 * size_t(n), q, mag, norm);
+    quat_to_rot(q, R);
+    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
+    covariance_from(R, s, cov);
+    const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
+    Splat rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1.f, int32_t(idx)};
+    uint2 rect = make_uint2(0, 0);
+    uint32_t count = 0;
+    ProjState<float> st;
+    const float Wf = float(W), Hf = float(H);
+    if (project_state(cam, p, cov, Wf, Hf, st)) {
+        const float mx = cam.fx * st.x / st.z + cam.cx;
+        const float my = cam.fy * st.y / st.z + cam.cy;
+        float c2[6];
+        projected_cov(st, c2);
+        int32_t rx, ry;
+        footprint_radii(c2, mx, my, eps, clip, Wf, Hf, rx, ry);
+        double cd[6], P[3], mu;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
+        conic_mu_f64(cd, double(eps), P, mu);
+        const float w = dens[n] * float(mu);
+        rec.mx = mx;
+        rec.my = my;
+        rec.A = 0.5f * float(P[0]) * kLog2e;
+        rec.B = float(P[1]) * kLog2e;
+        rec.C = 0.5f * float(P[2]) * kLog2e;
+        rec.w = w;
+        rec.S = (w != 0.f) ? __log2f(fabsf(w)) + kLog2Inv1e5 : -1.f;
+        uint32_t x0, x1, y0, y1;
+        if (rx > 0 && ry > 0 && tile_span(mx, rx, kT, ntx, x0, x1) && tile_span(my, ry, kT, nty, y0, y1)) {
+            count = (x1 - x0) * (y1 - y0);
+            rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
+        }
+    }
+    recs[idx] = rec;
+    rects[idx] = rect;
+    counts[idx] = count;
+}
+
+// Records (key = cam * n_tiles + tile, value = cam * N + n) at their scanned
+// positions; entries that would pass the capacity raise the overflow flag.
+__global__ void __launch_bounds__(256)
+fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ cum, uint32_t total,
+                  uint32_t N, uint32_t ntx, uint32_t nt, uint32_t capacity, uint32_t* __restrict__ keys,
+                  uint32_t* __restrict__ vals, uint32_t* __restrict__ overflow) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t w0 = cum[i], cnt = cum[i + 1] - w0;
+    if (cnt == 0) return;
+    if (w0 + cnt > capacity) {
+        *overflow = 1u;
+        return;
+    }
+    const uint2 r = rects[i];
+    const uint32_t x0 = r.x & 0xffffu, x1 = r.x >> 16, y0 = r.y & 0xffffu, y1 = r.y >> 16;
+    const uint32_t base = (i / N) * nt;
+    uint32_t w = w0;
+    for (uint32_t y = y0; y < y1; ++y)
+        for (uint32_t x = x0; x < x1; ++x, ++w) {
+            keys[w] = base + y * ntx + x;
+            vals[w] = i;
+        }
+}
+
+__global__ void __launch_bounds__(256)
+fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                     uint32_t n_keys, int32_t* __restrict__ offsets) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_keys) return;
+    const uint32_t n = min(*d_n, capacity);
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    offsets[k] = int32_t(lo);
+}
+
+__global__ void __launch_bounds__(256)
+fused_pack_kernel(const Splat* __restrict__ recs, const uint32_t* __restrict__ vals,
+                  const uint32_t* __restrict__ d_n, uint32_t capacity, Splat* __restrict__ packed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= min(*d_n, capacity)) return;
+    packed[i] = recs[vals[i]];
+}
+
+// Tile -> [start, end) of its packed list. Every quantity is wave-uniform.
+__device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, uint32_t tile,
+                                           uint32_t n_tiles_total, uint32_t n_live, int32_t& start,
+                                           int32_t& end) {
+    start = offsets[tile];
+    end = (tile + 1 == n_tiles_total) ? int32_t(n_live) : offsets[tile + 1];
+    start = max(0, min(start, int32_t(n_live)));
+    end = max(start, min(end, int32_t(n_live)));
+}
+
+// ---------------------------------------------------------------------------
+// Forward: one wave per 16x16 tile; lane = column (lane % 16) and row group
+// (lane / 16), 4 rows per lane (r, r+4, r+8, r+12). Records are wave-uniform
+// (scalar loads); sigma reuses the column terms across the lane's 4 pixels.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kRasterWaves * kWave)
+fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
+                        const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
+                        uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H, float* __restrict__ out) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t tile = blockIdx.x * kRasterWaves + wave;
+    if (tile >= n_tiles_total) return;
+    const uint32_t n_live = min(*d_n, capacity);
+    int32_t start, end;
+    fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    const uint32_t lane = threadIdx.x % kWave, col = lane % 16, row0 = lane / 16;
+    const float px = float(tx * kT + col) + 0.5f;
+    float py[4], acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        py[k] = float(ty * kT + row0 + 4 * k) + 0.5f;
+        acc[k] = 0.f;
+    }
+    for (int32_t i = start; i < end; ++i) {
+        const Splat s = packed[i];
+        const float dx = px - s.mx;
+        const float adx2 = s.A * dx * dx, bdx = s.B * dx;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float dy = py[k] - s.my;
+            const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);
+            const float e = __builtin_amdgcn_exp2f(-q);
+            acc[k] += (q >= 0.f && q <= s.S) ? s.w * e : 0.f;
+        }
+    }
+    const uint32_t x = tx * kT + col;
+    if (x < W) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t y = ty * kT + row0 + 4 * k;
+            if (y < H) out[(size_t(cam) * H + y) * W + x] = acc[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward: one wave per tile, one lane per primitive of the tile's list
+// (chunks of 64). The tile's adjoint image sits in LDS and is read with
+// wave-uniform (broadcast) loads, row by row, so each lane sums its own
+// primitive's moments over the 256 pixels without any cross-lane reduction:
+//   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
+// with dx = px - mx, and the VJP is
+//   v_mean = w (a Mx + b My, b Mx + c My), v_conic = -w (Mxx/2, Mxy, Myy/2),
+//   v_w = M0, accumulated into grad6[cam*N + n] with one atomic per field.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kRasterWaves * kWave)
+fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
+                        const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
+                        uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
+                        const float* __restrict__ v_img, float* __restrict__ grad6) {
+    __shared__ __align__(16) float s_v[kRasterWaves][kT * kT];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t tile = blockIdx.x * kRasterWaves + wave;
+    if (tile >= n_tiles_total) return;
+    const uint32_t n_live = min(*d_n, capacity);
+    int32_t start, end;
+    fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    const uint32_t lane = threadIdx.x % kWave;
+    float* sv = s_v[wave];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t p = lane + kWave * k, r = p / kT, cidx = p % kT;
+        const uint32_t x = tx * kT + cidx, y = ty * kT + r;
+        sv[p] = (x < W && y < H) ? v_img[(size_t(cam) * H + y) * W + x] : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float x0 = float(tx * kT) + 0.5f, y0 = float(ty * kT) + 0.5f;
+    for (int32_t chunk = start; chunk < end; chunk += kWave) {
+        const int32_t i = chunk + int32_t(lane);
+        const bool active = i < end;
+        Splat s = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1.f, -1};
+        if (active) s = packed[i];
+        const float dx0 = x0 - s.mx;
+        float m0 = 0.f, mx = 0.f, my = 0.f, mxx = 0.f, mxy = 0.f, myy = 0.f;
+        for (uint32_t r = 0; r < kT; ++r) {
+            const float dy = y0 + float(r) - s.my;
+            const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
+            const float4* row = reinterpret_cast<const float4*>(sv + r * kT);
+            float v[16];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float4 q4 = row[k];
+                v[4 * k] = q4.x;
+                v[4 * k + 1] = q4.y;
+                v[4 * k + 2] = q4.z;
+                v[4 * k + 3] = q4.w;
+            }
+            float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const float dx = dx0 + float(j);
+                const float q = fmaf(dx, fmaf(s.A, dx, bdy), cdy2);
+                const float e = __builtin_amdgcn_exp2f(-q);
+                const float g = (q >= 0.f && q <= s.S) ? e * v[j] : 0.f;
+                r0 += g;
+                const float gx = g * dx;
+                r1 += gx;
+                r2 = fmaf(gx, dx, r2);
+            }
+            m0 += r0;
+            mx += r1;
+            mxx += r2;
+            my = fmaf(dy, r0, my);
+            mxy = fmaf(dy, r1, mxy);
+            myy = fmaf(dy * dy, r0, myy);
+        }
+        if (active) {
+            // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
+            const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
+            float* gp = grad6 + 6 * size_t(s.g);
+            atomic_add(gp + 0, s.w * (a * mx + b * my));
+            atomic_add(gp + 1, s.w * (b * mx + c * my));
+            atomic_add(gp + 2, -0.5f * s.w * mxx);
+            atomic_add(gp + 3, -s.w * mxy);
+            atomic_add(gp + 4, -0.5f * s.w * myy);
+            atomic_add(gp + 5, m0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-primitive VJP chain, summed over cameras in order (deterministic):
+// conic/mu VJP (double) -> projection VJP -> parameter-transform VJP.
+// grad6[cam*N + n] = (v_mx, v_my, v_a, v_b, v_c, v_w) with w = density * mu.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restrict__ quats,
+                           const float* __restrict__ scales, const float* __restrict__ dens,
+                           const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
+                           uint32_t H, float eps, uint32_t C, uint32_t N, const float* __restrict__ grad6,
+                           float* __restrict__ v_means, float* __restrict__ v_quats,
+                           float* __restrict__ v_scales, float* __restrict__ v_dens) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float q[4], mag, norm, R[9], cov[6];
+    unit_quat(quats + 4 * size_t(n), q, mag, norm);
+    quat_to_rot(q, R);
+    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
+    covariance_from(R, s, cov);
+    const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
+    const float opac = dens[n];
+    float vp[3] = {0.f, 0.f, 0.f}, vcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vd = 0.f;
+    const float Wf = float(W), Hf = float(H);
+    for (uint32_t c = 0; c < C; ++c) {
+        const float* g = grad6 + 6 * (size_t(c) * N + n);
+        const float g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5];
+        if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f) continue;
+        Camera<float> cam;
+        cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
+        ProjState<float> st;
+        if (!project_state(cam, p, cov, Wf, Hf, st)) continue;
+        float c2[6];
+        projected_cov(st, c2);
+        double cd[6], P[3], mu;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
+        conic_mu_f64(cd, double(eps), P, mu);
+        // the forward stored the conic and mu in float32
+        double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
+        const double muf = double(float(mu));
+        vd += float(muf) * g5;
+        const double vP[3] = {double(g2), double(g3), double(g4)};
+        double gcov[6];
+        conic_mu_vjp_f64(cd, Pf, muf, vP, double(opac) * double(g5), gcov);
+        const float vm[2] = {g0, g1};
+        float vc[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) vc[k] = float(gcov[k]);
+        projection_vjp(cam, st, vm, vc, vp, vcov);
+    }
+    float gq[4], vs[3];
+    covariance_vjp(q, R, s, vcov, gq, vs);
+    normalize_vjp(q, mag, norm, gq, v_quats + 4 * size_t(n));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v_means[3 * size_t(n) + k] = vp[k];
+        v_scales[3 * size_t(n) + k] = vs[k];
+    }
+    v_dens[n] = vd;
+}
+
+// ---------------------------------------------------------------------------
+// Forward scratch layout (released by the caller after the forward)
+// ---------------------------------------------------------------------------
+struct FusedScratch {
+    size_t recs, rects, cum, scan, sort, sort_bytes, keys, vals, bytes;
+};
+
+static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t capacity) {
+    FusedScratch L;
+    const size_t CN = size_t(C) * N;
+    size_t o = 0;
+    L.recs = o; o += al(CN * sizeof(Splat));
+    L.rects = o; o += al(CN * sizeof(uint2));
+    L.cum = o; o += al((CN + 1) * 4);
+    L.scan = o; o += al(scan_workspace_bytes(CN));
+    L.sort_bytes = sort_workspace_bytes(capacity);
+    L.sort = o; o += al(L.sort_bytes);
+    L.keys = o; o += al(size_t(capacity) * 4 + 4);
+    L.vals = o; o += al(size_t(capacity) * 4 + 4);
+    L.bytes = o;
+    return L;
+}
+
+template <typename T>
+static T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(ws) + off); }
+
+}  // namespace r2s
+
+using namespace r2s;
+
+extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity) {
+    return fused_scratch(C, N, capacity).bytes;
+}
+
+extern "C" size_t r2s_fused_record_bytes(void) { return sizeof(Splat); }
+
+extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
+                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
+                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                                 int32_t* tile_offsets, uint32_t* counters, float* image,
+                                 r2s_stream_t stream) {
+    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
+    const FusedScratch L = fused_scratch(C, N, capacity);
+    R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
+    R2S_REQUIRE(uint64_t(C) * N < (1ull << 31), "C*N exceeds the int32 index range");
+    R2S_REQUIRE(uint64_t(C) * ntx * nty < (1ull << 32), "C*n_tiles exceeds the key range");
+    R2S_REQUIRE(ntx < 65536 && nty < 65536, "tile grid too large");
+    R2S_REQUIRE(capacity > 0 && capacity < (1u << 31), "capacity must be in (0, 2^31)");
+    const uint32_t CN = C * N, n_tiles = C * ntx * nty;
+    if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
+    uint32_t* cum = at<uint32_t>(scratch, L.cum);
+    if (CN > 0) {
+        dim3 grid(ceil_div(N, 256), C);
+        fused_project_kernel<<<grid, 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks, width,
+                                                       height, eps, radius_clip, ntx, nty, N,
+                                                       at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), cum);
+        if (int rc = check_launch("fused_project")) return rc;
+    }
+    if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
+    void* sort_ws = at<void>(scratch, L.sort);
+    if (CN > 0) {
+        fused_emit_kernel<<<ceil_div(CN, 256), 256, 0, stream>>>(at<uint2>(scratch, L.rects), cum, CN, N, ntx,
+                                                                 ntx * nty, capacity, sort_keys_buffer(sort_ws),
+                                                                 sort_vals_buffer(sort_ws, capacity), counters + 1);
+        if (int rc = check_launch("fused_emit")) return rc;
+    }
+    uint32_t* keys = at<uint32_t>(scratch, L.keys);
+    uint32_t* vals = at<uint32_t>(scratch, L.vals);
+    if (int rc = sort_pairs_u32(sort_ws, L.sort_bytes, capacity, counters, key_end_bit(n_tiles), keys, vals,
+                                stream))
+        return rc;
+    fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, counters, capacity, n_tiles,
+                                                                     tile_offsets);
+    if (int rc = check_launch("fused_offsets")) return rc;
+    fused_pack_kernel<<<ceil_div(capacity, 256), 256, 0, stream>>>(at<Splat>(scratch, L.recs), vals, counters,
+                                                                   capacity, static_cast<Splat*>(packed));
+    if (int rc = check_launch("fused_pack")) return rc;
+    fused_raster_fwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
+        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
+        image);
+    return check_launch("fused_raster_fwd");
+}
+
+extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
+                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                  uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
+                                  const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                                  float* grad_scratch, const float* v_image, float* v_means, float* v_quats,
+                                  float* v_scales, float* v_densities, r2s_stream_t stream) {
+    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
+    const uint32_t CN = C * N, n_tiles = C * ntx * nty;
+    if (int rc = fill_zero<float>(grad_scratch, size_t(CN) * 6, stream)) return rc;
+    fused_raster_bwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
+        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
+        v_image, grad_scratch);
+    if (int rc = check_launch("fused_raster_bwd")) return rc;
+    if (N == 0) return R2S_OK;
+    fused_primitive_bwd_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks,
+                                                                     width, height, eps, C, N, grad_scratch,
+                                                                     v_means, v_quats, v_scales, v_densities);
+    return check_launch("fused_primitive_bwd");
+}

@@ -13,11 +13,13 @@ rasterization, and the VJPs in reverse).
 
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 from typing import Optional, Sequence, Union
 
 import torch
 
+from . import fused
 from . import hip as ops
 from ._beta import beta_path_log_grad, beta_path_ratio
 from ._validation import _constant, _options, _primitives, _shape, _tensor, _vector
@@ -29,6 +31,14 @@ Output = Union[Tensor, tuple]
 RASTER_TILE = 16
 VOXEL_TILE = 8
 _FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
+
+# "fused": float32 Gaussian mean-only projections run the fused pipeline
+# (r2splat_amd.fused); every other case, and "ops", runs one operator per stage.
+BACKEND = os.environ.get("R2S_BACKEND", "fused")
+
+
+def _use_fused(means, beta, uncertainty) -> bool:
+    return BACKEND == "fused" and means.dtype == torch.float32 and beta is None and uncertainty is None
 
 
 @dataclass
@@ -157,6 +167,11 @@ def rasterize_fwd(
     if N == 0:
         image = means.new_zeros(shape)
         output = image if uncertainty is None else (image, image.clone())
+    elif _use_fused(means, beta, uncertainty):
+        with stage("rasterize_fwd"):
+            output, fwd = fused.forward(means, quats, scales, densities, viewmats, Ks, width, height, eps,
+                                        radius_clip)
+        buffers["fused"] = fwd
     else:
         T = RASTER_TILE
         with stage("project"):
@@ -204,6 +219,11 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
     if v_image is None:
         v_image = means.new_zeros(context.shape)
     b = context.buffers
+    if "fused" in b:
+        with stage("rasterize_bwd"):
+            v_means, v_quats, v_scales, v_dens = fused.backward(b["fused"], means, quats, scales, densities,
+                                                                b["viewmats"], b["Ks"], v_image)
+        return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_dens))
     _, height, width = context.shape
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""

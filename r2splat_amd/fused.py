@@ -1,0 +1,108 @@
+"""Fused float32 Gaussian rasterization (16x16 tiles) on the HIP library.
+
+``FusedForward`` holds what the backward needs (per-intersection records,
+tile offsets, device counters) for one forward call. Capacity for the
+intersection list is sized from the previous call with the same shapes
+(``CapacityCache``) and grown on overflow, so steady-state calls perform no
+host-visible sizing besides the overflow check.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import torch
+
+from .hip import _backend as be
+from .hip._backend import call, ptr
+
+TILE = 16
+
+
+class CapacityCache:
+    """Per (device, C, N, H, W) intersection capacity, grown on overflow."""
+
+    def __init__(self, headroom: float = 1.15):
+        self.headroom = headroom
+        self._cap = {}
+
+    def get(self, key, C, N):
+        return self._cap.get(key, max(1024, 4 * C * N))
+
+    def grow(self, key, needed: int):
+        self._cap[key] = max(1024, int(math.ceil(needed * self.headroom)))
+        return self._cap[key]
+
+    def settle(self, key, used: int):
+        cap = self._cap.get(key)
+        if cap is None or used > cap or cap > 2 * used * self.headroom + 4096:
+            self.grow(key, used)
+
+
+CAPACITY = CapacityCache()
+
+
+@dataclass
+class FusedForward:
+    packed: torch.Tensor
+    offsets: torch.Tensor
+    counters: torch.Tensor
+    capacity: int
+    n_isects: int
+    width: int
+    height: int
+    eps: float
+
+
+def _scratch(C, N, capacity, device):
+    nbytes = be.lib().r2s_fused_scratch_bytes(C, N, capacity)
+    return torch.empty((nbytes,), dtype=torch.uint8, device=device)
+
+
+def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
+            check=True):
+    """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
+    device = means.device
+    C, N = viewmats.shape[0], means.shape[0]
+    key = (device, C, N, height, width)
+    cap = int(capacity or CAPACITY.get(key, C, N))
+    ntx, nty = -(-width // TILE), -(-height // TILE)
+    rec = be.lib().r2s_fused_record_bytes()
+    image = torch.empty((C, height, width), dtype=torch.float32, device=device)
+    offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
+    counters = torch.zeros((2,), dtype=torch.int32, device=device)
+    with torch.cuda.device(device):
+        stream = be.stream_of(device)
+        while True:
+            scratch = _scratch(C, N, cap, device)
+            packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
+            call("r2s_fused_forward", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats), ptr(Ks),
+                 C, N, width, height, float(eps), float(radius_clip), cap, ptr(scratch), scratch.numel(),
+                 ptr(packed), ptr(offsets), ptr(counters), ptr(image), stream)
+            del scratch
+            if not check:
+                return image, FusedForward(packed, offsets, counters, cap, -1, width, height, float(eps))
+            n_isects, overflow = (int(v) for v in counters.tolist())
+            if not overflow:
+                break
+            cap = CAPACITY.grow(key, n_isects)
+    CAPACITY.settle(key, n_isects)
+    return image, FusedForward(packed, offsets, counters, cap, n_isects, width, height, float(eps))
+
+
+def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image):
+    """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities)."""
+    device = means.device
+    C, N = viewmats.shape[0], means.shape[0]
+    grad = torch.empty((C * N * 6,), dtype=torch.float32, device=device)
+    v_means = torch.empty_like(means)
+    v_quats = torch.empty_like(quats)
+    v_scales = torch.empty_like(scales)
+    v_dens = torch.empty_like(densities)
+    with torch.cuda.device(device):
+        call("r2s_fused_backward", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats), ptr(Ks),
+             C, N, fwd.width, fwd.height, fwd.eps, fwd.capacity, ptr(fwd.packed), ptr(fwd.offsets),
+             ptr(fwd.counters), ptr(grad), ptr(v_image), ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens),
+             be.stream_of(device))
+    return v_means, v_quats, v_scales, v_dens
