@@ -34,6 +34,7 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--path", default="explicit", choices=["explicit"])
+    p.add_argument("--plain-steps", type=int, default=5, help="timed steps of the staged operator path")
     p.add_argument("--cpu-views", type=int, default=16, help="views in the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
@@ -62,20 +63,26 @@ def main():
         v = (2.0 / img.numel()) * (img - target)
         return r2splat_amd.rasterize_bwd(ctx, v), ctx
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    profiling.reset()
-    profiling.enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        grads, ctx = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    profiling.enable(False)
-    stages = profiling.collect()
-    ms = (t1 - t0) * 1e3 / args.steps
+    def timed(backend, steps, warmup):
+        explicit.BACKEND = backend
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        profiling.reset()
+        profiling.enable(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        profiling.enable(False)
+        return (t1 - t0) * 1e3 / steps, profiling.collect()
+
+    from r2splat_amd import explicit
+
+    plain_ms, plain_stages = timed("ops", args.plain_steps, 1)
+    ms, stages = timed("fused", args.steps, args.warmup)
 
     # intersections of this workload (fixed across steps: the parameters do not change)
     from r2splat_amd import hip as ops
@@ -117,6 +124,11 @@ def main():
                                "through rasterize_fwd/rasterize_bwd",
                    "path": args.path, "n_isects": n_isects},
         "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
+        "plain_path": {"value": round(1e3 / plain_ms, 3), "unit": "iters/s", "ms_per_step": round(plain_ms, 3),
+                       "speedup_of_value": round(plain_ms / ms, 3),
+                       "description": "same step through one HIP operator per stage (staged reference op "
+                                      "sequence, host-side intersection sizing)",
+                       "stages_ms": {k: round(v[0], 4) for k, v in plain_stages.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

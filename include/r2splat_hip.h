@@ -306,10 +306,14 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
  * packed = capacity records of r2s_fused_record_bytes() each, tile_offsets
- * int32 [C, ceil(H/16), ceil(W/16)] and counters uint32 [2] persist from the
- * forward to the backward. counters[0] = intersections, counters[1] = 1 when
- * capacity was exceeded (that forward's outputs are then incomplete and the
- * call must be repeated with a larger capacity). grad_scratch: float [C*N*6]. */
+ * int32 [C, ceil(H/16), ceil(W/16)], record_offsets uint32 [C*N + 1] (first
+ * intersection of each (camera, primitive) in emission order) and counters
+ * uint32 [2] persist from the forward to the backward. counters[0] =
+ * intersections, counters[1] = 1 when capacity was exceeded (that forward's
+ * outputs are then incomplete and the call must be repeated with a larger
+ * capacity). isect_grads: float [capacity * 6], one VJP partial per
+ * intersection, summed per primitive in a fixed order (the backward is
+ * deterministic and uses no atomics). */
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity);
 size_t r2s_fused_record_bytes(void);
 
@@ -317,17 +321,42 @@ int r2s_fused_forward(const float* means, const float* quats, const float* scale
                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
                       uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                      int32_t* tile_offsets, uint32_t* counters, float* image,
-                      r2s_stream_t stream);
+                      int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                      float* image, r2s_stream_t stream);
 
 /* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N]
  * (overwritten; camera sums in a fixed order). */
 int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                        const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                        uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
-                       const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                       float* grad_scratch, const float* v_image, float* v_means, float* v_quats,
-                       float* v_scales, float* v_densities, r2s_stream_t stream);
+                       const void* packed, const int32_t* tile_offsets, const uint32_t* record_offsets,
+                       const uint32_t* counters, float* isect_grads, const float* v_image,
+                       float* v_means, float* v_quats, float* v_scales, float* v_densities,
+                       r2s_stream_t stream);
+
+/* The same two calls split into stages: forward = prepare (projection,
+ * binning, sort, offsets, record packing) + render (rasterization);
+ * backward = raster_vjp (rasterization VJP, one partial per intersection) +
+ * param_vjp (per-primitive sums, conic/mu, projection and parameter-transform
+ * VJPs, summed over cameras). */
+int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
+                      const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                      uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
+                      uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                      int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                      r2s_stream_t stream);
+int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                     uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, float* image,
+                     r2s_stream_t stream);
+int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                         uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+                         const float* v_image, float* isect_grads, r2s_stream_t stream);
+int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
+                        const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                        uint32_t N, uint32_t width, uint32_t height, float eps,
+                        const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
+                        const float* isect_grads, float* v_means, float* v_quats, float* v_scales,
+                        float* v_densities, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Losses                                                                    */

@@ -236,8 +236,10 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 
 // `n` is the element count, or the capacity when `d_n` (device count) is given.
+// A device count above the capacity means the emitted records are incomplete:
+// the sort then treats the input as empty (the caller re-runs with more room).
 __device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* d_n) {
-    return d_n ? min(*d_n, n) : n;
+    return d_n ? ((*d_n <= n) ? *d_n : 0u) : n;
 }
 
 __global__ void __launch_bounds__(kSortThreads)

@@ -5,9 +5,12 @@
 //           per (camera, primitive)) -> device scan -> record emission ->
 //           capacity-bounded stable radix sort -> tile offsets -> per-
 //           intersection record packing -> rasterization.
-// Backward: rasterization VJP into per-(camera, primitive) accumulators ->
-//           one lane per primitive summing the conic/mu, projection and
-//           parameter VJPs over cameras in a fixed order.
+// Backward: rasterization VJP, one partial per intersection stored at the
+//           intersection's emission index (the emission order groups each
+//           (camera, primitive)'s intersections contiguously) -> one lane per
+//           primitive summing its partials and the conic/mu, projection and
+//           parameter VJPs over cameras, all in a fixed order: the backward
+//           uses no atomics and is bitwise reproducible.
 // All sizes that depend on the intersection count are read on the device, so
 // the whole sequence can be captured into a hipGraph; the only host-visible
 // condition is capacity overflow (reported in a device flag).
@@ -33,6 +36,13 @@ constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
 constexpr uint32_t kT = 16;                          // tile edge of the fused path
 constexpr int kRasterWaves = 4;                      // waves (tiles) per workgroup
+
+// Live intersection count; an overflowed forward (count > capacity) has an
+// incomplete record list, so every consumer treats it as empty.
+__device__ __forceinline__ uint32_t live(const uint32_t* __restrict__ d_n, uint32_t capacity) {
+    const uint32_t n = *d_n;
+    return n <= capacity ? n : 0u;
+}
 
 // ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
@@ -120,7 +130,7 @@ fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restri
                      uint32_t n_keys, int32_t* __restrict__ offsets) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_keys) return;
-    const uint32_t n = min(*d_n, capacity);
+    const uint32_t n = live(d_n, capacity);
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -129,12 +139,25 @@ fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restri
     offsets[k] = int32_t(lo);
 }
 
+// packed[i] = record of sorted intersection i, with g = its emission index
+// cum[v] + (ty - y0) (x1 - x0) + (tx - x0) (row-major within the record's
+// tile rectangle, the order fused_emit_kernel wrote it in).
 __global__ void __launch_bounds__(256)
-fused_pack_kernel(const Splat* __restrict__ recs, const uint32_t* __restrict__ vals,
+fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects,
+                  const uint32_t* __restrict__ cum, uint32_t n_recs, uint32_t N, uint32_t ntx, uint32_t nt,
+                  const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                   const uint32_t* __restrict__ d_n, uint32_t capacity, Splat* __restrict__ packed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= min(*d_n, capacity)) return;
-    packed[i] = recs[vals[i]];
+    if (i >= live(d_n, capacity)) return;
+    const uint32_t v = vals[i];
+    if (v >= n_recs) return;
+    const uint32_t t = keys[i] - (v / N) * nt;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    const uint2 r = rects[v];
+    const uint32_t x0 = r.x & 0xffffu, x1 = r.x >> 16, y0 = r.y & 0xffffu;
+    Splat rec = recs[v];
+    rec.g = int32_t(cum[v] + (ty - y0) * (x1 - x0) + (tx - x0));
+    packed[i] = rec;
 }
 
 // Tile -> [start, end) of its packed list. Every quantity is wave-uniform.
@@ -159,7 +182,7 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = blockIdx.x * kRasterWaves + wave;
     if (tile >= n_tiles_total) return;
-    const uint32_t n_live = min(*d_n, capacity);
+    const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
@@ -202,18 +225,18 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 //   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
 // with dx = px - mx, and the VJP is
 //   v_mean = w (a Mx + b My, b Mx + c My), v_conic = -w (Mxx/2, Mxy, Myy/2),
-//   v_w = M0, accumulated into grad6[cam*N + n] with one atomic per field.
+//   v_w = M0, stored (no atomics) as the partial of emission index g.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kRasterWaves * kWave)
 fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
                         uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
-                        const float* __restrict__ v_img, float* __restrict__ grad6) {
+                        const float* __restrict__ v_img, float* __restrict__ isect_grads) {
     __shared__ __align__(16) float s_v[kRasterWaves][kT * kT];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = blockIdx.x * kRasterWaves + wave;
     if (tile >= n_tiles_total) return;
-    const uint32_t n_live = min(*d_n, capacity);
+    const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
@@ -269,30 +292,31 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
             mxy = fmaf(dy, r1, mxy);
             myy = fmaf(dy * dy, r0, myy);
         }
-        if (active) {
+        if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
             // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
             const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
-            float* gp = grad6 + 6 * size_t(s.g);
-            atomic_add(gp + 0, s.w * (a * mx + b * my));
-            atomic_add(gp + 1, s.w * (b * mx + c * my));
-            atomic_add(gp + 2, -0.5f * s.w * mxx);
-            atomic_add(gp + 3, -s.w * mxy);
-            atomic_add(gp + 4, -0.5f * s.w * myy);
-            atomic_add(gp + 5, m0);
+            float2* gp = reinterpret_cast<float2*>(isect_grads + 6 * size_t(s.g));
+            gp[0] = make_float2(s.w * (a * mx + b * my), s.w * (b * mx + c * my));
+            gp[1] = make_float2(-0.5f * s.w * mxx, -s.w * mxy);
+            gp[2] = make_float2(-0.5f * s.w * myy, m0);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Per-primitive VJP chain, summed over cameras in order (deterministic):
-// conic/mu VJP (double) -> projection VJP -> parameter-transform VJP.
-// grad6[cam*N + n] = (v_mx, v_my, v_a, v_b, v_c, v_w) with w = density * mu.
+// Per-primitive VJP chain, summed over cameras in order (deterministic): the
+// (camera, primitive)'s intersection partials [cum[v], cum[v+1]) summed in
+// emission order -> conic/mu VJP (double) -> projection VJP -> parameter-
+// transform VJP. A partial is (v_mx, v_my, v_a, v_b, v_c, v_w) with
+// w = density * mu.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                            const float* __restrict__ scales, const float* __restrict__ dens,
                            const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
-                           uint32_t H, float eps, uint32_t C, uint32_t N, const float* __restrict__ grad6,
+                           uint32_t H, float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
+                           const uint32_t* __restrict__ d_n, uint32_t capacity,
+                           const float* __restrict__ isect_grads,
                            float* __restrict__ v_means, float* __restrict__ v_quats,
                            float* __restrict__ v_scales, float* __restrict__ v_dens) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,9 +330,16 @@ fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restr
     const float opac = dens[n];
     float vp[3] = {0.f, 0.f, 0.f}, vcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vd = 0.f;
     const float Wf = float(W), Hf = float(H);
+    const uint32_t n_live = live(d_n, capacity);
     for (uint32_t c = 0; c < C; ++c) {
-        const float* g = grad6 + 6 * (size_t(c) * N + n);
-        const float g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5];
+        const size_t v = size_t(c) * N + n;
+        const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
+        for (uint32_t j = lo; j < hi; ++j) {
+            const float2* gp = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(j));
+            const float2 a = gp[0], b = gp[1], d = gp[2];
+            g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y; g4 += d.x; g5 += d.y;
+        }
         if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f) continue;
         Camera<float> cam;
         cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
@@ -348,7 +379,7 @@ fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restr
 // Forward scratch layout (released by the caller after the forward)
 // ---------------------------------------------------------------------------
 struct FusedScratch {
-    size_t recs, rects, cum, scan, sort, sort_bytes, keys, vals, bytes;
+    size_t recs, rects, scan, sort, sort_bytes, keys, vals, bytes;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -359,7 +390,6 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t capacity) {
     size_t o = 0;
     L.recs = o; o += al(CN * sizeof(Splat));
     L.rects = o; o += al(CN * sizeof(uint2));
-    L.cum = o; o += al((CN + 1) * 4);
     L.scan = o; o += al(scan_workspace_bytes(CN));
     L.sort_bytes = sort_workspace_bytes(capacity);
     L.sort = o; o += al(L.sort_bytes);
@@ -382,22 +412,27 @@ extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capac
 
 extern "C" size_t r2s_fused_record_bytes(void) { return sizeof(Splat); }
 
-extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
-                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
-                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                                 int32_t* tile_offsets, uint32_t* counters, float* image,
-                                 r2s_stream_t stream) {
-    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
-    const FusedScratch L = fused_scratch(C, N, capacity);
-    R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
+static int fused_check(uint32_t C, uint32_t N, uint32_t ntx, uint32_t nty, uint32_t capacity) {
     R2S_REQUIRE(uint64_t(C) * N < (1ull << 31), "C*N exceeds the int32 index range");
     R2S_REQUIRE(uint64_t(C) * ntx * nty < (1ull << 32), "C*n_tiles exceeds the key range");
     R2S_REQUIRE(ntx < 65536 && nty < 65536, "tile grid too large");
     R2S_REQUIRE(capacity > 0 && capacity < (1u << 31), "capacity must be in (0, 2^31)");
+    return R2S_OK;
+}
+
+extern "C" int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
+                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
+                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                                 int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                                 r2s_stream_t stream) {
+    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
+    if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
+    const FusedScratch L = fused_scratch(C, N, capacity);
+    R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
-    uint32_t* cum = at<uint32_t>(scratch, L.cum);
+    uint32_t* cum = record_offsets;
     if (CN > 0) {
         dim3 grid(ceil_div(N, 256), C);
         fused_project_kernel<<<grid, 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks, width,
@@ -421,31 +456,73 @@ extern "C" int r2s_fused_forward(const float* means, const float* quats, const f
     fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, counters, capacity, n_tiles,
                                                                      tile_offsets);
     if (int rc = check_launch("fused_offsets")) return rc;
-    fused_pack_kernel<<<ceil_div(capacity, 256), 256, 0, stream>>>(at<Splat>(scratch, L.recs), vals, counters,
-                                                                   capacity, static_cast<Splat*>(packed));
-    if (int rc = check_launch("fused_pack")) return rc;
+    fused_pack_kernel<<<ceil_div(capacity, 256), 256, 0, stream>>>(
+        at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), cum, CN, N, ntx, ntx * nty, keys, vals, counters,
+        capacity, static_cast<Splat*>(packed));
+    return check_launch("fused_pack");
+}
+
+extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                                uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, float* image,
+                                r2s_stream_t stream) {
+    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT, n_tiles = C * ntx * nty;
+    if (n_tiles == 0) return R2S_OK;
     fused_raster_fwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
         static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
         image);
     return check_launch("fused_raster_fwd");
 }
 
+extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                                    uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+                                    const float* v_image, float* isect_grads, r2s_stream_t stream) {
+    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
+    const uint32_t n_tiles = C * ntx * nty;
+    if (n_tiles == 0) return R2S_OK;
+    fused_raster_bwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
+        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
+        v_image, isect_grads);
+    return check_launch("fused_raster_bwd");
+}
+
+extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
+                                   const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                   uint32_t N, uint32_t width, uint32_t height, float eps,
+                                   const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
+                                   const float* isect_grads, float* v_means, float* v_quats, float* v_scales,
+                                   float* v_densities, r2s_stream_t stream) {
+    if (N == 0) return R2S_OK;
+    if (int rc = fused_check(C, N, 1, 1, capacity)) return rc;
+    fused_primitive_bwd_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
+        means, quats, scales, densities, viewmats, Ks, width, height, eps, C, N, record_offsets, counters, capacity,
+        isect_grads, v_means, v_quats, v_scales, v_densities);
+    return check_launch("fused_primitive_bwd");
+}
+
+extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
+                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
+                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                                 int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                                 float* image, r2s_stream_t stream) {
+    if (int rc = r2s_fused_prepare(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
+                                   radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
+                                   record_offsets, counters, stream))
+        return rc;
+    return r2s_fused_render(packed, tile_offsets, counters, capacity, C, width, height, image, stream);
+}
+
 extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                                   const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                                   uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
-                                  const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                                  float* grad_scratch, const float* v_image, float* v_means, float* v_quats,
-                                  float* v_scales, float* v_densities, r2s_stream_t stream) {
-    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
-    const uint32_t CN = C * N, n_tiles = C * ntx * nty;
-    if (int rc = fill_zero<float>(grad_scratch, size_t(CN) * 6, stream)) return rc;
-    fused_raster_bwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
-        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
-        v_image, grad_scratch);
-    if (int rc = check_launch("fused_raster_bwd")) return rc;
-    if (N == 0) return R2S_OK;
-    fused_primitive_bwd_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks,
-                                                                     width, height, eps, C, N, grad_scratch,
-                                                                     v_means, v_quats, v_scales, v_densities);
-    return check_launch("fused_primitive_bwd");
+                                  const void* packed, const int32_t* tile_offsets, const uint32_t* record_offsets,
+                                  const uint32_t* counters, float* isect_grads, const float* v_image,
+                                  float* v_means, float* v_quats, float* v_scales, float* v_densities,
+                                  r2s_stream_t stream) {
+    if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, counters, capacity, C, width, height, v_image,
+                                      isect_grads, stream))
+        return rc;
+    return r2s_fused_param_vjp(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
+                               record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,
+                               v_densities, stream);
 }

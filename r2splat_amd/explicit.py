@@ -168,9 +168,7 @@ def rasterize_fwd(
         image = means.new_zeros(shape)
         output = image if uncertainty is None else (image, image.clone())
     elif _use_fused(means, beta, uncertainty):
-        with stage("rasterize_fwd"):
-            output, fwd = fused.forward(means, quats, scales, densities, viewmats, Ks, width, height, eps,
-                                        radius_clip)
+        output, fwd = fused.forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip)
         buffers["fused"] = fwd
     else:
         T = RASTER_TILE
@@ -220,9 +218,8 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
         v_image = means.new_zeros(context.shape)
     b = context.buffers
     if "fused" in b:
-        with stage("rasterize_bwd"):
-            v_means, v_quats, v_scales, v_dens = fused.backward(b["fused"], means, quats, scales, densities,
-                                                                b["viewmats"], b["Ks"], v_image)
+        v_means, v_quats, v_scales, v_dens = fused.backward(b["fused"], means, quats, scales, densities,
+                                                            b["viewmats"], b["Ks"], v_image)
         return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_dens))
     _, height, width = context.shape
     T = RASTER_TILE

@@ -1,7 +1,9 @@
 """Fused float32 Gaussian rasterization (16x16 tiles) on the HIP library.
 
 ``FusedForward`` holds what the backward needs (per-intersection records,
-tile offsets, device counters) for one forward call. Capacity for the
+tile offsets, per-(camera, primitive) intersection offsets, device counters)
+for one forward call. The backward stores one partial per intersection and
+sums them per primitive in a fixed order, so it is bitwise reproducible. Capacity for the
 intersection list is sized from the previous call with the same shapes
 (``CapacityCache``) and grown on overflow, so steady-state calls perform no
 host-visible sizing besides the overflow check.
@@ -16,6 +18,7 @@ import torch
 
 from .hip import _backend as be
 from .hip._backend import call, ptr
+from .profiling import stage
 
 TILE = 16
 
@@ -47,6 +50,7 @@ CAPACITY = CapacityCache()
 class FusedForward:
     packed: torch.Tensor
     offsets: torch.Tensor
+    record_offsets: torch.Tensor
     counters: torch.Tensor
     capacity: int
     n_isects: int
@@ -71,38 +75,48 @@ def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     rec = be.lib().r2s_fused_record_bytes()
     image = torch.empty((C, height, width), dtype=torch.float32, device=device)
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
+    record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
     counters = torch.zeros((2,), dtype=torch.int32, device=device)
     with torch.cuda.device(device):
         stream = be.stream_of(device)
+        n_isects = -1
         while True:
             scratch = _scratch(C, N, cap, device)
             packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
-            call("r2s_fused_forward", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats), ptr(Ks),
-                 C, N, width, height, float(eps), float(radius_clip), cap, ptr(scratch), scratch.numel(),
-                 ptr(packed), ptr(offsets), ptr(counters), ptr(image), stream)
+            with stage("project_bin"):
+                call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
+                     ptr(Ks), C, N, width, height, float(eps), float(radius_clip), cap, ptr(scratch),
+                     scratch.numel(), ptr(packed), ptr(offsets), ptr(record_offsets), ptr(counters), stream)
             del scratch
             if not check:
-                return image, FusedForward(packed, offsets, counters, cap, -1, width, height, float(eps))
+                break
             n_isects, overflow = (int(v) for v in counters.tolist())
             if not overflow:
+                CAPACITY.settle(key, n_isects)
                 break
             cap = CAPACITY.grow(key, n_isects)
-    CAPACITY.settle(key, n_isects)
-    return image, FusedForward(packed, offsets, counters, cap, n_isects, width, height, float(eps))
+        with stage("rasterize_fwd"):
+            call("r2s_fused_render", ptr(packed), ptr(offsets), ptr(counters), cap, C, width, height, ptr(image),
+                 stream)
+    return image, FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps))
 
 
 def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image):
     """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities)."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
-    grad = torch.empty((C * N * 6,), dtype=torch.float32, device=device)
+    grad = torch.empty((fwd.capacity * 6,), dtype=torch.float32, device=device)
     v_means = torch.empty_like(means)
     v_quats = torch.empty_like(quats)
     v_scales = torch.empty_like(scales)
     v_dens = torch.empty_like(densities)
     with torch.cuda.device(device):
-        call("r2s_fused_backward", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats), ptr(Ks),
-             C, N, fwd.width, fwd.height, fwd.eps, fwd.capacity, ptr(fwd.packed), ptr(fwd.offsets),
-             ptr(fwd.counters), ptr(grad), ptr(v_image), ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens),
-             be.stream_of(device))
+        stream = be.stream_of(device)
+        with stage("rasterize_bwd"):
+            call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
+                 fwd.width, fwd.height, ptr(v_image), ptr(grad), stream)
+        with stage("param_bwd"):
+            call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
+                 ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps, ptr(fwd.record_offsets), ptr(fwd.counters),
+                 fwd.capacity, ptr(grad), ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens), stream)
     return v_means, v_quats, v_scales, v_dens
