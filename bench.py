@@ -1,9 +1,11 @@
 """Benchmark: fwd+bwd iterations/s of the CT splatting path on one MI355X.
 
 Workload (one step): N = 100,000 Gaussians, 64 views of 256 x 256 (tile 16),
-float32; rasterize forward -> MSE adjoint 2 (img - target) / numel against a
-target rendered from a second Gaussian set -> rasterize backward to physical
-parameter gradients. Inputs are resident in HBM before timing starts.
+float32, through the torch.autograd.Function wrapper: rasterize forward -> MSE
+against a target rendered from a second Gaussian set -> backward to the
+physical parameter gradients. Inputs are resident in HBM before timing starts.
+Voxelization of the same Gaussians to 128^3 (extent 1, tile 8) is timed
+separately (fwd + bwd of a fixed random adjoint).
 
 Prints one JSON line (see DESIGN.md for the field definitions), including the
 roofline of the dominant kernel measured with HIP events on its launch stream
@@ -25,6 +27,12 @@ sys.path.insert(0, ROOT)
 METRIC = "fwd+bwd iters/s (100k Gauss, 64×256² views) vs plain path; p99 fit latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# Reference-equivalent FP32 operations per pixel-primitive pair, counted from
+# the reference's per-pair arithmetic with an exp as one operation: forward 16
+# (delta 2, sigma 9, negate+exp 2, opacity*mu*e 2, accumulate 1;
+# RasterizeToPixelsFwd.cu:160-179), backward 46 (sigma recompute 13, alpha 2,
+# v_sigma 2, v_xy 12, v_conic 11, v_opacity 3, v_mu 3;
+# RasterizeToPixelsBwd.cu:430-468).
 FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
 
 
@@ -35,8 +43,10 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--path", default="explicit", choices=["explicit"])
     p.add_argument("--plain-steps", type=int, default=5, help="timed steps of the staged operator path")
-    p.add_argument("--cpu-views", type=int, default=16, help="views in the CPU-baseline sample")
+    p.add_argument("--cpu-views", type=int, default=64, help="views in the CPU-baseline sample")
+    p.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline sample time")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
     return p.parse_args()
 
 
@@ -58,10 +68,14 @@ def main():
         target, _ = r2splat_amd.rasterize_fwd(*tgt_scene.params(), tgt_scene.viewmats, tgt_scene.Ks, sc.image_size)
     del tgt_scene
 
+    from r2splat_amd import autograd as ag
+
+    params = [t.detach().requires_grad_() for t in sc.params()]
+
     def step():
-        img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
-        v = (2.0 / img.numel()) * (img - target)
-        return r2splat_amd.rasterize_bwd(ctx, v), ctx
+        img = ag.rasterize(*params, sc.viewmats, sc.Ks, sc.image_size, recompute=False)
+        loss = ((img - target) ** 2).mean()
+        return torch.autograd.grad(loss, params)
 
     def timed(backend, steps, warmup):
         explicit.BACKEND = backend
@@ -81,7 +95,7 @@ def main():
 
     from r2splat_amd import explicit
 
-    plain_ms, plain_stages = timed("ops", args.plain_steps, 1)
+    plain = timed("ops", args.plain_steps, 1) if args.plain_steps > 0 else None
     ms, stages = timed("fused", args.steps, args.warmup)
 
     # intersections of this workload (fixed across steps: the parameters do not change)
@@ -103,9 +117,11 @@ def main():
                 "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
                 "avg_launch_ms": round(dom_ms, 4)}
 
+    voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
+
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sc, target, args.cpu_views)
+        cpu = cpu_baseline(sc, target, args.cpu_views, args.cpu_seconds)
 
     result = {
         "metric": METRIC,
@@ -124,18 +140,50 @@ def main():
                                "through rasterize_fwd/rasterize_bwd",
                    "path": args.path, "n_isects": n_isects},
         "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
-        "plain_path": {"value": round(1e3 / plain_ms, 3), "unit": "iters/s", "ms_per_step": round(plain_ms, 3),
-                       "speedup_of_value": round(plain_ms / ms, 3),
-                       "description": "same step through one HIP operator per stage (staged reference op "
-                                      "sequence, host-side intersection sizing)",
-                       "stages_ms": {k: round(v[0], 4) for k, v in plain_stages.items()}},
+        "plain_path": None if plain is None else {
+            "value": round(1e3 / plain[0], 3), "unit": "iters/s", "ms_per_step": round(plain[0], 3),
+            "speedup_of_value": round(plain[0] / ms, 3),
+            "description": "same step through one HIP operator per stage (staged reference op sequence, "
+                           "host-side intersection sizing)",
+            "stages_ms": {k: round(v[0], 4) for k, v in plain[1].items()}},
+        "voxelize_128": voxel,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(sc, target, views):
+def voxel_timing(sc, steps):
+    """voxelize fwd+bwd of the same Gaussians to 128^3, extent (1, 1, 1), centre 0."""
+    import torch
+
+    from r2splat_amd import autograd as ag
+    from r2splat_amd import profiling
+
+    params = [t.detach().requires_grad_() for t in sc.params()]
+    shape = (128, 128, 128)
+    w = torch.randn(shape, generator=torch.Generator(device=sc.means.device).manual_seed(3),
+                    device=sc.means.device)
+
+    def vstep():
+        vol = ag.voxelize(*params, shape, volume_extent=(1.0, 1.0, 1.0), recompute=False)
+        return torch.autograd.grad(vol, params, w)
+
+    vstep()
+    torch.cuda.synchronize()
+    profiling.reset()
+    profiling.enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        vstep()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    profiling.enable(False)
+    return {"ms_per_step": round(ms, 3), "iters_per_s": round(1e3 / ms, 3),
+            "stages_ms": {k: round(v[0], 4) for k, v in profiling.collect().items()}}
+
+
+def cpu_baseline(sc, target, views, min_seconds):
     import numpy as np
 
     from oracle import cpu_baseline as cb
@@ -147,13 +195,17 @@ def cpu_baseline(sc, target, views):
     vm = sc.viewmats[:views].cpu().numpy()
     K = sc.Ks[:views].cpu().numpy()
     tg = target[:views].cpu().numpy().astype(np.float32)
-    t0 = time.perf_counter()
-    cb.iteration(*params, vm, K, sc.height, sc.width, tg, threads=threads)
-    dt = time.perf_counter() - t0
-    per_iter = dt * C / views
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        cb.iteration(*params, vm, K, sc.height, sc.width, tg, threads=threads)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    per_iter = dt / reps * C / views
     return {"value": round(1.0 / per_iter, 5), "unit": "iters/s", "cores": threads, "kind": "port",
-            "sample": f"{views} of {C} views x all {sc.means.shape[0]} Gaussians, one fwd+MSE+bwd iteration "
-                      f"({dt:.2f} s), scaled by {C}/{views}"}
+            "sample": f"{reps} fwd+MSE+bwd iterations over {views} of {C} views x all {sc.means.shape[0]} "
+                      f"Gaussians ({dt:.1f} s), scaled by {C}/{views}"}
 
 
 if __name__ == "__main__":

@@ -17,8 +17,12 @@
 //
 // Semantics are those of the operator kernels (raster.hip, geometry.hip) with
 // two precomputations per primitive: sigma is evaluated in log2 units
-// (A = a/2 log2 e, B = b log2 e, C = c/2 log2 e) and the contribution test
-// |w e^-sigma| >= 1e-5 becomes 0 <= sigma log2 e <= S with S = log2(|w| 1e5).
+// (A = a/2 log2 e, B = b log2 e, C = c/2 log2 e, q = sigma log2 e) and the
+// contribution test "sigma >= 0 and |w e^-sigma| >= 1e-5" becomes
+// 0 <= q <= S with S = log2(|w| 1e5). Non-negative floats order like their
+// bit patterns and every negative float or NaN has a larger one, so the test
+// is the single unsigned compare bits(q) < qlim with qlim = bits(S) + 1, and
+// qlim = 0 (nothing passes) when S < 0, w = 0 or S is NaN.
 #include "binning.hpp"
 #include "common.hpp"
 #include "geometry_common.hpp"
@@ -27,7 +31,8 @@
 namespace r2s {
 
 struct __align__(16) Splat {
-    float mx, my, A, B, C, w, S;
+    float mx, my, A, B, C, w;
+    uint32_t qlim;  // bits(S) + 1, or 0 when the primitive contributes nowhere
     int32_t g;
 };
 static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
@@ -35,7 +40,15 @@ static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
 constexpr uint32_t kT = 16;                          // tile edge of the fused path
-constexpr int kRasterWaves = 4;                      // waves (tiles) per workgroup
+constexpr int kFwdWaves = 8;                         // waves sharing one tile (forward)
+constexpr int kBwdWaves = 8;                         // waves sharing one tile (backward)
+
+__device__ __forceinline__ uint32_t qlim_of(float w) {
+    const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
+    return (w != 0.f && S >= 0.f) ? __float_as_uint(S) + 1u : 0u;
+}
+
+__device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
 
 // Live intersection count; an overflowed forward (count > capacity) has an
 // incomplete record list, so every consumer treats it as empty.
@@ -65,7 +78,7 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     covariance_from(R, s, cov);
     const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
-    Splat rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1.f, int32_t(idx)};
+    Splat rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u, int32_t(idx)};
     uint2 rect = make_uint2(0, 0);
     uint32_t count = 0;
     ProjState<float> st;
@@ -88,7 +101,7 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
         rec.B = float(P[1]) * kLog2e;
         rec.C = 0.5f * float(P[2]) * kLog2e;
         rec.w = w;
-        rec.S = (w != 0.f) ? __log2f(fabsf(w)) + kLog2Inv1e5 : -1.f;
+        rec.qlim = qlim_of(w);
         uint32_t x0, x1, y0, y1;
         if (rx > 0 && ry > 0 && tile_span(mx, rx, kT, ntx, x0, x1) && tile_span(my, ry, kT, nty, y0, y1)) {
             count = (x1 - x0) * (y1 - y0);
@@ -171,20 +184,53 @@ __device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, 
 }
 
 // ---------------------------------------------------------------------------
-// Forward: one wave per 16x16 tile; lane = column (lane % 16) and row group
-// (lane / 16), 4 rows per lane (r, r+4, r+8, r+12). Records are wave-uniform
-// (scalar loads); sigma reuses the column terms across the lane's 4 pixels.
+// Forward: one workgroup of kFwdWaves waves per 16x16 tile. Wave k takes the
+// k-th contiguous slice of the tile's list and walks it in batches of 64
+// records: each lane loads one record of the next batch (coalesced vector
+// loads issued a whole batch ahead, so HBM latency is hidden behind the
+// current batch) and the wave reads the current batch back from its own LDS
+// slot with broadcast loads. Lane = column (lane % 16) and row group
+// (lane / 16) with 4 rows per lane (r, r+4, r+8, r+12); sigma reuses the
+// column terms across the lane's 4 pixels. The slices' partial images are
+// summed in LDS in wave order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kRasterWaves * kWave)
+__device__ __forceinline__ void splat_accumulate(const Splat& s, float px, const float (&py)[4], float (&acc)[4]) {
+    const float dx = px - s.mx;
+    const float adx2 = s.A * dx * dx, bdx = s.B * dx;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float dy = py[k] - s.my;
+        const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);
+        const float e = __builtin_amdgcn_exp2f(-q);
+        acc[k] = fmaf(contributes(q, s.qlim) ? e : 0.f, s.w, acc[k]);
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ Splat load_splat(const Splat* __restrict__ packed, int32_t i, int32_t hi) {
+    Splat s = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u, -1};
+    if (i < hi) s = packed[i];
+    return s;
+}
+
+__global__ void __launch_bounds__(kFwdWaves * kWave)
 fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
                         uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H, float* __restrict__ out) {
+    __shared__ float part[kFwdWaves][kT * kT];
+    __shared__ Splat stage[kFwdWaves][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = blockIdx.x * kRasterWaves + wave;
-    if (tile >= n_tiles_total) return;
+    const uint32_t tile = blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    const int32_t per = (end - start + kFwdWaves - 1) / kFwdWaves;
+    const int32_t lo = min(end, start + int32_t(wave) * per), hi = min(end, lo + per);
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
     const uint32_t lane = threadIdx.x % kWave, col = lane % 16, row0 = lane / 16;
@@ -195,71 +241,74 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
         py[k] = float(ty * kT + row0 + 4 * k) + 0.5f;
         acc[k] = 0.f;
     }
-    for (int32_t i = start; i < end; ++i) {
-        const Splat s = packed[i];
-        const float dx = px - s.mx;
-        const float adx2 = s.A * dx * dx, bdx = s.B * dx;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float dy = py[k] - s.my;
-            const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);
-            const float e = __builtin_amdgcn_exp2f(-q);
-            acc[k] += (q >= 0.f && q <= s.S) ? s.w * e : 0.f;
-        }
+    Splat* slot = stage[wave];
+    Splat next = load_splat(packed, lo + int32_t(lane), hi);
+    for (int32_t base = lo; base < hi; base += kWave) {
+        wave_lds_sync();  // every lane is done reading the previous batch
+        slot[lane] = next;
+        wave_lds_sync();
+        next = load_splat(packed, base + kWave + int32_t(lane), hi);
+        const int32_t n = min(int32_t(kWave), hi - base);
+#pragma unroll 2
+        for (int32_t j = 0; j < n; ++j) splat_accumulate(slot[j], px, py, acc);
     }
-    const uint32_t x = tx * kT + col;
-    if (x < W) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t y = ty * kT + row0 + 4 * k;
-            if (y < H) out[(size_t(cam) * H + y) * W + x] = acc[k];
-        }
+    for (int k = 0; k < 4; ++k) part[wave][(row0 + 4 * k) * kT + col] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < kT * kT) {
+        const uint32_t p = threadIdx.x;
+        float v = 0.f;
+#pragma unroll
+        for (int k = 0; k < kFwdWaves; ++k) v += part[k][p];
+        const uint32_t x = tx * kT + p % kT, y = ty * kT + p / kT;
+        if (x < W && y < H) out[(size_t(cam) * H + y) * W + x] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Backward: one wave per tile, one lane per primitive of the tile's list
-// (chunks of 64). The tile's adjoint image sits in LDS and is read with
-// wave-uniform (broadcast) loads, row by row, so each lane sums its own
-// primitive's moments over the 256 pixels without any cross-lane reduction:
+// Backward: one workgroup of kBwdWaves waves per tile; wave k takes chunks
+// k, k + kBwdWaves, ... of 64 records, one lane per record. The tile's
+// adjoint image sits in LDS and is read with wave-uniform (broadcast) loads,
+// row by row, so each lane sums its own primitive's moments over the 256
+// pixels without any cross-lane reduction:
 //   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
-// with dx = px - mx, and the VJP is
+// with dx = px - mx (the moments are summed directly in dx: expanding them
+// around the tile corner cancels badly for primitives centred in the tile).
+// The VJP is
 //   v_mean = w (a Mx + b My, b Mx + c My), v_conic = -w (Mxx/2, Mxy, Myy/2),
 //   v_w = M0, stored (no atomics) as the partial of emission index g.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kRasterWaves * kWave)
+__global__ void __launch_bounds__(kBwdWaves * kWave)
 fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
                         uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
                         const float* __restrict__ v_img, float* __restrict__ isect_grads) {
-    __shared__ __align__(16) float s_v[kRasterWaves][kT * kT];
+    __shared__ __align__(16) float sv[kT * kT];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = blockIdx.x * kRasterWaves + wave;
-    if (tile >= n_tiles_total) return;
+    const uint32_t tile = blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    if (start == end) return;  // uniform over the workgroup
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
     const uint32_t lane = threadIdx.x % kWave;
-    float* sv = s_v[wave];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t p = lane + kWave * k, r = p / kT, cidx = p % kT;
-        const uint32_t x = tx * kT + cidx, y = ty * kT + r;
+    if (threadIdx.x < kT * kT) {
+        const uint32_t p = threadIdx.x, x = tx * kT + p % kT, y = ty * kT + p / kT;
         sv[p] = (x < W && y < H) ? v_img[(size_t(cam) * H + y) * W + x] : 0.f;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
     const float x0 = float(tx * kT) + 0.5f, y0 = float(ty * kT) + 0.5f;
-    for (int32_t chunk = start; chunk < end; chunk += kWave) {
+    constexpr int32_t kStride = kBwdWaves * kWave;
+    int32_t chunk = start + int32_t(wave) * kWave;
+    Splat next = load_splat(packed, chunk + int32_t(lane), end);
+    for (; chunk < end; chunk += kStride) {
         const int32_t i = chunk + int32_t(lane);
         const bool active = i < end;
-        Splat s = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1.f, -1};
-        if (active) s = packed[i];
+        const Splat s = next;
+        next = load_splat(packed, chunk + kStride + int32_t(lane), end);  // next chunk, a chunk ahead
         const float dx0 = x0 - s.mx;
-        float m0 = 0.f, mx = 0.f, my = 0.f, mxx = 0.f, mxy = 0.f, myy = 0.f;
+        float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f;
         for (uint32_t r = 0; r < kT; ++r) {
             const float dy = y0 + float(r) - s.my;
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
@@ -279,7 +328,7 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
                 const float dx = dx0 + float(j);
                 const float q = fmaf(dx, fmaf(s.A, dx, bdy), cdy2);
                 const float e = __builtin_amdgcn_exp2f(-q);
-                const float g = (q >= 0.f && q <= s.S) ? e * v[j] : 0.f;
+                const float g = contributes(q, s.qlim) ? e * v[j] : 0.f;
                 r0 += g;
                 const float gx = g * dx;
                 r1 += gx;
@@ -467,7 +516,7 @@ extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets,
                                 r2s_stream_t stream) {
     const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT, n_tiles = C * ntx * nty;
     if (n_tiles == 0) return R2S_OK;
-    fused_raster_fwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
+    fused_raster_fwd_kernel<<<n_tiles, kFwdWaves * kWave, 0, stream>>>(
         static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
         image);
     return check_launch("fused_raster_fwd");
@@ -479,7 +528,7 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
     const uint32_t n_tiles = C * ntx * nty;
     if (n_tiles == 0) return R2S_OK;
-    fused_raster_bwd_kernel<<<ceil_div(n_tiles, kRasterWaves), kRasterWaves * kWave, 0, stream>>>(
+    fused_raster_bwd_kernel<<<n_tiles, kBwdWaves * kWave, 0, stream>>>(
         static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
         v_image, isect_grads);
     return check_launch("fused_raster_bwd");

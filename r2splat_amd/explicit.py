@@ -118,6 +118,57 @@ def _consume(ctx, gradients):
 # ---------------------------------------------------------------------------
 # projection
 # ---------------------------------------------------------------------------
+_STATE_KEEP = ("viewmats", "Ks", "extent", "center")
+
+
+def _drop_state(buffers, eps, radius_clip):
+    for key in [k for k in buffers if k not in _STATE_KEEP]:
+        del buffers[key]
+    buffers["recompute"] = (eps, radius_clip)
+
+
+def _raster_state(prims, viewmats, Ks, width, height, eps, radius_clip) -> dict:
+    """Projection and tile binning: everything the rasterizer and its VJP read."""
+    means, quats, scales, densities, beta, uncertainty = prims
+    C = viewmats.shape[0]
+    if _use_fused(means, beta, uncertainty):
+        return {"fused": fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,
+                                       radius_clip)}
+    T = RASTER_TILE
+    with stage("project"):
+        covars, _ = ops.parameter_transform_fwd(quats, scales)
+        mapped, cov2d, radii = ops.world_to_image_projection_fwd(means, covars, viewmats, Ks, width, height, eps,
+                                                                 radius_clip)
+        conics, base_mus = ops.conics_mu_fwd(cov2d, eps)
+    opacity = densities.unsqueeze(0).expand(C, -1).contiguous()
+    mus, beta2d, ratio = base_mus, None, None
+    if beta is not None:
+        ratio = beta_path_ratio(beta)
+        mus = base_mus * ratio.unsqueeze(0)
+        beta2d = (beta + 0.5).unsqueeze(0).expand(C, -1).contiguous()
+    tiles_x, tiles_y = (width + T - 1) // T, (height + T - 1) // T
+    with stage("bin"):
+        _, isect_ids, flatten_ids = ops.isect_tiles(mapped, radii, T, tiles_x, tiles_y)
+        offsets = ops.isect_offset_encode(isect_ids, C, tiles_x, tiles_y)
+    return dict(covars=covars, mapped=mapped, cov2d=cov2d, conics=conics, base_mus=base_mus, mus=mus,
+                opacity=opacity, beta2d=beta2d, ratio=ratio, offsets=offsets, isect_ids=isect_ids,
+                flatten_ids=flatten_ids)
+
+
+def _raster_render(b, prims, shape):
+    means, _, _, _, beta, uncertainty = prims
+    C, height, width = shape
+    if "fused" in b:
+        return fused.render(b["fused"], C, means.device)
+    T = RASTER_TILE
+    prefix = "beta_" if beta is not None else ""
+    args = (b["mapped"], b["conics"], b["opacity"], b["mus"]) + (() if beta is None else (b["beta2d"],))
+    with stage("rasterize_fwd"):
+        if uncertainty is None:
+            return getattr(ops, prefix + "rasterize_to_pixels_fwd")(*args, b["offsets"], b["flatten_ids"], width,
+                                                                    height, T)
+        return getattr(ops, prefix + "rasterize_to_pixels_mean_var_fwd")(
+            *args, uncertainty, b["offsets"], b["flatten_ids"], width, height, T)
 @torch.no_grad()
 def rasterize_fwd(
     means: Tensor,
@@ -134,8 +185,15 @@ def rasterize_fwd(
     radius_clip: Optional[float] = None,
     detach_variance: bool = False,
     backward_mode: str = "primitive",
+    recompute: bool = False,
 ):
-    """Projects primitives to CT images [C, H, W] (or (mean, variance)) and a VJP context."""
+    """Projects primitives to CT images [C, H, W] (or (mean, variance)) and a VJP context.
+
+    ``recompute=True`` keeps only the inputs in the context and re-runs the
+    projection and tile binning in the backward instead of holding the
+    per-intersection lists between the two calls (for camera batches whose
+    intersection lists do not fit next to the rest of the working set).
+    """
     originals = list(zip(("means", "quats", "scales", "densities", "viewmats", "Ks", "beta", "uncertainty"),
                          (means, quats, scales, densities, viewmats, Ks, beta, uncertainty)))
     height, width = _shape("image_size", image_size, 2)
@@ -163,42 +221,14 @@ def rasterize_fwd(
         raise ValueError("viewmats must have homogeneous last row [0, 0, 0, 1]")
     shape = (C, height, width)
     buffers = {"viewmats": viewmats, "Ks": Ks}
-    N = means.shape[0]
-    if N == 0:
+    if means.shape[0] == 0:
         image = means.new_zeros(shape)
         output = image if uncertainty is None else (image, image.clone())
-    elif _use_fused(means, beta, uncertainty):
-        output, fwd = fused.forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip)
-        buffers["fused"] = fwd
     else:
-        T = RASTER_TILE
-        with stage("project"):
-            covars, _ = ops.parameter_transform_fwd(quats, scales)
-            mapped, cov2d, radii = ops.world_to_image_projection_fwd(means, covars, viewmats, Ks, width, height,
-                                                                     eps, radius_clip)
-            conics, base_mus = ops.conics_mu_fwd(cov2d, eps)
-        opacity = densities.unsqueeze(0).expand(C, -1).contiguous()
-        mus, beta2d, ratio = base_mus, None, None
-        if beta is not None:
-            ratio = beta_path_ratio(beta)
-            mus = base_mus * ratio.unsqueeze(0)
-            beta2d = (beta + 0.5).unsqueeze(0).expand(C, -1).contiguous()
-        tiles_x, tiles_y = (width + T - 1) // T, (height + T - 1) // T
-        with stage("bin"):
-            _, isect_ids, flatten_ids = ops.isect_tiles(mapped, radii, T, tiles_x, tiles_y)
-            offsets = ops.isect_offset_encode(isect_ids, C, tiles_x, tiles_y)
-        prefix = "beta_" if beta is not None else ""
-        args = (mapped, conics, opacity, mus) + (() if beta is None else (beta2d,))
-        with stage("rasterize_fwd"):
-            if uncertainty is None:
-                output = getattr(ops, prefix + "rasterize_to_pixels_fwd")(*args, offsets, flatten_ids, width,
-                                                                         height, T)
-            else:
-                output = getattr(ops, prefix + "rasterize_to_pixels_mean_var_fwd")(
-                    *args, uncertainty, offsets, flatten_ids, width, height, T)
-        buffers.update(covars=covars, mapped=mapped, cov2d=cov2d, conics=conics, base_mus=base_mus, mus=mus,
-                       opacity=opacity, beta2d=beta2d, ratio=ratio, offsets=offsets, isect_ids=isect_ids,
-                       flatten_ids=flatten_ids)
+        buffers.update(_raster_state(prims, viewmats, Ks, width, height, eps, radius_clip))
+        output = _raster_render(buffers, prims, shape)
+        if recompute:
+            _drop_state(buffers, eps, radius_clip)
     ctx = _make_context("rasterize", shape, prims, buffers, originals, eps, mode, detach_variance)
     return output, ctx
 
@@ -217,6 +247,9 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
     if v_image is None:
         v_image = means.new_zeros(context.shape)
     b = context.buffers
+    if "recompute" in b:
+        _, height, width = context.shape
+        b.update(_raster_state(context.primitives, b["viewmats"], b["Ks"], width, height, *b.pop("recompute")))
     if "fused" in b:
         v_means, v_quats, v_scales, v_dens = fused.backward(b["fused"], means, quats, scales, densities,
                                                             b["viewmats"], b["Ks"], v_image)
@@ -258,6 +291,37 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
 # ---------------------------------------------------------------------------
 # voxelization
 # ---------------------------------------------------------------------------
+def _voxel_state(prims, extent, center, shape, eps, radius_clip) -> dict:
+    """Volume mapping and 3D tile binning."""
+    means, quats, scales, densities, beta, uncertainty = prims
+    depth, height, width = shape
+    T = VOXEL_TILE
+    with stage("voxel_map"):
+        covars, max_scales = ops.parameter_transform_fwd(quats, scales)
+        mapped, conics, radii = ops.world_to_volume_mapping_fwd(means, covars, max_scales, extent, center, width,
+                                                                height, depth, eps, radius_clip)
+    tx, ty, tz = (width + T - 1) // T, (height + T - 1) // T, (depth + T - 1) // T
+    with stage("voxel_bin"):
+        _, isect_ids, flatten_ids = ops.isect_tiles_3d(mapped, radii, T, tx, ty, tz)
+        offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
+    return dict(covars=covars, mapped=mapped, conics=conics, offsets=offsets, isect_ids=isect_ids,
+                flatten_ids=flatten_ids)
+
+
+def _voxel_render(b, prims, shape):
+    means, quats, scales, densities, beta, uncertainty = prims
+    depth, height, width = shape
+    T = VOXEL_TILE
+    prefix = "beta_" if beta is not None else ""
+    args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
+    with stage("voxelize_fwd"):
+        if uncertainty is None:
+            return getattr(ops, prefix + "voxelize_to_voxels_fwd")(*args, b["offsets"], b["flatten_ids"], width,
+                                                                   height, depth, T)
+        return getattr(ops, prefix + "voxelize_to_voxels_mean_var_fwd")(
+            *args, uncertainty, b["offsets"], b["flatten_ids"], width, height, depth, T)
+
+
 @torch.no_grad()
 def voxelize_fwd(
     means: Tensor,
@@ -274,8 +338,13 @@ def voxelize_fwd(
     radius_clip: Optional[float] = None,
     detach_variance: bool = False,
     backward_mode: str = "primitive",
+    recompute: bool = False,
 ):
-    """Samples primitives at voxel centres: volume [D, H, W] (or (mean, variance)) and a VJP context."""
+    """Samples primitives at voxel centres: volume [D, H, W] (or (mean, variance)) and a VJP context.
+
+    ``recompute=True`` re-runs the volume mapping and 3D binning in the
+    backward instead of holding the intersection lists in the context.
+    """
     originals = list(zip(("means", "quats", "scales", "densities", "volume_extent", "volume_center", "beta",
                           "uncertainty"),
                          (means, quats, scales, densities, volume_extent, volume_center, beta, uncertainty)))
@@ -299,23 +368,10 @@ def voxelize_fwd(
         volume = means.new_zeros(shape)
         output = volume if uncertainty is None else (volume, volume.clone())
     else:
-        T = VOXEL_TILE
-        covars, max_scales = ops.parameter_transform_fwd(quats, scales)
-        mapped, conics, radii = ops.world_to_volume_mapping_fwd(means, covars, max_scales, extent, center, width,
-                                                                height, depth, eps, radius_clip)
-        tx, ty, tz = (width + T - 1) // T, (height + T - 1) // T, (depth + T - 1) // T
-        _, isect_ids, flatten_ids = ops.isect_tiles_3d(mapped, radii, T, tx, ty, tz)
-        offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
-        prefix = "beta_" if beta is not None else ""
-        args = (mapped, conics, densities) + (() if beta is None else (beta,))
-        if uncertainty is None:
-            output = getattr(ops, prefix + "voxelize_to_voxels_fwd")(*args, offsets, flatten_ids, width, height,
-                                                                     depth, T)
-        else:
-            output = getattr(ops, prefix + "voxelize_to_voxels_mean_var_fwd")(
-                *args, uncertainty, offsets, flatten_ids, width, height, depth, T)
-        buffers.update(covars=covars, mapped=mapped, conics=conics, offsets=offsets, isect_ids=isect_ids,
-                       flatten_ids=flatten_ids)
+        buffers.update(_voxel_state(prims, extent, center, shape, eps, radius_clip))
+        output = _voxel_render(buffers, prims, shape)
+        if recompute:
+            _drop_state(buffers, eps, radius_clip)
     ctx = _make_context("voxelize", shape, prims, buffers, originals, eps, mode, detach_variance)
     return output, ctx
 
@@ -334,29 +390,33 @@ def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Opti
     if v_volume is None:
         v_volume = means.new_zeros(context.shape)
     b = context.buffers
+    if "recompute" in b:
+        b.update(_voxel_state(context.primitives, b["extent"], b["center"], context.shape, *b.pop("recompute")))
     depth, height, width = context.shape
     T = VOXEL_TILE
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
-    if v_variance is not None:
-        grads = getattr(ops, prefix + "voxelize_to_voxels_mean_var_bwd_per_gaussian")(
-            *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, v_variance, uncertainty,
-            context.detach_variance, width, height, depth, T)
-        v_uncertainty = grads[-1]
-    else:
-        if context.backward_mode == "primitive":
-            grads = getattr(ops, prefix + "voxelize_to_voxels_bwd_per_gaussian")(
-                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, width, height, depth, T)
+    with stage("voxelize_bwd"):
+        if v_variance is not None:
+            grads = getattr(ops, prefix + "voxelize_to_voxels_mean_var_bwd_per_gaussian")(
+                *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, v_variance, uncertainty,
+                context.detach_variance, width, height, depth, T)
+            v_uncertainty = grads[-1]
         else:
-            grads = getattr(ops, prefix + "voxelize_to_voxels_bwd")(
-                *args, b["offsets"], b["flatten_ids"], v_volume, width, height, depth, T)
-        v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
+            if context.backward_mode == "primitive":
+                grads = getattr(ops, prefix + "voxelize_to_voxels_bwd_per_gaussian")(
+                    *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, width, height, depth, T)
+            else:
+                grads = getattr(ops, prefix + "voxelize_to_voxels_bwd")(
+                    *args, b["offsets"], b["flatten_ids"], v_volume, width, height, depth, T)
+            v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
     v_mapped, v_conics, v_densities = grads[:3]
     v_beta = None if beta is None else grads[3]
-    v_means, v_covars = ops.world_to_volume_mapping_bwd(means, b["covars"], b["extent"], b["center"], width, height,
-                                                        depth, context.eps, v_mapped.contiguous(),
-                                                        v_conics.contiguous())
-    v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
+    with stage("voxel_map_bwd"):
+        v_means, v_covars = ops.world_to_volume_mapping_bwd(means, b["covars"], b["extent"], b["center"], width,
+                                                            height, depth, context.eps, v_mapped.contiguous(),
+                                                            v_conics.contiguous())
+        v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
     return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
 
 

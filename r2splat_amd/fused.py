@@ -64,16 +64,15 @@ def _scratch(C, N, capacity, device):
     return torch.empty((nbytes,), dtype=torch.uint8, device=device)
 
 
-def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=True):
-    """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
+def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
+            check=True) -> FusedForward:
+    """Projection, binning, sort and record packing (everything but the image). float32, one device."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
     key = (device, C, N, height, width)
     cap = int(capacity or CAPACITY.get(key, C, N))
     ntx, nty = -(-width // TILE), -(-height // TILE)
     rec = be.lib().r2s_fused_record_bytes()
-    image = torch.empty((C, height, width), dtype=torch.float32, device=device)
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
     record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
     counters = torch.zeros((2,), dtype=torch.int32, device=device)
@@ -95,10 +94,25 @@ def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
                 CAPACITY.settle(key, n_isects)
                 break
             cap = CAPACITY.grow(key, n_isects)
+    return FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps))
+
+
+def render(fwd: FusedForward, C: int, device) -> torch.Tensor:
+    """The image [C, H, W] of a prepared forward."""
+    image = torch.empty((C, fwd.height, fwd.width), dtype=torch.float32, device=device)
+    with torch.cuda.device(device):
         with stage("rasterize_fwd"):
-            call("r2s_fused_render", ptr(packed), ptr(offsets), ptr(counters), cap, C, width, height, ptr(image),
-                 stream)
-    return image, FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps))
+            call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
+                 fwd.width, fwd.height, ptr(image), be.stream_of(device))
+    return image
+
+
+def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
+            check=True):
+    """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
+    fwd = prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip,
+                  capacity=capacity, check=check)
+    return render(fwd, viewmats.shape[0], means.device), fwd
 
 
 def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image):

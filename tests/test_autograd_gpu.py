@@ -1,0 +1,214 @@
+"""torch.autograd.Function wrappers: agreement with the explicit calls,
+recompute-in-backward equal to stored intersection lists, and FP64 central
+finite differences (directional; a direction whose step crosses a
+contribution cut-off is detected by comparing steps h and h/2 and skipped)."""
+
+import pytest
+import torch
+
+import r2splat_amd
+from r2splat_amd import autograd as ag
+from r2splat_amd import explicit, workloads
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
+
+
+def _leaves(sc, names):
+    out = []
+    for name in names:
+        t = getattr(sc, name)
+        out.append(None if t is None else t.detach().clone().requires_grad_())
+    return out
+
+
+def _central(loss_fn, params, direction, h):
+    with torch.no_grad():
+        for p, d in zip(params, direction):
+            p.add_(d, alpha=h)
+        fp = float(loss_fn())
+        for p, d in zip(params, direction):
+            p.sub_(d, alpha=2 * h)
+        fm = float(loss_fn())
+        for p, d in zip(params, direction):
+            p.add_(d, alpha=h)
+    return (fp - fm) / (2 * h)
+
+
+def _check_fd(loss_fn, params, grads, *, seed, n_dirs=4, h=1e-6, rtol=2e-5, jump_tol=1e-5, mask=None):
+    """Analytic directional derivatives <grad, d> against central differences for
+    random directions d (restricted to `mask` rows when given). A direction is
+    skipped when the central differences at h and h/2 disagree beyond their
+    O(h^2) truncation error: the step then crossed a contribution cut-off or a
+    tile-coverage boundary, where the loss jumps."""
+    gen = torch.Generator(device=params[0].device).manual_seed(seed)
+    checked = 0
+    for _ in range(3 * n_dirs):
+        direction = []
+        for p in params:
+            d = torch.randn(p.shape, generator=gen, device=p.device, dtype=p.dtype)
+            if mask is not None:
+                d = d * mask.reshape((-1,) + (1,) * (p.dim() - 1)).to(p.dtype)
+            direction.append(d)
+        analytic = sum(float((g * d).sum()) for g, d in zip(grads, direction))
+        c1 = _central(loss_fn, params, direction, h)
+        c2 = _central(loss_fn, params, direction, h / 2)
+        scale = max(abs(analytic), abs(c1), 1e-12)
+        if abs(c1 - c2) > jump_tol * scale + 1e-9:
+            continue
+        assert abs(analytic - c2) <= rtol * scale + 1e-9, (analytic, c2)
+        checked += 1
+        if checked == n_dirs:
+            return
+    pytest.fail(f"only {checked} of {n_dirs} directions were free of cut-off crossings")
+
+
+@pytest.mark.parametrize("kind", ["gauss", "beta", "beta_var", "gauss_var"])
+def test_rasterize_fd_fp64(device, kind):
+    beta = kind.startswith("beta")
+    var = kind.endswith("var")
+    sc = workloads.scene(24, 2, 32, 32, scale_range=(0.04, 0.12), seed=3, dtype=torch.float64, device=device,
+                         beta_range=(0.5, 4.0) if beta else None, uncertainty_max=0.1 if var else None)
+    names = [n for n in FIELDS if getattr(sc, n) is not None]
+    params = _leaves(sc, names)
+    kw = dict(zip(names, params))
+    gen = torch.Generator(device=device).manual_seed(11)
+    w = torch.rand((2, 32, 32), generator=gen, device=device, dtype=torch.float64)
+    wv = torch.rand((2, 32, 32), generator=gen, device=device, dtype=torch.float64)
+
+    def loss_of(out):
+        if var:
+            return (out[0] * w).sum() + (out[1] * wv).sum()
+        return (out * w).sum()
+
+    def run():
+        return ag.rasterize(kw["means"], kw["quats"], kw["scales"], kw["densities"], sc.viewmats, sc.Ks,
+                            (32, 32), beta=kw.get("beta"), uncertainty=kw.get("uncertainty"), radius_clip=5.0)
+
+    grads = torch.autograd.grad(loss_of(run()), params)
+    _check_fd(lambda: loss_of(run()), params, grads, seed=5)
+
+
+def test_voxelize_fd_fp64(device):
+    sc = workloads.scene(30, 1, 8, 8, scale_range=(0.05, 0.15), seed=8, dtype=torch.float64, device=device)
+    names = ["means", "quats", "scales", "densities"]
+    params = _leaves(sc, names)
+    w = torch.rand((12, 14, 16), generator=torch.Generator(device=device).manual_seed(2), device=device,
+                   dtype=torch.float64)
+
+    def run():
+        return ag.voxelize(*params, (12, 14, 16), volume_extent=(1.0, 1.1, 1.2), radius_clip=5.0)
+
+    grads = torch.autograd.grad((run() * w).sum(), params)
+    # a 3D footprint puts many voxels near the 1e-6 cut-off: a small step keeps crossings rare
+    _check_fd(lambda: (run() * w).sum(), params, grads, seed=6, h=1e-8)
+
+
+def test_config1_fd_on_32_primitive_subset_fp64(device):
+    """100k Gaussians x 64 views of 256^2 in float64 through the wrapper; MSE
+    against the seed-1 target; central FD along random directions supported on
+    32 random primitives (all 11 parameters each), and per parameter group.
+
+    The loss difference is contracted from the image difference,
+    L(+) - L(-) = mean((img+ - img-) (img+ + img- - 2 target)), so the tiny
+    change of the scene-wide loss is not lost to the rounding of the loss
+    itself; a small step keeps 1e-5 cut-off crossings rare (a direction with
+    one is detected by comparing h and h/2 and skipped)."""
+    sc = workloads.config1(device=device, dtype=torch.float64)
+    tgt = workloads.config1(device=device, dtype=torch.float64, seed=1)
+    with torch.no_grad():
+        target, _ = r2splat_amd.rasterize_fwd(*tgt.params(), tgt.viewmats, tgt.Ks, tgt.image_size)
+    del tgt
+    params = _leaves(sc, FIELDS[:4])
+
+    def render():
+        return ag.rasterize(*params, sc.viewmats, sc.Ks, sc.image_size)
+
+    img = render()
+    grads = torch.autograd.grad(((img - target) ** 2).mean(), params)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    idx = torch.randperm(sc.means.shape[0], generator=gen)[:32].to(device)
+    mask = torch.zeros(sc.means.shape[0], device=device, dtype=torch.float64)
+    mask[idx] = 1.0
+    dgen = torch.Generator(device=device).manual_seed(21)
+
+    def central(group, direction, h):
+        with torch.no_grad():
+            for p, d in zip(group, direction):
+                p.add_(d, alpha=h)
+            ip = render()
+            for p, d in zip(group, direction):
+                p.sub_(d, alpha=2 * h)
+            im = render()
+            for p, d in zip(group, direction):
+                p.add_(d, alpha=h)
+            return float(((ip - im) * (ip + im - 2 * target)).mean()) / (2 * h)
+
+    for group_ids, n_dirs in (((0, 1, 2, 3), 3), ((0,), 1), ((1,), 1), ((2,), 1), ((3,), 1)):
+        group = [params[k] for k in group_ids]
+        checked = 0
+        for _ in range(4 * n_dirs):
+            direction = [torch.randn(p.shape, generator=dgen, device=device, dtype=p.dtype)
+                         * mask.reshape((-1,) + (1,) * (p.dim() - 1)) for p in group]
+            analytic = sum(float((grads[k] * d).sum()) for k, d in zip(group_ids, direction))
+            c1, c2 = central(group, direction, 1e-8), central(group, direction, 5e-9)
+            scale = max(abs(analytic), abs(c1))
+            if abs(c1 - c2) > 1e-5 * scale:
+                continue  # the step crossed a cut-off
+            assert abs(analytic - c2) <= 1e-5 * scale, (group_ids, analytic, c2)
+            checked += 1
+            if checked == n_dirs:
+                break
+        assert checked == n_dirs, f"group {group_ids}: only {checked} directions free of cut-off crossings"
+
+
+def test_autograd_equals_explicit_fused_float32(device):
+    sc = workloads.config0(device=device)
+    img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+    v = (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
+    ref = r2splat_amd.rasterize_bwd(ctx, v)
+    params = _leaves(sc, FIELDS[:4])
+    out = ag.rasterize(*params, sc.viewmats, sc.Ks, sc.image_size)
+    assert torch.equal(out, img)
+    grads = torch.autograd.grad(out, params, v)
+    for name, g in zip(FIELDS, grads):
+        assert torch.equal(g, getattr(ref, name)), name
+
+
+@pytest.mark.parametrize("case", ["fused", "ops_beta_var", "voxel"])
+def test_recompute_in_backward_equals_stored_lists(device, case):
+    dtype = torch.float32 if case != "ops_beta_var" else torch.float64
+    sc = workloads.scene(800, 3, 48, 40, scale_range=(0.02, 0.1), seed=12, dtype=dtype, device=device,
+                         beta_range=(0.5, 4.0) if case == "ops_beta_var" else None,
+                         uncertainty_max=0.1 if case == "ops_beta_var" else None)
+    names = [n for n in FIELDS if getattr(sc, n) is not None]
+    results = []
+    for recompute in (False, True):
+        params = _leaves(sc, names)
+        kw = dict(zip(names, params))
+        if case == "voxel":
+            out = ag.voxelize(kw["means"], kw["quats"], kw["scales"], kw["densities"], (16, 20, 24),
+                              volume_extent=(1.0, 1.0, 1.0), recompute=recompute)
+        else:
+            out = ag.rasterize(kw["means"], kw["quats"], kw["scales"], kw["densities"], sc.viewmats, sc.Ks,
+                               sc.image_size, beta=kw.get("beta"), uncertainty=kw.get("uncertainty"),
+                               recompute=recompute)
+        outs = out if isinstance(out, tuple) else (out,)
+        gens = [torch.Generator(device=device).manual_seed(4 + k) for k in range(len(outs))]
+        vs = [torch.randn(o.shape, generator=g, device=device, dtype=dtype) for o, g in zip(outs, gens)]
+        grads = torch.autograd.grad(outs, params, vs)
+        results.append((outs, grads))
+    (o0, g0), (o1, g1) = results
+    for a, b in zip(o0, o1):
+        assert torch.equal(a, b)
+    for name, a, b in zip(names, g0, g1):
+        assert torch.equal(a, b), name
+
+
+def test_recompute_context_holds_no_intersection_lists(device):
+    sc = workloads.config0(device=device)
+    _, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, recompute=True)
+    assert set(ctx.buffers) == {"viewmats", "Ks", "recompute"}
+    assert ag._should_recompute(None, 1024, 1000, 4, sc.means.device)
+    assert not ag._should_recompute(None, 1 << 40, 1000, 4, sc.means.device)
