@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline sample time")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
+    p.add_argument("--no-graph", action="store_true", help="time the eager step instead of its hipGraph replay")
     return p.parse_args()
 
 
@@ -96,7 +97,8 @@ def main():
     from r2splat_amd import explicit
 
     plain = timed("ops", args.plain_steps, 1) if args.plain_steps > 0 else None
-    ms, stages = timed("fused", args.steps, args.warmup)
+    eager_ms, stages = timed("fused", args.steps, args.warmup)
+    ms = graph_timing(step, args.steps, args.warmup) if not args.no_graph else eager_ms
 
     # intersections of this workload (fixed across steps: the parameters do not change)
     from r2splat_amd import hip as ops
@@ -140,6 +142,8 @@ def main():
                                "through rasterize_fwd/rasterize_bwd",
                    "path": args.path, "n_isects": n_isects},
         "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
+        "eager_path": {"value": round(1e3 / eager_ms, 3), "unit": "iters/s", "ms_per_step": round(eager_ms, 3),
+                       "description": "the same fused step issued eagerly (one host overflow check per step)"},
         "plain_path": None if plain is None else {
             "value": round(1e3 / plain[0], 3), "unit": "iters/s", "ms_per_step": round(plain[0], 3),
             "speedup_of_value": round(plain[0] / ms, 3),
@@ -151,6 +155,37 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
+
+
+def graph_timing(step, steps, warmup):
+    """Captures one step (autograd wrapper, fused path) into a hipGraph and
+    times `steps` replays; the intersection capacity is the one sized by the
+    eager warm-up, and the overflow flag is checked after the timed region."""
+    import torch
+
+    from r2splat_amd import explicit, fused
+
+    explicit.BACKEND = "fused"
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(max(1, warmup)):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(max(1, warmup)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    if fused.LAST_COUNTERS is not None and int(fused.LAST_COUNTERS[1].item()) != 0:
+        raise RuntimeError("intersection capacity overflowed inside the captured step")
+    return ms
 
 
 def voxel_timing(sc, steps):

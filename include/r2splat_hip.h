@@ -294,7 +294,7 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
     void* v_uncertainties, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
-/* Fused float32 Gaussian path (16x16 tiles)                                 */
+/* Fused float32 Gaussian path (8x8, 16x16 or 32x32 tiles)                  */
 /* ------------------------------------------------------------------------ */
 /* One call runs the whole rasterize_fwd sequence of the reference
  * (r2splat/explicit.py:160-183: parameter transform, projection, conics/mu,
@@ -304,14 +304,19 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * sizes stay on the device: the sort covers min(counters[0], capacity)
  * records, so the pair can be captured into a hipGraph.
  *
+ * tile_size is 8, 16 or 32 (the reference's tile_size argument); waves is
+ * the number of wave64s sharing one tile in the raster kernels (4, 8 or 16;
+ * 0 = 8). The backward's results do not depend on waves; the forward's
+ * per-pixel summation order does (rounding only).
+ *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
  * packed = capacity records of r2s_fused_record_bytes() each, tile_offsets
- * int32 [C, ceil(H/16), ceil(W/16)], record_offsets uint32 [C*N + 1] (first
- * intersection of each (camera, primitive) in emission order) and counters
- * uint32 [2] persist from the forward to the backward. counters[0] =
- * intersections, counters[1] = 1 when capacity was exceeded (that forward's
- * outputs are then incomplete and the call must be repeated with a larger
- * capacity). isect_grads: float [capacity * 6], one VJP partial per
+ * int32 [C, ceil(H/tile), ceil(W/tile)], record_offsets uint32 [C*N + 1]
+ * (first intersection of each (camera, primitive) in emission order) and
+ * counters uint32 [2] persist from the forward to the backward. counters[0]
+ * = intersections, counters[1] = 1 when capacity was exceeded (that
+ * forward's outputs are then incomplete and the call must be repeated with
+ * a larger capacity). isect_grads: float [capacity * 6], one VJP partial per
  * intersection, summed per primitive in a fixed order (the backward is
  * deterministic and uses no atomics). */
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity);
@@ -319,20 +324,20 @@ size_t r2s_fused_record_bytes(void);
 
 int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                      uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
-                      uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                      int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                      uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                      float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                      void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
                       float* image, r2s_stream_t stream);
 
 /* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N]
  * (overwritten; camera sums in a fixed order). */
 int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                        const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                       uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
-                       const void* packed, const int32_t* tile_offsets, const uint32_t* record_offsets,
-                       const uint32_t* counters, float* isect_grads, const float* v_image,
-                       float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                       r2s_stream_t stream);
+                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                       uint32_t capacity, const void* packed, const int32_t* tile_offsets,
+                       const uint32_t* record_offsets, const uint32_t* counters, float* isect_grads,
+                       const float* v_image, float* v_means, float* v_quats, float* v_scales,
+                       float* v_densities, r2s_stream_t stream);
 
 /* The same two calls split into stages: forward = prepare (projection,
  * binning, sort, offsets, record packing) + render (rasterization);
@@ -341,22 +346,46 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
  * VJPs, summed over cameras). */
 int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                      uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
-                      uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                      int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                      uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                      float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                      void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
                       r2s_stream_t stream);
 int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                     uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, float* image,
-                     r2s_stream_t stream);
+                     uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
+                     uint32_t waves, float* image, r2s_stream_t stream);
 int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
                          uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                         const float* v_image, float* isect_grads, r2s_stream_t stream);
+                         uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
+                         r2s_stream_t stream);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                         const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                         uint32_t N, uint32_t width, uint32_t height, float eps,
                         const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
                         const float* isect_grads, float* v_means, float* v_quats, float* v_scales,
                         float* v_densities, r2s_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Training-step kernels (no counterpart in the reference, which leaves image */
+/* losses beyond L1 and the optimizer to the caller; used by graph-captured   */
+/* fits that must not depend on host-side scalars)                            */
+/* ------------------------------------------------------------------------ */
+/* Number of per-block loss partials r2s_mse_adjoint writes for n pixels. */
+size_t r2s_mse_partials(uint64_t n);
+
+/* v_image = 2 (image - target) / n; when partials (r2s_mse_partials(n)
+ * floats) and loss are given, loss[0] = mean((image - target)^2) summed in a
+ * fixed order. float32. */
+int r2s_mse_adjoint(const float* image, const float* target, uint64_t n, float* v_image,
+                    float* partials, float* loss, r2s_stream_t stream);
+
+/* Adam over a flat float32 buffer (torch.optim.Adam semantics, no weight
+ * decay). state uint32 [2]: applied steps (bias correction uses state[0]+1)
+ * and skipped steps. When skip_flag is non-null and *skip_flag != 0 (the
+ * fused forward's overflow flag, counters + 1) the update is skipped and
+ * state[1] is incremented instead. */
+int r2s_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq, uint64_t n,
+                  float lr, float beta1, float beta2, float eps, uint32_t* state,
+                  const uint32_t* skip_flag, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Losses                                                                    */

@@ -2,8 +2,9 @@
 
 Errors raised here happen before any kernel launch: ValueError for bad values
 and shapes, TypeError for bad types. Value checks read tensor contents, which
-synchronises with the GPU; the batched/graph path in ``r2splat_amd.batched``
-validates once per job instead of once per call.
+synchronises with the GPU; during hipGraph capture (where a host read is not
+allowed) only the shape, dtype and device checks run, so a captured call must
+have been validated by an eager call with the same inputs first.
 """
 
 from __future__ import annotations
@@ -16,6 +17,11 @@ import torch
 
 Tensor = torch.Tensor
 INT32_MAX = 2**31 - 1
+
+
+def _value_checks() -> bool:
+    """False while the current stream is being captured into a graph."""
+    return not (torch.cuda.is_available() and torch.cuda.is_current_stream_capturing())
 
 
 def _shape(name: str, value: Sequence[int], rank: int) -> tuple:
@@ -51,7 +57,7 @@ def _tensor(name: str, value, shape: tuple, reference: Optional[Tensor] = None) 
         raise ValueError(f"{name} must be a CUDA tensor")
     if reference is not None and (value.device != reference.device or value.dtype != reference.dtype):
         raise ValueError(f"{name} must have the same device and dtype as means")
-    if not bool(torch.isfinite(value).all()):
+    if _value_checks() and not bool(torch.isfinite(value).all()):
         raise ValueError(f"{name} must contain only finite values")
     return value.contiguous()
 
@@ -76,19 +82,20 @@ def _primitives(means, quats, scales, densities, beta, uncertainty):
     quats = _tensor("quats", quats, (n, 4), means)
     scales = _tensor("scales", scales, (n, 3), means)
     densities = _tensor("densities", densities, (n,), means)
-    if bool((quats == 0).all(dim=-1).any()):
+    checks = _value_checks()
+    if checks and bool((quats == 0).all(dim=-1).any()):
         raise ValueError("quats must have nonzero norm")
-    if bool((scales <= 0).any()):
+    if checks and bool((scales <= 0).any()):
         raise ValueError("scales must be strictly positive physical lengths")
-    if bool((densities < 0).any()):
+    if checks and bool((densities < 0).any()):
         raise ValueError("densities must be nonnegative")
     if beta is not None:
         beta = _tensor("beta", beta, (n,), means)
-        if bool((beta <= 0).any()):
+        if checks and bool((beta <= 0).any()):
             raise ValueError("beta must contain positive 3D exponents")
     if uncertainty is not None:
         uncertainty = _tensor("uncertainty", uncertainty, (n,), means)
-        if bool((uncertainty < 0).any()):
+        if checks and bool((uncertainty < 0).any()):
             raise ValueError("uncertainty must be nonnegative relative variance")
     return means, quats, scales, densities, beta, uncertainty
 

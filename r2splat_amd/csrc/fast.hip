@@ -39,9 +39,7 @@ static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
-constexpr uint32_t kT = 16;                          // tile edge of the fused path
-constexpr int kFwdWaves = 8;                         // waves sharing one tile (forward)
-constexpr int kBwdWaves = 8;                         // waves sharing one tile (backward)
+constexpr int kDefaultWaves = 8;                     // waves sharing one tile unless chosen otherwise
 
 __device__ __forceinline__ uint32_t qlim_of(float w) {
     const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
@@ -64,8 +62,9 @@ __global__ void __launch_bounds__(256)
 fused_project_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                      const float* __restrict__ scales, const float* __restrict__ dens,
                      const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
-                     uint32_t H, float eps, float clip, uint32_t ntx, uint32_t nty, uint32_t N,
-                     Splat* __restrict__ recs, uint2* __restrict__ rects, uint32_t* __restrict__ counts) {
+                     uint32_t H, float eps, float clip, uint32_t tile_size, uint32_t ntx, uint32_t nty,
+                     uint32_t N, Splat* __restrict__ recs, uint2* __restrict__ rects,
+                     uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = blockIdx.y;
     if (n >= N) return;
@@ -103,7 +102,8 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
         rec.w = w;
         rec.qlim = qlim_of(w);
         uint32_t x0, x1, y0, y1;
-        if (rx > 0 && ry > 0 && tile_span(mx, rx, kT, ntx, x0, x1) && tile_span(my, ry, kT, nty, y0, y1)) {
+        if (rx > 0 && ry > 0 && tile_span(mx, rx, tile_size, ntx, x0, x1) &&
+            tile_span(my, ry, tile_size, nty, y0, y1)) {
             count = (x1 - x0) * (y1 - y0);
             rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
         }
@@ -115,14 +115,16 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
 
 // Records (key = cam * n_tiles + tile, value = cam * N + n) at their scanned
 // positions; entries that would pass the capacity raise the overflow flag.
+// The projected record's g field is set to its first emission index.
 __global__ void __launch_bounds__(256)
 fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ cum, uint32_t total,
                   uint32_t N, uint32_t ntx, uint32_t nt, uint32_t capacity, uint32_t* __restrict__ keys,
-                  uint32_t* __restrict__ vals, uint32_t* __restrict__ overflow) {
+                  uint32_t* __restrict__ vals, uint32_t* __restrict__ overflow, Splat* __restrict__ recs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint32_t w0 = cum[i], cnt = cum[i + 1] - w0;
     if (cnt == 0) return;
+    recs[i].g = int32_t(w0);
     if (w0 + cnt > capacity) {
         *overflow = 1u;
         return;
@@ -152,15 +154,19 @@ fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restri
     offsets[k] = int32_t(lo);
 }
 
+// Workgroup b of a grid of 8k groups, renumbered so that the k consecutive
+// ranges of the sorted list land on the 8 XCDs one each (workgroups are
+// dealt to XCDs round-robin): each XCD's L2 then serves one band of cameras.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) { return (b % 8u) * (nb / 8u) + b / 8u; }
+
 // packed[i] = record of sorted intersection i, with g = its emission index
-// cum[v] + (ty - y0) (x1 - x0) + (tx - x0) (row-major within the record's
-// tile rectangle, the order fused_emit_kernel wrote it in).
+// w0 + (ty - y0) (x1 - x0) + (tx - x0) (row-major within the record's tile
+// rectangle, the order fused_emit_kernel wrote it in; w0 is the record's g).
 __global__ void __launch_bounds__(256)
-fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects,
-                  const uint32_t* __restrict__ cum, uint32_t n_recs, uint32_t N, uint32_t ntx, uint32_t nt,
-                  const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t N,
+                  uint32_t ntx, uint32_t nt, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                   const uint32_t* __restrict__ d_n, uint32_t capacity, Splat* __restrict__ packed) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= live(d_n, capacity)) return;
     const uint32_t v = vals[i];
     if (v >= n_recs) return;
@@ -169,7 +175,7 @@ fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rect
     const uint2 r = rects[v];
     const uint32_t x0 = r.x & 0xffffu, x1 = r.x >> 16, y0 = r.y & 0xffffu;
     Splat rec = recs[v];
-    rec.g = int32_t(cum[v] + (ty - y0) * (x1 - x0) + (tx - x0));
+    rec.g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
     packed[i] = rec;
 }
 
@@ -184,21 +190,24 @@ __device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, 
 }
 
 // ---------------------------------------------------------------------------
-// Forward: one workgroup of kFwdWaves waves per 16x16 tile. Wave k takes the
-// k-th contiguous slice of the tile's list and walks it in batches of 64
-// records: each lane loads one record of the next batch (coalesced vector
-// loads issued a whole batch ahead, so HBM latency is hidden behind the
-// current batch) and the wave reads the current batch back from its own LDS
-// slot with broadcast loads. Lane = column (lane % 16) and row group
-// (lane / 16) with 4 rows per lane (r, r+4, r+8, r+12); sigma reuses the
-// column terms across the lane's 4 pixels. The slices' partial images are
-// summed in LDS in wave order, so the result does not depend on scheduling.
+// Forward: one workgroup of WAVES waves per T x T tile (T = 8, 16, 32). Wave
+// k takes the k-th contiguous slice of the tile's list and walks it in
+// batches of 64 records: each lane loads one record of the next batch
+// (coalesced vector loads issued a whole batch ahead, so HBM latency is
+// hidden behind the current batch) and the wave reads the current batch back
+// from its own LDS slot with broadcast loads. Lane = column (lane % T) and
+// row group (lane / T), with PX = T^2 / 64 rows per lane spaced 64 / T apart;
+// sigma reuses the column terms across the lane's pixels. The slices'
+// partial images are summed in LDS in wave order, so the result does not
+// depend on scheduling.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void splat_accumulate(const Splat& s, float px, const float (&py)[4], float (&acc)[4]) {
+template <int PX>
+__device__ __forceinline__ void splat_accumulate(const Splat& s, float px, const float (&py)[PX],
+                                                 float (&acc)[PX]) {
     const float dx = px - s.mx;
     const float adx2 = s.A * dx * dx, bdx = s.B * dx;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < PX; ++k) {
         const float dy = py[k] - s.my;
         const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);
         const float e = __builtin_amdgcn_exp2f(-q);
@@ -218,27 +227,37 @@ __device__ __forceinline__ Splat load_splat(const Splat* __restrict__ packed, in
     return s;
 }
 
-__global__ void __launch_bounds__(kFwdWaves * kWave)
+// Marks a record's registers as used here: the loads that produced them (a
+// whole chunk earlier) are waited for at this point, before the next
+// prefetch is issued, so later waits need not drain that prefetch.
+__device__ __forceinline__ void settle(const Splat& s) {
+    asm volatile("" ::"v"(s.mx), "v"(s.my), "v"(s.A), "v"(s.B), "v"(s.C), "v"(s.w), "v"(s.qlim), "v"(s.g));
+}
+
+template <int T, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWave)
 fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
                         uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H, float* __restrict__ out) {
-    __shared__ float part[kFwdWaves][kT * kT];
-    __shared__ Splat stage[kFwdWaves][kWave];
+    constexpr int PX = T * T / kWave, STEP = kWave / T;
+    static_assert(PX * kWave == T * T && STEP * T == kWave, "tile must be 8, 16 or 32");
+    __shared__ float part[WAVES][T * T];
+    __shared__ Splat stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
-    const int32_t per = (end - start + kFwdWaves - 1) / kFwdWaves;
+    const int32_t per = (end - start + WAVES - 1) / WAVES;
     const int32_t lo = min(end, start + int32_t(wave) * per), hi = min(end, lo + per);
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
-    const uint32_t lane = threadIdx.x % kWave, col = lane % 16, row0 = lane / 16;
-    const float px = float(tx * kT + col) + 0.5f;
-    float py[4], acc[4];
+    const uint32_t lane = threadIdx.x % kWave, col = lane % T, row0 = lane / T;
+    const float px = float(tx * T + col) + 0.5f;
+    float py[PX], acc[PX];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        py[k] = float(ty * kT + row0 + 4 * k) + 0.5f;
+    for (int k = 0; k < PX; ++k) {
+        py[k] = float(ty * T + row0 + STEP * k) + 0.5f;
         acc[k] = 0.f;
     }
     Splat* slot = stage[wave];
@@ -249,28 +268,33 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
         wave_lds_sync();
         next = load_splat(packed, base + kWave + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
-#pragma unroll 2
-        for (int32_t j = 0; j < n; ++j) splat_accumulate(slot[j], px, py, acc);
+        // LDS reads pipelined one record ahead (the batch holds n >= 1 records)
+        Splat cur = slot[0];
+        for (int32_t j = 1; j < n; ++j) {
+            const Splat nxt = slot[j];
+            splat_accumulate<PX>(cur, px, py, acc);
+            cur = nxt;
+        }
+        splat_accumulate<PX>(cur, px, py, acc);
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) part[wave][(row0 + 4 * k) * kT + col] = acc[k];
+    for (int k = 0; k < PX; ++k) part[wave][(row0 + STEP * k) * T + col] = acc[k];
     __syncthreads();
-    if (threadIdx.x < kT * kT) {
-        const uint32_t p = threadIdx.x;
+    for (uint32_t p = threadIdx.x; p < uint32_t(T * T); p += WAVES * kWave) {
         float v = 0.f;
 #pragma unroll
-        for (int k = 0; k < kFwdWaves; ++k) v += part[k][p];
-        const uint32_t x = tx * kT + p % kT, y = ty * kT + p / kT;
+        for (int k = 0; k < WAVES; ++k) v += part[k][p];
+        const uint32_t x = tx * T + p % T, y = ty * T + p / T;
         if (x < W && y < H) out[(size_t(cam) * H + y) * W + x] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Backward: one workgroup of kBwdWaves waves per tile; wave k takes chunks
-// k, k + kBwdWaves, ... of 64 records, one lane per record. The tile's
-// adjoint image sits in LDS and is read with wave-uniform (broadcast) loads,
-// row by row, so each lane sums its own primitive's moments over the 256
-// pixels without any cross-lane reduction:
+// Backward: one workgroup of WAVES waves per T x T tile; wave k takes chunks
+// k, k + WAVES, ... of 64 records, one lane per record. The tile's adjoint
+// image sits in LDS and is read with wave-uniform (broadcast) loads, row by
+// row, so each lane sums its own primitive's moments over the T^2 pixels
+// without any cross-lane reduction:
 //   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
 // with dx = px - mx (the moments are summed directly in dx: expanding them
 // around the tile corner cancels badly for primitives centred in the tile).
@@ -278,12 +302,13 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 //   v_mean = w (a Mx + b My, b Mx + c My), v_conic = -w (Mxx/2, Mxy, Myy/2),
 //   v_w = M0, stored (no atomics) as the partial of emission index g.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBwdWaves * kWave)
+template <int T, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWave)
 fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
                         uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
                         const float* __restrict__ v_img, float* __restrict__ isect_grads) {
-    __shared__ __align__(16) float sv[kT * kT];
+    __shared__ __align__(16) float sv[T * T];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
@@ -293,29 +318,30 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
     const uint32_t lane = threadIdx.x % kWave;
-    if (threadIdx.x < kT * kT) {
-        const uint32_t p = threadIdx.x, x = tx * kT + p % kT, y = ty * kT + p / kT;
+    for (uint32_t p = threadIdx.x; p < uint32_t(T * T); p += WAVES * kWave) {
+        const uint32_t x = tx * T + p % T, y = ty * T + p / T;
         sv[p] = (x < W && y < H) ? v_img[(size_t(cam) * H + y) * W + x] : 0.f;
     }
     __syncthreads();
-    const float x0 = float(tx * kT) + 0.5f, y0 = float(ty * kT) + 0.5f;
-    constexpr int32_t kStride = kBwdWaves * kWave;
+    const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f;
+    constexpr int32_t kStride = WAVES * kWave;
     int32_t chunk = start + int32_t(wave) * kWave;
     Splat next = load_splat(packed, chunk + int32_t(lane), end);
     for (; chunk < end; chunk += kStride) {
         const int32_t i = chunk + int32_t(lane);
         const bool active = i < end;
         const Splat s = next;
+        settle(s);
         next = load_splat(packed, chunk + kStride + int32_t(lane), end);  // next chunk, a chunk ahead
         const float dx0 = x0 - s.mx;
         float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f;
-        for (uint32_t r = 0; r < kT; ++r) {
+        for (uint32_t r = 0; r < uint32_t(T); ++r) {
             const float dy = y0 + float(r) - s.my;
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
-            const float4* row = reinterpret_cast<const float4*>(sv + r * kT);
-            float v[16];
+            const float4* row = reinterpret_cast<const float4*>(sv + r * T);
+            float v[T];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < T / 4; ++k) {
                 const float4 q4 = row[k];
                 v[4 * k] = q4.x;
                 v[4 * k + 1] = q4.y;
@@ -324,7 +350,7 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
             }
             float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
+            for (int j = 0; j < T; ++j) {
                 const float dx = dx0 + float(j);
                 const float q = fmaf(dx, fmaf(s.A, dx, bdy), cdy2);
                 const float e = __builtin_amdgcn_exp2f(-q);
@@ -469,13 +495,54 @@ static int fused_check(uint32_t C, uint32_t N, uint32_t ntx, uint32_t nty, uint3
     return R2S_OK;
 }
 
+static int check_tile(uint32_t tile_size, uint32_t waves) {
+    R2S_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "fused tile_size must be 8, 16 or 32 (got %u)",
+                tile_size);
+    R2S_REQUIRE(waves == 0 || waves == 4 || waves == 8 || waves == 16, "waves per tile must be 4, 8 or 16 (got %u)",
+                waves);
+    return R2S_OK;
+}
+
+// Launches KERNEL<T, WAVES> for the runtime tile size and wave count.
+template <template <int, int> class Launch, typename... Args>
+static void launch_tiled(uint32_t tile, uint32_t waves, Args&&... args) {
+    if (waves == 0) waves = kDefaultWaves;
+#define R2S_TW(T_, W_) \
+    if (tile == T_ && waves == W_) return Launch<T_, W_>::run(args...);
+    R2S_TW(8, 4) R2S_TW(8, 8) R2S_TW(8, 16)
+    R2S_TW(16, 4) R2S_TW(16, 8) R2S_TW(16, 16)
+    R2S_TW(32, 4) R2S_TW(32, 8) R2S_TW(32, 16)
+#undef R2S_TW
+}
+
+template <int T, int WAVES>
+struct LaunchFwd {
+    static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
+                    const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
+                    float* image) {
+        fused_raster_fwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(packed, offsets, counters, capacity,
+                                                                                n_tiles, ntx, nty, W, H, image);
+    }
+};
+
+template <int T, int WAVES>
+struct LaunchBwd {
+    static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
+                    const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
+                    const float* v_image, float* isect_grads) {
+        fused_raster_bwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            packed, offsets, counters, capacity, n_tiles, ntx, nty, W, H, v_image, isect_grads);
+    }
+};
+
 extern "C" int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
-                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                                 int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                                 void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
                                  r2s_stream_t stream) {
-    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
+    if (int rc = check_tile(tile_size, 0)) return rc;
+    const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
     const FusedScratch L = fused_scratch(C, N, capacity);
     R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
@@ -485,7 +552,7 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     if (CN > 0) {
         dim3 grid(ceil_div(N, 256), C);
         fused_project_kernel<<<grid, 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks, width,
-                                                       height, eps, radius_clip, ntx, nty, N,
+                                                       height, eps, radius_clip, tile_size, ntx, nty, N,
                                                        at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
@@ -494,7 +561,8 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     if (CN > 0) {
         fused_emit_kernel<<<ceil_div(CN, 256), 256, 0, stream>>>(at<uint2>(scratch, L.rects), cum, CN, N, ntx,
                                                                  ntx * nty, capacity, sort_keys_buffer(sort_ws),
-                                                                 sort_vals_buffer(sort_ws, capacity), counters + 1);
+                                                                 sort_vals_buffer(sort_ws, capacity), counters + 1,
+                                                                 at<Splat>(scratch, L.recs));
         if (int rc = check_launch("fused_emit")) return rc;
     }
     uint32_t* keys = at<uint32_t>(scratch, L.keys);
@@ -505,32 +573,37 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, counters, capacity, n_tiles,
                                                                      tile_offsets);
     if (int rc = check_launch("fused_offsets")) return rc;
-    fused_pack_kernel<<<ceil_div(capacity, 256), 256, 0, stream>>>(
-        at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), cum, CN, N, ntx, ntx * nty, keys, vals, counters,
-        capacity, static_cast<Splat*>(packed));
+    const uint32_t pack_blocks = (ceil_div(capacity, 256) + 7u) & ~7u;  // a multiple of the 8 XCDs
+    fused_pack_kernel<<<pack_blocks, 256, 0, stream>>>(at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, N,
+                                                       ntx, ntx * nty, keys, vals, counters, capacity,
+                                                       static_cast<Splat*>(packed));
     return check_launch("fused_pack");
 }
 
 extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                                uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, float* image,
-                                r2s_stream_t stream) {
-    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT, n_tiles = C * ntx * nty;
+                                uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
+                                uint32_t waves, float* image, r2s_stream_t stream) {
+    if (int rc = check_tile(tile_size, waves)) return rc;
+    const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
+    const uint64_t n_tiles = uint64_t(C) * ntx * nty;
+    R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    fused_raster_fwd_kernel<<<n_tiles, kFwdWaves * kWave, 0, stream>>>(
-        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
-        image);
+    launch_tiled<LaunchFwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
+                            tile_offsets, counters, capacity, ntx, nty, width, height, image);
     return check_launch("fused_raster_fwd");
 }
 
 extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
                                     uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                                    const float* v_image, float* isect_grads, r2s_stream_t stream) {
-    const uint32_t ntx = (width + kT - 1) / kT, nty = (height + kT - 1) / kT;
-    const uint32_t n_tiles = C * ntx * nty;
+                                    uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
+                                    r2s_stream_t stream) {
+    if (int rc = check_tile(tile_size, waves)) return rc;
+    const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
+    const uint64_t n_tiles = uint64_t(C) * ntx * nty;
+    R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    fused_raster_bwd_kernel<<<n_tiles, kBwdWaves * kWave, 0, stream>>>(
-        static_cast<const Splat*>(packed), tile_offsets, counters, capacity, n_tiles, ntx, nty, width, height,
-        v_image, isect_grads);
+    launch_tiled<LaunchBwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
+                            tile_offsets, counters, capacity, ntx, nty, width, height, v_image, isect_grads);
     return check_launch("fused_raster_bwd");
 }
 
@@ -550,26 +623,27 @@ extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const
 
 extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                 uint32_t N, uint32_t width, uint32_t height, float eps, float radius_clip,
-                                 uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                                 int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
+                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                                 void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
                                  float* image, r2s_stream_t stream) {
-    if (int rc = r2s_fused_prepare(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
-                                   radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
+    if (int rc = r2s_fused_prepare(means, quats, scales, densities, viewmats, Ks, C, N, width, height, tile_size,
+                                   eps, radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
                                    record_offsets, counters, stream))
         return rc;
-    return r2s_fused_render(packed, tile_offsets, counters, capacity, C, width, height, image, stream);
+    return r2s_fused_render(packed, tile_offsets, counters, capacity, C, width, height, tile_size, 0, image,
+                            stream);
 }
 
 extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                                   const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                  uint32_t N, uint32_t width, uint32_t height, float eps, uint32_t capacity,
-                                  const void* packed, const int32_t* tile_offsets, const uint32_t* record_offsets,
-                                  const uint32_t* counters, float* isect_grads, const float* v_image,
-                                  float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                                  r2s_stream_t stream) {
-    if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, counters, capacity, C, width, height, v_image,
-                                      isect_grads, stream))
+                                  uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                                  uint32_t capacity, const void* packed, const int32_t* tile_offsets,
+                                  const uint32_t* record_offsets, const uint32_t* counters, float* isect_grads,
+                                  const float* v_image, float* v_means, float* v_quats, float* v_scales,
+                                  float* v_densities, r2s_stream_t stream) {
+    if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, counters, capacity, C, width, height, tile_size, 0,
+                                      v_image, isect_grads, stream))
         return rc;
     return r2s_fused_param_vjp(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
                                record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,

@@ -22,7 +22,7 @@ import torch
 from . import fused
 from . import hip as ops
 from ._beta import beta_path_log_grad, beta_path_ratio
-from ._validation import _constant, _options, _primitives, _shape, _tensor, _vector
+from ._validation import _constant, _options, _primitives, _shape, _tensor, _value_checks, _vector
 from .profiling import stage
 
 Tensor = torch.Tensor
@@ -213,12 +213,13 @@ def rasterize_fwd(
     if max(height, width) > 16 * 65535:
         raise ValueError("image_size exceeds the kernels' grid limits")
     Ks = _constant("Ks", _tensor("Ks", Ks, (C, 3, 3), means))
-    if bool((Ks[:, (0, 1), (0, 1)] <= 0).any()):
-        raise ValueError("Ks must have positive fx and fy")
-    if bool((Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any()) or bool((Ks[:, 2, 2] != 1).any()):
-        raise ValueError("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]")
-    if not torch.equal(viewmats[:, 3], means.new_tensor([0, 0, 0, 1]).expand(C, -1)):
-        raise ValueError("viewmats must have homogeneous last row [0, 0, 0, 1]")
+    if _value_checks():
+        if bool((Ks[:, (0, 1), (0, 1)] <= 0).any()):
+            raise ValueError("Ks must have positive fx and fy")
+        if bool((Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any()) or bool((Ks[:, 2, 2] != 1).any()):
+            raise ValueError("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]")
+        if not torch.equal(viewmats[:, 3], means.new_tensor([0, 0, 0, 1]).expand(C, -1)):
+            raise ValueError("viewmats must have homogeneous last row [0, 0, 0, 1]")
     shape = (C, height, width)
     buffers = {"viewmats": viewmats, "Ks": Ks}
     if means.shape[0] == 0:
@@ -360,7 +361,7 @@ def voxelize_fwd(
             _constant(name, value)
     extent = _vector("volume_extent", volume_extent, means)
     center = _vector("volume_center", volume_center, means)
-    if bool((extent <= 0).any()):
+    if _value_checks() and bool((extent <= 0).any()):
         raise ValueError("volume_extent must contain positive physical lengths")
     shape = (depth, height, width)
     buffers = {"extent": extent, "center": center}

@@ -1,4 +1,4 @@
-"""Fused float32 Gaussian rasterization (16x16 tiles) on the HIP library.
+"""Fused float32 Gaussian rasterization (8, 16 or 32 pixel tiles) on the HIP library.
 
 ``FusedForward`` holds what the backward needs (per-intersection records,
 tile offsets, per-(camera, primitive) intersection offsets, device counters)
@@ -20,7 +20,7 @@ from .hip import _backend as be
 from .hip._backend import call, ptr
 from .profiling import stage
 
-TILE = 16
+TILE = 16  # default tile edge (the explicit API's)
 
 
 class CapacityCache:
@@ -32,6 +32,9 @@ class CapacityCache:
 
     def get(self, key, C, N):
         return self._cap.get(key, max(1024, 4 * C * N))
+
+    def known(self, key) -> bool:
+        return key in self._cap
 
     def grow(self, key, needed: int):
         self._cap[key] = max(1024, int(math.ceil(needed * self.headroom)))
@@ -45,6 +48,10 @@ class CapacityCache:
 
 CAPACITY = CapacityCache()
 
+# Device counters of the most recent prepare() ({intersections, overflow});
+# lets a caller of captured steps check the overflow flag after replaying.
+LAST_COUNTERS = None
+
 
 @dataclass
 class FusedForward:
@@ -57,6 +64,7 @@ class FusedForward:
     width: int
     height: int
     eps: float
+    tile_size: int = TILE
 
 
 def _scratch(C, N, capacity, device):
@@ -65,13 +73,24 @@ def _scratch(C, N, capacity, device):
 
 
 def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=True) -> FusedForward:
-    """Projection, binning, sort and record packing (everything but the image). float32, one device."""
+            check=None, tile_size: int = TILE) -> FusedForward:
+    """Projection, binning, sort and record packing (everything but the image). float32, one device.
+
+    ``check`` (default: unless the stream is being captured into a graph)
+    reads the overflow flag and re-runs with more capacity. Under capture the
+    capacity of the last eager call with the same shapes is used and the flag
+    is left on the device (``FusedForward.counters[1]``) for the caller."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
-    key = (device, C, N, height, width)
+    key = (device, C, N, height, width, tile_size)
+    capturing = torch.cuda.is_current_stream_capturing()
+    if check is None:
+        check = not capturing
+    if capturing and capacity is None and not CAPACITY.known(key):
+        raise RuntimeError("fused.prepare under graph capture needs a capacity: run one eager call with the "
+                           "same shapes first")
     cap = int(capacity or CAPACITY.get(key, C, N))
-    ntx, nty = -(-width // TILE), -(-height // TILE)
+    ntx, nty = -(-width // tile_size), -(-height // tile_size)
     rec = be.lib().r2s_fused_record_bytes()
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
     record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
@@ -84,7 +103,7 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
             packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
             with stage("project_bin"):
                 call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
-                     ptr(Ks), C, N, width, height, float(eps), float(radius_clip), cap, ptr(scratch),
+                     ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
                      scratch.numel(), ptr(packed), ptr(offsets), ptr(record_offsets), ptr(counters), stream)
             del scratch
             if not check:
@@ -94,28 +113,31 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
                 CAPACITY.settle(key, n_isects)
                 break
             cap = CAPACITY.grow(key, n_isects)
-    return FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps))
+    global LAST_COUNTERS
+    LAST_COUNTERS = counters
+    return FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps),
+                        tile_size)
 
 
-def render(fwd: FusedForward, C: int, device) -> torch.Tensor:
-    """The image [C, H, W] of a prepared forward."""
+def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
+    """The image [C, H, W] of a prepared forward (`waves` per tile: 4, 8, 16; 0 = 8)."""
     image = torch.empty((C, fwd.height, fwd.width), dtype=torch.float32, device=device)
     with torch.cuda.device(device):
         with stage("rasterize_fwd"):
             call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
-                 fwd.width, fwd.height, ptr(image), be.stream_of(device))
+                 fwd.width, fwd.height, fwd.tile_size, waves, ptr(image), be.stream_of(device))
     return image
 
 
 def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=True):
+            check=None, tile_size: int = TILE, waves: int = 0):
     """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
     fwd = prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip,
-                  capacity=capacity, check=check)
-    return render(fwd, viewmats.shape[0], means.device), fwd
+                  capacity=capacity, check=check, tile_size=tile_size)
+    return render(fwd, viewmats.shape[0], means.device, waves), fwd
 
 
-def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image):
+def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image, waves: int = 0):
     """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities)."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
@@ -128,7 +150,7 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
         stream = be.stream_of(device)
         with stage("rasterize_bwd"):
             call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
-                 fwd.width, fwd.height, ptr(v_image), ptr(grad), stream)
+                 fwd.width, fwd.height, fwd.tile_size, waves, ptr(v_image), ptr(grad), stream)
         with stage("param_bwd"):
             call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
                  ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps, ptr(fwd.record_offsets), ptr(fwd.counters),

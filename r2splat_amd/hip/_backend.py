@@ -56,12 +56,15 @@ SIGNATURES = {
     "r2s_voxelize_to_voxels_bwd_per_gaussian": [I32, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, P, P, P, P, P, P, P],
     "r2s_fused_scratch_bytes": [U32, U32, U32],
     "r2s_fused_record_bytes": [],
-    "r2s_fused_forward": [P, P, P, P, P, P, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P],
-    "r2s_fused_backward": [P, P, P, P, P, P, U32, U32, U32, U32, F, U32, P, P, P, P, P, P, P, P, P, P, P],
-    "r2s_fused_prepare": [P, P, P, P, P, P, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P],
-    "r2s_fused_render": [P, P, P, U32, U32, U32, U32, P, P],
-    "r2s_fused_raster_vjp": [P, P, P, U32, U32, U32, U32, P, P, P],
+    "r2s_fused_forward": [P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P],
+    "r2s_fused_backward": [P, P, P, P, P, P, U32, U32, U32, U32, U32, F, U32, P, P, P, P, P, P, P, P, P, P, P],
+    "r2s_fused_prepare": [P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P],
+    "r2s_fused_render": [P, P, P, U32, U32, U32, U32, U32, U32, P, P],
+    "r2s_fused_raster_vjp": [P, P, P, U32, U32, U32, U32, U32, U32, P, P, P],
     "r2s_fused_param_vjp": [P, P, P, P, P, P, U32, U32, U32, U32, F, P, P, U32, P, P, P, P, P, P],
+    "r2s_mse_partials": [U64],
+    "r2s_mse_adjoint": [P, P, U64, P, P, P, P],
+    "r2s_adam_step": [P, P, P, P, U64, F, F, F, F, P, P, P],
     "r2s_volume_tv_fwd": [I32, P, U32, U32, U32, U32, P, P, P],
     "r2s_volume_tv_bwd": [I32, P, P, U32, U32, U32, U32, D, P, P],
     "r2s_image_l1_fwd": [I32, P, P, U64, P, P, P],
@@ -74,6 +77,7 @@ _RESTYPES = {
     "r2s_sort_workspace_bytes": SZ,
     "r2s_fused_scratch_bytes": SZ,
     "r2s_fused_record_bytes": SZ,
+    "r2s_mse_partials": SZ,
 }
 
 
@@ -81,8 +85,11 @@ class _Library:
     def __init__(self):
         self._lib = None
         self._error = None
+        self._gpu = False
 
     def load(self, require_gpu: bool = True):
+        if self._gpu:
+            return self._lib
         if self._lib is None and self._error is None:
             try:
                 lib = ctypes.CDLL(str(LIB_PATH))
@@ -98,8 +105,10 @@ class _Library:
                 f"r2splat_amd HIP library is not available ({self._error}). "
                 "Build it with `python -m r2splat_amd._build`."
             )
-        if require_gpu and not torch.cuda.is_available():
-            raise RuntimeError("r2splat_amd operators require a ROCm GPU; none is visible to PyTorch")
+        if require_gpu:
+            if not torch.cuda.is_available():
+                raise RuntimeError("r2splat_amd operators require a ROCm GPU; none is visible to PyTorch")
+            self._gpu = True
         return self._lib
 
 
@@ -125,9 +134,10 @@ def is_available() -> bool:
 
 def call(name: str, *args) -> None:
     """Calls a status-returning entry point, raising RuntimeError on failure."""
-    status = getattr(lib(), name)(*args)
+    handle = lib()
+    status = getattr(handle, name)(*args)
     if status != 0:
-        message = lib().r2s_last_error().decode(errors="replace")
+        message = handle.r2s_last_error().decode(errors="replace")
         raise RuntimeError(f"{name}: {message}")
 
 

@@ -1,0 +1,203 @@
+"""Graph-captured fits of float32 Gaussian scenes to target images.
+
+``GaussianFit`` owns one reconstruction job: a flat parameter buffer (means,
+quats, scales, densities as views), its gradient buffer, Adam state and every
+intermediate of the fused pipeline, all allocated once. One step is
+
+    fused prepare -> render -> MSE adjoint -> raster VJP -> parameter VJP
+    [-> Adam]
+
+issued on one stream with no host synchronisation, so it can be captured into
+a hipGraph (``torch.cuda.CUDAGraph``) and replayed. The only data-dependent
+size, the intersection count, is bounded by a capacity: a step whose count
+exceeds it raises the overflow flag, renders nothing, and its Adam update is
+skipped on the device (counted in ``adam_state[1]``). ``sync()`` reads the
+flags, grows the capacity, re-captures and replays the skipped steps, so the
+sequence of applied updates is exactly that of an unbounded run.
+
+The reference has no training loop (r2splat/explicit.py leaves activations,
+losses and optimizers to the caller); this is the batched-fit layer on top of
+the same operators.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import fused
+from .hip import _backend as be
+from .hip._backend import call, ptr
+
+TILE = fused.TILE
+
+
+@dataclass
+class AdamConfig:
+    lr: float = 1e-2
+    beta1: float = 0.9
+    beta2: float = 0.999
+    eps: float = 1e-8
+
+
+class GaussianFit:
+    """One job: N Gaussians, C views of H x W, MSE against ``target`` [C, H, W]."""
+
+    def __init__(self, means, quats, scales, densities, viewmats, Ks, target, *, eps: float = 1e-8,
+                 radius_clip: float = 3.0, adam: Optional[AdamConfig] = None, capacity: Optional[int] = None,
+                 headroom: float = 1.15, stream: Optional[torch.cuda.Stream] = None, tile_size: int = TILE,
+                 fwd_waves: int = 0, bwd_waves: int = 0):
+        device = means.device
+        if means.dtype != torch.float32:
+            raise TypeError("GaussianFit runs the float32 fused pipeline")
+        self.device = device
+        self.N = int(means.shape[0])
+        self.C = int(viewmats.shape[0])
+        self.H, self.W = int(target.shape[-2]), int(target.shape[-1])
+        if tuple(target.shape) != (self.C, self.H, self.W):
+            raise ValueError("target must be [C, H, W]")
+        self.eps, self.radius_clip = float(eps), float(radius_clip)
+        self.adam = adam
+        self.headroom = headroom
+        self.tile_size, self.fwd_waves, self.bwd_waves = int(tile_size), int(fwd_waves), int(bwd_waves)
+        self.stream = stream
+        N = self.N
+        self.params = torch.empty((11 * N,), dtype=torch.float32, device=device)
+        self.grads = torch.zeros_like(self.params)
+        self.means, self.quats, self.scales, self.densities = self._views(self.params)
+        self.v_means, self.v_quats, self.v_scales, self.v_densities = self._views(self.grads)
+        for dst, src in zip((self.means, self.quats, self.scales, self.densities), (means, quats, scales, densities)):
+            dst.copy_(src)
+        self.viewmats = viewmats.contiguous().to(device, torch.float32)
+        self.Ks = Ks.contiguous().to(device, torch.float32)
+        self.target = target.contiguous().to(device, torch.float32)
+        if adam is not None:
+            self.exp_avg = torch.zeros_like(self.params)
+            self.exp_avg_sq = torch.zeros_like(self.params)
+        self.adam_state = torch.zeros((2,), dtype=torch.int32, device=device)
+        n_pix = self.C * self.H * self.W
+        self.image = torch.empty((self.C, self.H, self.W), dtype=torch.float32, device=device)
+        self.v_image = torch.empty_like(self.image)
+        self.loss_partials = torch.empty((be.lib().r2s_mse_partials(n_pix),), dtype=torch.float32, device=device)
+        self.loss = torch.zeros((1,), dtype=torch.float32, device=device)
+        ntx, nty = -(-self.W // self.tile_size), -(-self.H // self.tile_size)
+        self.tile_offsets = torch.empty((self.C, nty, ntx), dtype=torch.int32, device=device)
+        self.record_offsets = torch.empty((self.C * N + 1,), dtype=torch.int32, device=device)
+        self.counters = torch.zeros((2,), dtype=torch.int32, device=device)
+        self.graph = None
+        self.steps_issued = 0
+        self._alloc(int(capacity) if capacity else self._measure_capacity())
+
+    # -- buffers --------------------------------------------------------------
+    def _views(self, flat):
+        N = self.N
+        return (flat[: 3 * N].view(N, 3), flat[3 * N: 7 * N].view(N, 4), flat[7 * N: 10 * N].view(N, 3),
+                flat[10 * N:])
+
+    def _alloc(self, capacity: int):
+        self.capacity = max(1024, int(capacity))
+        lib = be.lib()
+        self.scratch = torch.empty((lib.r2s_fused_scratch_bytes(self.C, self.N, self.capacity),), dtype=torch.uint8,
+                                   device=self.device)
+        self.packed = torch.empty((self.capacity * lib.r2s_fused_record_bytes(),), dtype=torch.uint8,
+                                  device=self.device)
+        self.isect_grads = torch.empty((self.capacity * 6,), dtype=torch.float32, device=self.device)
+        self.graph = None
+
+    def _measure_capacity(self) -> int:
+        fwd = fused.prepare(self.means, self.quats, self.scales, self.densities, self.viewmats, self.Ks, self.W,
+                            self.H, self.eps, self.radius_clip, tile_size=self.tile_size)
+        return int(math.ceil(fwd.n_isects * self.headroom)) + 1024
+
+    def memory_bytes(self) -> int:
+        """Device bytes held by this job (parameters, state and intermediates)."""
+        tensors = [self.params, self.grads, self.image, self.v_image, self.loss_partials, self.tile_offsets,
+                   self.record_offsets, self.scratch, self.packed, self.isect_grads, self.target]
+        if self.adam is not None:
+            tensors += [self.exp_avg, self.exp_avg_sq]
+        return sum(t.numel() * t.element_size() for t in tensors)
+
+    # -- one step -------------------------------------------------------------
+    def _issue(self, with_loss: bool = True):
+        """Enqueues one step on the current stream (no host synchronisation)."""
+        s = be.stream_of(self.device)
+        C, N, W, H = self.C, self.N, self.W, self.H
+        call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
+             ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size, self.eps, self.radius_clip, self.capacity,
+             ptr(self.scratch), self.scratch.numel(), ptr(self.packed), ptr(self.tile_offsets),
+             ptr(self.record_offsets), ptr(self.counters), s)
+        call("r2s_fused_render", ptr(self.packed), ptr(self.tile_offsets), ptr(self.counters), self.capacity, C, W,
+             H, self.tile_size, self.fwd_waves, ptr(self.image), s)
+        call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
+             ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
+        call("r2s_fused_raster_vjp", ptr(self.packed), ptr(self.tile_offsets), ptr(self.counters), self.capacity, C,
+             W, H, self.tile_size, self.bwd_waves, ptr(self.v_image), ptr(self.isect_grads), s)
+        call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
+             ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.eps, ptr(self.record_offsets), ptr(self.counters),
+             self.capacity, ptr(self.isect_grads), ptr(self.v_means), ptr(self.v_quats), ptr(self.v_scales),
+             ptr(self.v_densities), s)
+        if self.adam is not None:
+            a = self.adam
+            call("r2s_adam_step", ptr(self.params), ptr(self.grads), ptr(self.exp_avg), ptr(self.exp_avg_sq),
+                 self.params.numel(), a.lr, a.beta1, a.beta2, a.eps, ptr(self.adam_state), ptr(self.counters[1:]),
+                 s)
+
+    def _ctx(self):
+        return torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.device(self.device)
+
+    def capture(self):
+        """Captures one step into a hipGraph (after one eager warm-up step)."""
+        with self._ctx():
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(self.device)
+            side.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    self._issue()
+            torch.cuda.current_stream(self.device).wait_stream(side)
+        self.graph = g
+        return g
+
+    def step(self, n: int = 1, graph: bool = True):
+        """Issues n steps (graph replays when captured) without synchronising."""
+        with self._ctx():
+            for _ in range(n):
+                if graph:
+                    if self.graph is None:
+                        raise RuntimeError("capture() first")
+                    self.graph.replay()
+                else:
+                    self._issue()
+        self.steps_issued += n
+
+    def sync(self) -> int:
+        """Waits for the issued steps; on an overflow, grows the capacity,
+        re-captures and re-runs the skipped steps. Returns the steps applied."""
+        with self._ctx():
+            torch.cuda.current_stream(self.device).synchronize()
+            while True:
+                count, overflow = (int(v) for v in self.counters.tolist())
+                applied, skipped = (int(v) for v in self.adam_state.tolist())
+                if self.adam is None:
+                    if not overflow:
+                        return self.steps_issued
+                    skipped = 1
+                if skipped == 0 and not overflow:
+                    return applied
+                recapture = self.graph is not None
+                self._alloc(int(math.ceil(max(count, self.capacity) * self.headroom)) + 1024)
+                self.adam_state[1] = 0
+                if recapture:
+                    self.capture()
+                for _ in range(skipped):
+                    self.step(1, graph=recapture)
+                torch.cuda.current_stream(self.device).synchronize()
+
+    def loss_value(self) -> float:
+        return float(self.loss.item())
+
+
+__all__ = ["AdamConfig", "GaussianFit"]

@@ -105,16 +105,44 @@ def test_voxelize_fd_fp64(device):
     _check_fd(lambda: (run() * w).sum(), params, grads, seed=6, h=1e-8)
 
 
-def test_config1_fd_on_32_primitive_subset_fp64(device):
-    """100k Gaussians x 64 views of 256^2 in float64 through the wrapper; MSE
-    against the seed-1 target; central FD along random directions supported on
-    32 random primitives (all 11 parameters each), and per parameter group.
+def _subset_oracle_vjp(sc, idx, v, device):
+    """FP64 oracle VJP of sum(v * image) w.r.t. the primitives `idx`, from their
+    own contribution images (the image is additive over primitives), with the
+    tile-coverage mask of their radii; one camera at a time."""
+    from oracle import splat as O
+    from r2splat_amd import hip as ops
 
-    The loss difference is contracted from the image difference,
-    L(+) - L(-) = mean((img+ - img-) (img+ + img- - 2 target)), so the tiny
-    change of the scene-wide loss is not lost to the rounding of the loss
-    itself; a small step keeps 1e-5 cut-off crossings rare (a direction with
-    one is detected by comparing h and h/2 and skipped)."""
+    sub = [t[idx].detach() for t in sc.params()]
+    covars, _ = ops.parameter_transform_fwd(sub[1].contiguous(), sub[2].contiguous())
+    m2, _, radii = ops.world_to_image_projection_fwd(sub[0].contiguous(), covars, sc.viewmats, sc.Ks, sc.width,
+                                                     sc.height, 1e-8, 3.0)
+    mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), 16, sc.width, sc.height)
+    leaves = [t.cpu().double().requires_grad_() for t in sub]
+    total = [torch.zeros_like(t) for t in leaves]
+    vm, Ks, vc = sc.viewmats.cpu().double(), sc.Ks.cpu().double(), v.cpu().double()
+    for c in range(vm.shape[0]):
+        img = O.rasterize_scene(*leaves, vm[c:c + 1], Ks[c:c + 1], sc.image_size, eps=1e-8, mask=mask[c:c + 1])
+        for acc, g in zip(total, torch.autograd.grad((img * vc[c:c + 1]).sum(), leaves)):
+            acc += g
+    return [t.to(device) for t in total]
+
+
+def test_config1_gradients_on_32_primitive_subset_fp64(device):
+    """100k Gaussians x 64 views of 256^2 in float64 through the wrapper, MSE
+    against the seed-1 target. On a random 32-primitive subset:
+
+    * the FP64 oracle's VJP (from those primitives' own contribution images,
+      tile-aware) equals the analytic gradient rows to 1e-8;
+    * central finite differences along random directions on the subset agree
+      with the analytic directional derivative: to 1e-5 for quats and
+      densities (step 1e-8, with a step-h/2 check that skips a direction
+      crossing a cut-off), and to 5e-2 for means and scales. A step in a mean
+      or scale moves the 1e-5 cut-off contours and tile-coverage edges of all
+      64 x 32 footprints across pixel centres often enough that central
+      differences of this loss drift by percents between steps 1e-5 and 1e-9,
+      so the oracle VJP carries the exact check for those two groups."""
+    from oracle import splat as O
+
     sc = workloads.config1(device=device, dtype=torch.float64)
     tgt = workloads.config1(device=device, dtype=torch.float64, seed=1)
     with torch.no_grad():
@@ -129,6 +157,13 @@ def test_config1_fd_on_32_primitive_subset_fp64(device):
     grads = torch.autograd.grad(((img - target) ** 2).mean(), params)
     gen = torch.Generator(device="cpu").manual_seed(0)
     idx = torch.randperm(sc.means.shape[0], generator=gen)[:32].to(device)
+
+    v = (2.0 / img.numel()) * (img.detach() - target)
+    expect = _subset_oracle_vjp(sc, idx, v, device)
+    for name, g, e in zip(FIELDS, grads, expect):
+        err = O.relative_error(g[idx], e)
+        assert err <= 1e-8, f"{name}: {err:.2e}"
+
     mask = torch.zeros(sc.means.shape[0], device=device, dtype=torch.float64)
     mask[idx] = 1.0
     dgen = torch.Generator(device=device).manual_seed(21)
@@ -143,24 +178,24 @@ def test_config1_fd_on_32_primitive_subset_fp64(device):
             im = render()
             for p, d in zip(group, direction):
                 p.add_(d, alpha=h)
+            # L(+) - L(-) = mean((img+ - img-)(img+ + img- - 2 target)): no cancellation of the loss itself
             return float(((ip - im) * (ip + im - 2 * target)).mean()) / (2 * h)
 
-    for group_ids, n_dirs in (((0, 1, 2, 3), 3), ((0,), 1), ((1,), 1), ((2,), 1), ((3,), 1)):
-        group = [params[k] for k in group_ids]
+    for k, h, tol, n_dirs in ((1, 1e-8, 1e-5, 2), (3, 1e-8, 1e-5, 2), (0, 1e-6, 5e-2, 1), (2, 1e-6, 5e-2, 1)):
         checked = 0
         for _ in range(4 * n_dirs):
-            direction = [torch.randn(p.shape, generator=dgen, device=device, dtype=p.dtype)
-                         * mask.reshape((-1,) + (1,) * (p.dim() - 1)) for p in group]
-            analytic = sum(float((grads[k] * d).sum()) for k, d in zip(group_ids, direction))
-            c1, c2 = central(group, direction, 1e-8), central(group, direction, 5e-9)
+            d = torch.randn(params[k].shape, generator=dgen, device=device, dtype=torch.float64)
+            d = d * mask.reshape((-1,) + (1,) * (params[k].dim() - 1))
+            analytic = float((grads[k] * d).sum())
+            c1, c2 = central([params[k]], [d], h), central([params[k]], [d], h / 2)
             scale = max(abs(analytic), abs(c1))
-            if abs(c1 - c2) > 1e-5 * scale:
+            if tol < 1e-3 and abs(c1 - c2) > 1e-5 * scale:
                 continue  # the step crossed a cut-off
-            assert abs(analytic - c2) <= 1e-5 * scale, (group_ids, analytic, c2)
+            assert abs(analytic - c2) <= tol * scale, (FIELDS[k], analytic, c1, c2)
             checked += 1
             if checked == n_dirs:
                 break
-        assert checked == n_dirs, f"group {group_ids}: only {checked} directions free of cut-off crossings"
+        assert checked == n_dirs, f"{FIELDS[k]}: only {checked} directions free of cut-off crossings"
 
 
 def test_autograd_equals_explicit_fused_float32(device):
@@ -202,8 +237,14 @@ def test_recompute_in_backward_equals_stored_lists(device, case):
     (o0, g0), (o1, g1) = results
     for a, b in zip(o0, o1):
         assert torch.equal(a, b)
+    # the fused backward is atomic-free (bitwise reproducible); the operator path sums with atomics
+    from oracle import splat as O
+
     for name, a, b in zip(names, g0, g1):
-        assert torch.equal(a, b), name
+        if case == "fused":
+            assert torch.equal(a, b), name
+        else:
+            assert O.relative_error(a, b) <= (1e-12 if dtype == torch.float64 else 1e-5), name
 
 
 def test_recompute_context_holds_no_intersection_lists(device):

@@ -59,7 +59,7 @@ def test_fused_matches_staged_ops_path(device):
 
 def test_fused_forward_is_deterministic_and_grows_capacity(device):
     sc = workloads.config0(device=device)
-    key = (device, 4, 1000, 64, 64)
+    key = (device, 4, 1000, 64, 64, 16)
     fused.CAPACITY._cap[key] = 1024  # force an overflow on the next call
     img1, f1 = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 64, 64, 1e-8, 3.0)
     assert f1.capacity >= f1.n_isects > 1024
@@ -134,3 +134,30 @@ def test_working_size_fused_agrees_with_staged_path(device):
     g2 = r2splat_amd.rasterize_bwd(ctx, (2.0 * vfn(img)).contiguous())
     for name in ("means", "quats", "scales", "densities"):
         assert torch.equal(getattr(g2, name), 2.0 * getattr(a, name)), name
+
+
+@pytest.mark.parametrize("tile,waves", [(8, 4), (8, 16), (32, 4), (32, 16), (16, 16)])
+def test_fused_tile_sizes_and_wave_counts_match_oracle(device, tile, waves):
+    W, H = 72, 56
+    sc = workloads.scene(700, 3, H, W, scale_range=(0.02, 0.1), seed=6, device=device)
+    img, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 3.0, tile_size=tile, waves=waves)
+    _, ids, _ = ops.isect_tiles(*_proj(sc), tile, -(-W // tile), -(-H // tile))
+    assert f.n_isects == ids.numel()
+    v = (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
+    grads = fused.backward(f, *sc.params(), sc.viewmats, sc.Ks, v, waves=waves)
+    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
+    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, W, H, 1e-8, 3.0)
+    mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), tile, W, H)
+    leaves = [t.detach().cpu().double().requires_grad_() for t in sc.params()]
+    ref = O.rasterize_scene(*leaves, sc.viewmats.cpu().double(), sc.Ks.cpu().double(), (H, W), eps=1e-8, mask=mask)
+    assert O.relative_error(img, ref) <= 1e-5
+    exp = torch.autograd.grad((ref * v.cpu().double()).sum(), leaves)
+    for name, g, e in zip(("means", "quats", "scales", "densities"), grads, exp):
+        err = O.relative_error(g, e)
+        assert err <= 1e-4, f"{name}: {err:.2e}"
+    # the backward does not depend on the wave count; the forward only through summation order
+    img8, f8 = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 3.0, tile_size=tile, waves=8)
+    assert O.relative_error(img8, img) <= 1e-6
+    grads8 = fused.backward(f8, *sc.params(), sc.viewmats, sc.Ks, v, waves=8)
+    for a, b in zip(grads, grads8):
+        assert torch.equal(a, b)
