@@ -34,6 +34,12 @@ FP32_PEAK_TFLOPS = 157.3
 # v_sigma 2, v_xy 12, v_conic 11, v_opacity 3, v_mu 3;
 # RasterizeToPixelsBwd.cu:430-468).
 FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
+# FP32 operations the fused kernels issue per pair (FMA = 2, exp = 1): the
+# forward shares the column terms of a lane's pixels (dy 1, two FMAs 4, exp 1,
+# select+FMA 2, plus the per-record column terms amortised over 4 pixels,
+# ~1); the backward reduces each record to six per-row moments (dx 1, two
+# FMAs 4, exp 1, v*e 1, three moment updates 4, per-row terms ~0.5).
+EXECUTED_FLOP_PER_PAIR = {"rasterize_fwd": 9.0, "rasterize_bwd": 11.5}
 
 
 def parse():
@@ -114,10 +120,14 @@ def main():
     dominant = max(kernel_stages, key=lambda k: kernel_stages[k][0])
     dom_ms = kernel_stages[dominant][0]
     achieved = pairs * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
+    executed = pairs * EXECUTED_FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
-                "avg_launch_ms": round(dom_ms, 4)}
+                "avg_launch_ms": round(dom_ms, 4),
+                "executed": {"flop_per_pair": EXECUTED_FLOP_PER_PAIR[dominant], "achieved": round(executed, 3),
+                             "frac": round(executed / FP32_PEAK_TFLOPS, 4),
+                             "note": "FP32 operations the kernel itself issues per pair (FMA = 2)"}}
 
     voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
 

@@ -311,7 +311,9 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
  * packed = capacity records of r2s_fused_record_bytes() each, tile_offsets
- * int32 [C, ceil(H/tile), ceil(W/tile)], record_offsets uint32 [C*N + 1]
+ * int32 [C, ceil(H/tile), ceil(W/tile)], tile_order uint32 [same count]
+ * (tiles by decreasing intersection count, the order the raster kernels
+ * walk them in; may be null, then natural order), record_offsets uint32 [C*N + 1]
  * (first intersection of each (camera, primitive) in emission order) and
  * counters uint32 [2] persist from the forward to the backward. counters[0]
  * = intersections, counters[1] = 1 when capacity was exceeded (that
@@ -319,15 +321,16 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * a larger capacity). isect_grads: float [capacity * 6], one VJP partial per
  * intersection, summed per primitive in a fixed order (the backward is
  * deterministic and uses no atomics). */
-size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity);
+size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
+                               uint32_t tile_size, uint32_t capacity);
 size_t r2s_fused_record_bytes(void);
 
 int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                      void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
-                      float* image, r2s_stream_t stream);
+                      void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                      uint32_t* counters, float* image, r2s_stream_t stream);
 
 /* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N]
  * (overwritten; camera sums in a fixed order). */
@@ -335,9 +338,10 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
                        const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                        uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                        uint32_t capacity, const void* packed, const int32_t* tile_offsets,
-                       const uint32_t* record_offsets, const uint32_t* counters, float* isect_grads,
-                       const float* v_image, float* v_means, float* v_quats, float* v_scales,
-                       float* v_densities, r2s_stream_t stream);
+                       const uint32_t* tile_order, const uint32_t* record_offsets,
+                       const uint32_t* counters, float* isect_grads, const float* v_image,
+                       float* v_means, float* v_quats, float* v_scales, float* v_densities,
+                       r2s_stream_t stream);
 
 /* The same two calls split into stages: forward = prepare (projection,
  * binning, sort, offsets, record packing) + render (rasterization);
@@ -348,13 +352,13 @@ int r2s_fused_prepare(const float* means, const float* quats, const float* scale
                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                      void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
-                      r2s_stream_t stream);
-int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                     uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
+                      void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                      uint32_t* counters, r2s_stream_t stream);
+int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
+                     const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
                      uint32_t waves, float* image, r2s_stream_t stream);
-int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                         uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
+                         const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
                          uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
                          r2s_stream_t stream);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,

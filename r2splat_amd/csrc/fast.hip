@@ -154,6 +154,27 @@ fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restri
     offsets[k] = int32_t(lo);
 }
 
+// Sort keys that order tiles by decreasing intersection count (in size classes
+// of 2^(1/8)): key = 255 - floor(8 log2(count + 1)). The raster kernels walk
+// tiles in that order (longest first), so the few heavy tiles of a view do
+// not all land on the same compute units at the end of the launch.
+__global__ void __launch_bounds__(256)
+fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n,
+                             uint32_t capacity, uint32_t n_tiles, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const uint32_t n_live = live(d_n, capacity);
+    int32_t start, end;
+    start = offsets[t];
+    end = (t + 1 == n_tiles) ? int32_t(n_live) : offsets[t + 1];
+    start = max(0, min(start, int32_t(n_live)));
+    end = max(start, min(end, int32_t(n_live)));
+    const float cls = floorf(8.f * __log2f(float(end - start) + 1.f));
+    keys[t] = 255u - min(255u, uint32_t(cls));
+    vals[t] = t;
+}
+
 // Workgroup b of a grid of 8k groups, renumbered so that the k consecutive
 // ranges of the sorted list land on the 8 XCDs one each (workgroups are
 // dealt to XCDs round-robin): each XCD's L2 then serves one band of cameras.
@@ -237,14 +258,15 @@ __device__ __forceinline__ void settle(const Splat& s) {
 template <int T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWave)
 fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
-                        const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
-                        uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H, float* __restrict__ out) {
+                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
+                        uint32_t H, float* __restrict__ out) {
     constexpr int PX = T * T / kWave, STEP = kWave / T;
     static_assert(PX * kWave == T * T && STEP * T == kWave, "tile must be 8, 16 or 32");
     __shared__ float part[WAVES][T * T];
     __shared__ Splat stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -305,12 +327,12 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 template <int T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWave)
 fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
-                        const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t n_tiles_total,
-                        uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
-                        const float* __restrict__ v_img, float* __restrict__ isect_grads) {
+                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
+                        uint32_t H, const float* __restrict__ v_img, float* __restrict__ isect_grads) {
     __shared__ __align__(16) float sv[T * T];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -454,12 +476,12 @@ fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restr
 // Forward scratch layout (released by the caller after the forward)
 // ---------------------------------------------------------------------------
 struct FusedScratch {
-    size_t recs, rects, scan, sort, sort_bytes, keys, vals, bytes;
+    size_t recs, rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, bytes;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
-static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t capacity) {
+static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity) {
     FusedScratch L;
     const size_t CN = size_t(C) * N;
     size_t o = 0;
@@ -470,6 +492,9 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t capacity) {
     L.sort = o; o += al(L.sort_bytes);
     L.keys = o; o += al(size_t(capacity) * 4 + 4);
     L.vals = o; o += al(size_t(capacity) * 4 + 4);
+    L.order_sort_bytes = sort_workspace_bytes(n_tiles);
+    L.order_sort = o; o += al(L.order_sort_bytes);
+    L.order_keys = o; o += al(size_t(n_tiles) * 4 + 4);
     L.bytes = o;
     return L;
 }
@@ -481,8 +506,13 @@ static T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<cha
 
 using namespace r2s;
 
-extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t capacity) {
-    return fused_scratch(C, N, capacity).bytes;
+extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
+                                          uint32_t tile_size, uint32_t capacity) {
+    if (tile_size == 0) return 0;
+    const uint64_t n_tiles = uint64_t(C) * ((width + tile_size - 1) / tile_size) *
+                             ((height + tile_size - 1) / tile_size);
+    if (n_tiles >= (1ull << 32)) return 0;
+    return fused_scratch(C, N, uint32_t(n_tiles), capacity).bytes;
 }
 
 extern "C" size_t r2s_fused_record_bytes(void) { return sizeof(Splat); }
@@ -518,20 +548,20 @@ static void launch_tiled(uint32_t tile, uint32_t waves, Args&&... args) {
 template <int T, int WAVES>
 struct LaunchFwd {
     static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
-                    const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
-                    float* image) {
-        fused_raster_fwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(packed, offsets, counters, capacity,
-                                                                                n_tiles, ntx, nty, W, H, image);
+                    const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
+                    uint32_t W, uint32_t H, float* image) {
+        fused_raster_fwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            packed, offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, image);
     }
 };
 
 template <int T, int WAVES>
 struct LaunchBwd {
     static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
-                    const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty, uint32_t W, uint32_t H,
-                    const float* v_image, float* isect_grads) {
+                    const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
+                    uint32_t W, uint32_t H, const float* v_image, float* isect_grads) {
         fused_raster_bwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            packed, offsets, counters, capacity, n_tiles, ntx, nty, W, H, v_image, isect_grads);
+            packed, offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, v_image, isect_grads);
     }
 };
 
@@ -539,12 +569,12 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                                  uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                                  float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                                 void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
-                                 r2s_stream_t stream) {
+                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                                 uint32_t* counters, r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, 0)) return rc;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
-    const FusedScratch L = fused_scratch(C, N, capacity);
+    const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity);
     R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
@@ -573,6 +603,15 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, counters, capacity, n_tiles,
                                                                      tile_offsets);
     if (int rc = check_launch("fused_offsets")) return rc;
+    if (tile_order) {
+        void* order_ws = at<void>(scratch, L.order_sort);
+        fused_tile_order_keys_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(
+            tile_offsets, counters, capacity, n_tiles, sort_keys_buffer(order_ws), sort_vals_buffer(order_ws, n_tiles));
+        if (int rc = check_launch("fused_tile_order")) return rc;
+        if (int rc = sort_pairs_u32(order_ws, L.order_sort_bytes, n_tiles, nullptr, 8, at<uint32_t>(scratch, L.order_keys),
+                                    tile_order, stream))
+            return rc;
+    }
     const uint32_t pack_blocks = (ceil_div(capacity, 256) + 7u) & ~7u;  // a multiple of the 8 XCDs
     fused_pack_kernel<<<pack_blocks, 256, 0, stream>>>(at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, N,
                                                        ntx, ntx * nty, keys, vals, counters, capacity,
@@ -580,8 +619,8 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     return check_launch("fused_pack");
 }
 
-extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                                uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
+extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
+                                const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
                                 uint32_t waves, float* image, r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
@@ -589,12 +628,12 @@ extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets,
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
     launch_tiled<LaunchFwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
-                            tile_offsets, counters, capacity, ntx, nty, width, height, image);
+                            tile_offsets, tile_order, counters, capacity, ntx, nty, width, height, image);
     return check_launch("fused_raster_fwd");
 }
 
-extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
-                                    uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
+                                    const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
                                     uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
                                     r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
@@ -603,7 +642,8 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
     launch_tiled<LaunchBwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
-                            tile_offsets, counters, capacity, ntx, nty, width, height, v_image, isect_grads);
+                            tile_offsets, tile_order, counters, capacity, ntx, nty, width, height, v_image,
+                            isect_grads);
     return check_launch("fused_raster_bwd");
 }
 
@@ -625,25 +665,25 @@ extern "C" int r2s_fused_forward(const float* means, const float* quats, const f
                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                                  uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                                  float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                                 void* packed, int32_t* tile_offsets, uint32_t* record_offsets, uint32_t* counters,
-                                 float* image, r2s_stream_t stream) {
+                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                                 uint32_t* counters, float* image, r2s_stream_t stream) {
     if (int rc = r2s_fused_prepare(means, quats, scales, densities, viewmats, Ks, C, N, width, height, tile_size,
                                    eps, radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
-                                   record_offsets, counters, stream))
+                                   tile_order, record_offsets, counters, stream))
         return rc;
-    return r2s_fused_render(packed, tile_offsets, counters, capacity, C, width, height, tile_size, 0, image,
-                            stream);
+    return r2s_fused_render(packed, tile_offsets, tile_order, counters, capacity, C, width, height, tile_size, 0,
+                            image, stream);
 }
 
 extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                                   const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                                   uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                                   uint32_t capacity, const void* packed, const int32_t* tile_offsets,
-                                  const uint32_t* record_offsets, const uint32_t* counters, float* isect_grads,
-                                  const float* v_image, float* v_means, float* v_quats, float* v_scales,
-                                  float* v_densities, r2s_stream_t stream) {
-    if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, counters, capacity, C, width, height, tile_size, 0,
-                                      v_image, isect_grads, stream))
+                                  const uint32_t* tile_order, const uint32_t* record_offsets,
+                                  const uint32_t* counters, float* isect_grads, const float* v_image, float* v_means,
+                                  float* v_quats, float* v_scales, float* v_densities, r2s_stream_t stream) {
+    if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, tile_order, counters, capacity, C, width, height,
+                                      tile_size, 0, v_image, isect_grads, stream))
         return rc;
     return r2s_fused_param_vjp(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
                                record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,

@@ -57,6 +57,7 @@ LAST_COUNTERS = None
 class FusedForward:
     packed: torch.Tensor
     offsets: torch.Tensor
+    tile_order: torch.Tensor
     record_offsets: torch.Tensor
     counters: torch.Tensor
     capacity: int
@@ -67,8 +68,8 @@ class FusedForward:
     tile_size: int = TILE
 
 
-def _scratch(C, N, capacity, device):
-    nbytes = be.lib().r2s_fused_scratch_bytes(C, N, capacity)
+def _scratch(C, N, width, height, tile_size, capacity, device):
+    nbytes = be.lib().r2s_fused_scratch_bytes(C, N, width, height, tile_size, capacity)
     return torch.empty((nbytes,), dtype=torch.uint8, device=device)
 
 
@@ -93,18 +94,20 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     ntx, nty = -(-width // tile_size), -(-height // tile_size)
     rec = be.lib().r2s_fused_record_bytes()
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
+    tile_order = torch.empty((C * nty * ntx,), dtype=torch.int32, device=device)
     record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
     counters = torch.zeros((2,), dtype=torch.int32, device=device)
     with torch.cuda.device(device):
         stream = be.stream_of(device)
         n_isects = -1
         while True:
-            scratch = _scratch(C, N, cap, device)
+            scratch = _scratch(C, N, width, height, tile_size, cap, device)
             packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
             with stage("project_bin"):
                 call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
                      ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
-                     scratch.numel(), ptr(packed), ptr(offsets), ptr(record_offsets), ptr(counters), stream)
+                     scratch.numel(), ptr(packed), ptr(offsets), ptr(tile_order), ptr(record_offsets), ptr(counters),
+                     stream)
             del scratch
             if not check:
                 break
@@ -115,8 +118,8 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
             cap = CAPACITY.grow(key, n_isects)
     global LAST_COUNTERS
     LAST_COUNTERS = counters
-    return FusedForward(packed, offsets, record_offsets, counters, cap, n_isects, width, height, float(eps),
-                        tile_size)
+    return FusedForward(packed, offsets, tile_order, record_offsets, counters, cap, n_isects, width, height,
+                        float(eps), tile_size)
 
 
 def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
@@ -124,7 +127,8 @@ def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
     image = torch.empty((C, fwd.height, fwd.width), dtype=torch.float32, device=device)
     with torch.cuda.device(device):
         with stage("rasterize_fwd"):
-            call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
+            call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
+                 fwd.capacity, C,
                  fwd.width, fwd.height, fwd.tile_size, waves, ptr(image), be.stream_of(device))
     return image
 
@@ -149,7 +153,8 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     with torch.cuda.device(device):
         stream = be.stream_of(device)
         with stage("rasterize_bwd"):
-            call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.counters), fwd.capacity, C,
+            call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
+                 fwd.capacity, C,
                  fwd.width, fwd.height, fwd.tile_size, waves, ptr(v_image), ptr(grad), stream)
         with stage("param_bwd"):
             call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),

@@ -85,6 +85,7 @@ class GaussianFit:
         self.loss = torch.zeros((1,), dtype=torch.float32, device=device)
         ntx, nty = -(-self.W // self.tile_size), -(-self.H // self.tile_size)
         self.tile_offsets = torch.empty((self.C, nty, ntx), dtype=torch.int32, device=device)
+        self.tile_order = torch.empty((self.C * nty * ntx,), dtype=torch.int32, device=device)
         self.record_offsets = torch.empty((self.C * N + 1,), dtype=torch.int32, device=device)
         self.counters = torch.zeros((2,), dtype=torch.int32, device=device)
         self.graph = None
@@ -100,8 +101,8 @@ class GaussianFit:
     def _alloc(self, capacity: int):
         self.capacity = max(1024, int(capacity))
         lib = be.lib()
-        self.scratch = torch.empty((lib.r2s_fused_scratch_bytes(self.C, self.N, self.capacity),), dtype=torch.uint8,
-                                   device=self.device)
+        nbytes = lib.r2s_fused_scratch_bytes(self.C, self.N, self.W, self.H, self.tile_size, self.capacity)
+        self.scratch = torch.empty((nbytes,), dtype=torch.uint8, device=self.device)
         self.packed = torch.empty((self.capacity * lib.r2s_fused_record_bytes(),), dtype=torch.uint8,
                                   device=self.device)
         self.isect_grads = torch.empty((self.capacity * 6,), dtype=torch.float32, device=self.device)
@@ -115,7 +116,7 @@ class GaussianFit:
     def memory_bytes(self) -> int:
         """Device bytes held by this job (parameters, state and intermediates)."""
         tensors = [self.params, self.grads, self.image, self.v_image, self.loss_partials, self.tile_offsets,
-                   self.record_offsets, self.scratch, self.packed, self.isect_grads, self.target]
+                   self.tile_order, self.record_offsets, self.scratch, self.packed, self.isect_grads, self.target]
         if self.adam is not None:
             tensors += [self.exp_avg, self.exp_avg_sq]
         return sum(t.numel() * t.element_size() for t in tensors)
@@ -127,14 +128,15 @@ class GaussianFit:
         C, N, W, H = self.C, self.N, self.W, self.H
         call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size, self.eps, self.radius_clip, self.capacity,
-             ptr(self.scratch), self.scratch.numel(), ptr(self.packed), ptr(self.tile_offsets),
+             ptr(self.scratch), self.scratch.numel(), ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.record_offsets), ptr(self.counters), s)
-        call("r2s_fused_render", ptr(self.packed), ptr(self.tile_offsets), ptr(self.counters), self.capacity, C, W,
-             H, self.tile_size, self.fwd_waves, ptr(self.image), s)
+        call("r2s_fused_render", ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
+             self.capacity, C, W, H, self.tile_size, self.fwd_waves, ptr(self.image), s)
         call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
              ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
-        call("r2s_fused_raster_vjp", ptr(self.packed), ptr(self.tile_offsets), ptr(self.counters), self.capacity, C,
-             W, H, self.tile_size, self.bwd_waves, ptr(self.v_image), ptr(self.isect_grads), s)
+        call("r2s_fused_raster_vjp", ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order),
+             ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, ptr(self.v_image),
+             ptr(self.isect_grads), s)
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.eps, ptr(self.record_offsets), ptr(self.counters),
              self.capacity, ptr(self.isect_grads), ptr(self.v_means), ptr(self.v_quats), ptr(self.v_scales),
