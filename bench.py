@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
     p.add_argument("--no-graph", action="store_true", help="time the eager step instead of its hipGraph replay")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run one plain 256 MiB device copy (reference kernel for HBM byte counters)")
     return p.parse_args()
 
 
@@ -69,6 +71,12 @@ def main():
     from r2splat_amd import profiling, workloads
 
     device = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones((64 << 20,), dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc = workloads.config1(device=device, dtype=torch.float32, seed=0)
     tgt_scene = workloads.config1(device=device, dtype=torch.float32, seed=1)
     with torch.no_grad():

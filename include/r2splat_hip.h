@@ -310,8 +310,10 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * per-pixel summation order does (rounding only).
  *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
- * packed = capacity records of r2s_fused_record_bytes() each, tile_offsets
- * int32 [C, ceil(H/tile), ceil(W/tile)], tile_order uint32 [same count]
+ * packed = capacity records of r2s_fused_record_bytes() each, in the
+ * reference's list order (camera, tile, primitive); tile_offsets int32
+ * [C, ceil(H/tile), ceil(W/tile)] (the reference's layout); tile_order
+ * uint32 [same count]
  * (tiles by decreasing intersection count, the order the raster kernels
  * walk them in; may be null, then natural order), record_offsets uint32 [C*N + 1]
  * (first intersection of each (camera, primitive) in emission order) and
@@ -339,9 +341,9 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
                        uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                        uint32_t capacity, const void* packed, const int32_t* tile_offsets,
                        const uint32_t* tile_order, const uint32_t* record_offsets,
-                       const uint32_t* counters, float* isect_grads, const float* v_image,
-                       float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                       r2s_stream_t stream);
+                       const uint32_t* counters, float* isect_grads, void* workspace,
+                       size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
+                       float* v_scales, float* v_densities, r2s_stream_t stream);
 
 /* The same two calls split into stages: forward = prepare (projection,
  * binning, sort, offsets, record packing) + render (rasterization);
@@ -361,12 +363,16 @@ int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const 
                          const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
                          uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
                          r2s_stream_t stream);
+/* workspace: r2s_fused_param_workspace_bytes(C, N) bytes (per-camera
+ * contributions, summed over cameras in order in a second pass). */
+size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                         const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                         uint32_t N, uint32_t width, uint32_t height, float eps,
                         const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
-                        const float* isect_grads, float* v_means, float* v_quats, float* v_scales,
-                        float* v_densities, r2s_stream_t stream);
+                        const float* isect_grads, void* workspace, size_t workspace_bytes,
+                        float* v_means, float* v_quats, float* v_scales, float* v_densities,
+                        r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Training-step kernels (no counterpart in the reference, which leaves image */

@@ -242,18 +242,32 @@ __device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* d_n) 
     return d_n ? ((*d_n <= n) ? *d_n : 0u) : n;
 }
 
+// Per-block digit histogram. Lanes of a wave holding the same digit are
+// grouped with ballots and counted once by their lowest lane, so runs of equal
+// keys (common in emission order) cost one LDS atomic per run, not per key.
 __global__ void __launch_bounds__(kSortThreads)
 radix_upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ d_n,
-                     uint32_t shift, uint32_t mask, uint32_t n_blocks, uint32_t* __restrict__ hist) {
+                     uint32_t shift, uint32_t nbits, uint32_t n_blocks, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
     n = live_count(n, d_n);
     h[threadIdx.x] = 0;
     __syncthreads();
+    const uint32_t mask = (1u << nbits) - 1u;
+    const int lane = lane_id();
     const size_t base = size_t(blockIdx.x) * kSortTile;
-#pragma unroll 4
     for (int k = 0; k < kSortRounds; ++k) {
         const size_t i = base + size_t(k) * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+        const bool valid = i < n;
+        const uint32_t d = valid ? (keys[i] >> shift) & mask : 0u;
+        uint64_t peers = __ballot(valid);
+        if (!peers) continue;  // wave-uniform
+        for (uint32_t b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int leader = __ffsll((unsigned long long)peers) - 1;
+        if (valid && lane == leader) atomicAdd(&h[d], uint32_t(__popcll(peers)));
     }
     __syncthreads();
     hist[size_t(threadIdx.x) * n_blocks + blockIdx.x] = h[threadIdx.x];
@@ -390,8 +404,7 @@ static int sort_pairs_impl(void* ws, size_t ws_bytes, uint32_t n, const uint32_t
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = p * kRadixBits;
         const uint32_t nb = std::min<uint32_t>(kRadixBits, bits - shift);
-        radix_upsweep_kernel<<<L.n_blocks, kSortThreads, 0, stream>>>(ka, n, d_n, shift, (1u << nb) - 1u,
-                                                                       L.n_blocks, hist);
+        radix_upsweep_kernel<<<L.n_blocks, kSortThreads, 0, stream>>>(ka, n, d_n, shift, nb, L.n_blocks, hist);
         if (int rc = check_launch("radix_upsweep")) return rc;
         if (int rc = scan_exclusive(hist, hist_n, hist, nullptr, scan_ws, stream)) return rc;
         if (p + 1 == passes) {

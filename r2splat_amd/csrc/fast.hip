@@ -113,9 +113,12 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     counts[idx] = count;
 }
 
-// Records (key = cam * n_tiles + tile, value = cam * N + n) at their scanned
-// positions; entries that would pass the capacity raise the overflow flag.
-// The projected record's g field is set to its first emission index.
+// Records (key = tile within the camera, ty * ntx + tx; value = cam * N + n)
+// at their scanned positions, which are camera-major; entries that would pass
+// the capacity raise the overflow flag. A stable sort on the key alone then
+// orders the list by (tile, camera, primitive): one radix pass for up to 256
+// tiles per camera. The projected record's g field is set to its first
+// emission index.
 __global__ void __launch_bounds__(256)
 fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ cum, uint32_t total,
                   uint32_t N, uint32_t ntx, uint32_t nt, uint32_t capacity, uint32_t* __restrict__ keys,
@@ -131,27 +134,41 @@ fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ 
     }
     const uint2 r = rects[i];
     const uint32_t x0 = r.x & 0xffffu, x1 = r.x >> 16, y0 = r.y & 0xffffu, y1 = r.y >> 16;
-    const uint32_t base = (i / N) * nt;
     uint32_t w = w0;
     for (uint32_t y = y0; y < y1; ++y)
         for (uint32_t x = x0; x < x1; ++x, ++w) {
-            keys[w] = base + y * ntx + x;
+            keys[w] = y * ntx + x;
             vals[w] = i;
         }
 }
 
+// Starts of the sorted (tile-major) lists: offsets[k] for list k = tile * C +
+// cam is the first sorted intersection whose (tile, camera) is not below it
+// (lower_bound on tile * C + value / N).
 __global__ void __launch_bounds__(256)
-fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_n, uint32_t capacity,
-                     uint32_t n_keys, int32_t* __restrict__ offsets) {
+fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                     const uint32_t* __restrict__ d_n, uint32_t capacity, uint32_t C, uint32_t N,
+                     uint32_t n_lists, int32_t* __restrict__ offsets) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_keys) return;
+    if (k >= n_lists) return;
     const uint32_t n = live(d_n, capacity);
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < k) lo = mid + 1; else hi = mid;
+        const uint64_t probe = uint64_t(keys[mid]) * C + vals[mid] / N;
+        if (probe < k) lo = mid + 1; else hi = mid;
     }
     offsets[k] = int32_t(lo);
+}
+
+// Tile (cam * nt + t) -> [start, end) of its packed list (camera-major
+// offsets, the reference's layout). Wave-uniform.
+__device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, uint32_t tile, uint32_t n_tiles,
+                                           uint32_t n_live, int32_t& start, int32_t& end) {
+    start = offsets[tile];
+    end = (tile + 1 == n_tiles) ? int32_t(n_live) : offsets[tile + 1];
+    start = max(0, min(start, int32_t(n_live)));
+    end = max(start, min(end, int32_t(n_live)));
 }
 
 // Sort keys that order tiles by decreasing intersection count (in size classes
@@ -160,16 +177,13 @@ fused_offsets_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restri
 // not all land on the same compute units at the end of the launch.
 __global__ void __launch_bounds__(256)
 fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n,
-                             uint32_t capacity, uint32_t n_tiles, uint32_t* __restrict__ keys,
+                             uint32_t capacity, uint32_t C, uint32_t nt, uint32_t* __restrict__ keys,
                              uint32_t* __restrict__ vals) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tiles) return;
+    if (t >= C * nt) return;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
-    start = offsets[t];
-    end = (t + 1 == n_tiles) ? int32_t(n_live) : offsets[t + 1];
-    start = max(0, min(start, int32_t(n_live)));
-    end = max(start, min(end, int32_t(n_live)));
+    fused_tile(offsets, t, C * nt, n_live, start, end);
     const float cls = floorf(8.f * __log2f(float(end - start) + 1.f));
     keys[t] = 255u - min(255u, uint32_t(cls));
     vals[t] = t;
@@ -180,34 +194,65 @@ fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t
 // dealt to XCDs round-robin): each XCD's L2 then serves one band of cameras.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) { return (b % 8u) * (nb / 8u) + b / 8u; }
 
-// packed[i] = record of sorted intersection i, with g = its emission index
-// w0 + (ty - y0) (x1 - x0) + (tx - x0) (row-major within the record's tile
-// rectangle, the order fused_emit_kernel wrote it in; w0 is the record's g).
+// Camera-major list k = cam * nt + t: its length in the tile-major sorted
+// arrays (starts tm[t * C + cam]).
 __global__ void __launch_bounds__(256)
-fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t N,
-                  uint32_t ntx, uint32_t nt, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                  const uint32_t* __restrict__ d_n, uint32_t capacity, Splat* __restrict__ packed) {
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= live(d_n, capacity)) return;
-    const uint32_t v = vals[i];
-    if (v >= n_recs) return;
-    const uint32_t t = keys[i] - (v / N) * nt;
-    const uint32_t ty = t / ntx, tx = t - ty * ntx;
-    const uint2 r = rects[v];
-    const uint32_t x0 = r.x & 0xffffu, x1 = r.x >> 16, y0 = r.y & 0xffffu;
-    Splat rec = recs[v];
-    rec.g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
-    packed[i] = rec;
+fused_list_lengths_kernel(const uint32_t* __restrict__ tm, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                          uint32_t C, uint32_t nt, uint32_t* __restrict__ lengths) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= C * nt) return;
+    const uint32_t n_live = live(d_n, capacity);
+    const uint32_t cam = k / nt, t = k - cam * nt, j = t * C + cam;
+    const uint32_t start = min(tm[j], n_live);
+    const uint32_t end = (j + 1 == C * nt) ? n_live : min(tm[j + 1], n_live);
+    lengths[k] = end > start ? end - start : 0u;
 }
 
-// Tile -> [start, end) of its packed list. Every quantity is wave-uniform.
-__device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, uint32_t tile,
-                                           uint32_t n_tiles_total, uint32_t n_live, int32_t& start,
-                                           int32_t& end) {
-    start = offsets[tile];
-    end = (tile + 1 == n_tiles_total) ? int32_t(n_live) : offsets[tile + 1];
-    start = max(0, min(start, int32_t(n_live)));
-    end = max(start, min(end, int32_t(n_live)));
+// One workgroup per camera-major list k = cam * nt + t (the lists of one XCD
+// are a contiguous band of cameras): packed[cm[k] + r] = record of the r-th
+// intersection of the list, with g = its emission index w0 + (ty - y0)
+// (x1 - x0) + (tx - x0) (row-major within the record's tile rectangle, the
+// order fused_emit_kernel wrote it in; w0 is the record's g). The packed list
+// is camera-major, tile, then primitive: the reference's order.
+constexpr int kPackItems = 4;
+
+__global__ void __launch_bounds__(256)
+fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t C,
+                  uint32_t nt, uint32_t ntx, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tm,
+                  const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                  Splat* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x);
+    if (k >= C * nt) return;
+    const uint32_t n_live = live(d_n, capacity);
+    const uint32_t cam = k / nt, t = k - cam * nt;
+    const uint32_t dst = cm[k], len = cm[k + 1] - dst, src = tm[t * C + cam];
+    if (threadIdx.x == 0) tile_offsets[k] = int32_t(dst);
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    for (uint32_t r0 = 0; r0 < len; r0 += blockDim.x * kPackItems) {
+        uint32_t v[kPackItems];
+#pragma unroll
+        for (int q = 0; q < kPackItems; ++q) {
+            const uint32_t r = r0 + q * blockDim.x + threadIdx.x;
+            v[q] = (r < len && src + r < n_live) ? vals[src + r] : n_recs;
+        }
+        Splat rec[kPackItems];
+        uint2 rc[kPackItems];
+#pragma unroll
+        for (int q = 0; q < kPackItems; ++q) {
+            if (v[q] < n_recs) {
+                rec[q] = recs[v[q]];
+                rc[q] = rects[v[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPackItems; ++q) {
+            const uint32_t r = r0 + q * blockDim.x + threadIdx.x;
+            if (v[q] >= n_recs || dst + r >= capacity) continue;
+            const uint32_t x0 = rc[q].x & 0xffffu, x1 = rc[q].x >> 16, y0 = rc[q].y & 0xffffu;
+            rec[q].g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
+            packed[dst + r] = rec[q];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -401,82 +446,106 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Per-primitive VJP chain, summed over cameras in order (deterministic): the
-// (camera, primitive)'s intersection partials [cum[v], cum[v+1]) summed in
-// emission order -> conic/mu VJP (double) -> projection VJP -> parameter-
-// transform VJP. A partial is (v_mx, v_my, v_a, v_b, v_c, v_w) with
-// w = density * mu.
+// Parameter VJP in two deterministic passes.
+// Pass 1, one thread per (camera, primitive): the record's intersection
+// partials [cum[v], cum[v+1]) summed in emission order -> conic/mu VJP
+// (double) -> projection VJP, giving this camera's contribution to v_mean (3),
+// v_covariance (6) and v_density (1), stored field-major ws[f][c][n].
+// Pass 2, one thread per primitive: the contributions summed over cameras in
+// order -> parameter-transform VJP. A partial is (v_mx, v_my, v_a, v_b, v_c,
+// v_w) with w = density * mu.
 // ---------------------------------------------------------------------------
+constexpr int kParamFields = 10;
+
 __global__ void __launch_bounds__(256)
-fused_primitive_bwd_kernel(const float* __restrict__ means, const float* __restrict__ quats,
-                           const float* __restrict__ scales, const float* __restrict__ dens,
-                           const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
-                           uint32_t H, float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
-                           const uint32_t* __restrict__ d_n, uint32_t capacity,
-                           const float* __restrict__ isect_grads,
-                           float* __restrict__ v_means, float* __restrict__ v_quats,
-                           float* __restrict__ v_scales, float* __restrict__ v_dens) {
+fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict__ quats,
+                        const float* __restrict__ scales, const float* __restrict__ dens,
+                        const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W, uint32_t H,
+                        float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
+                        const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
+                        float* __restrict__ ws) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.y;
     if (n >= N) return;
-    float q[4], mag, norm, R[9], cov[6];
-    unit_quat(quats + 4 * size_t(n), q, mag, norm);
-    quat_to_rot(q, R);
-    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
-    covariance_from(R, s, cov);
-    const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
-    const float opac = dens[n];
-    float vp[3] = {0.f, 0.f, 0.f}, vcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vd = 0.f;
-    const float Wf = float(W), Hf = float(H);
+    const size_t v = size_t(c) * N + n;
     const uint32_t n_live = live(d_n, capacity);
-    for (uint32_t c = 0; c < C; ++c) {
-        const size_t v = size_t(c) * N + n;
-        const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
-        float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
-        for (uint32_t j = lo; j < hi; ++j) {
-            const float2* gp = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(j));
-            const float2 a = gp[0], b = gp[1], d = gp[2];
-            g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y; g4 += d.x; g5 += d.y;
-        }
-        if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f) continue;
+    const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
+    for (uint32_t j = lo; j < hi; ++j) {
+        const float2* gp = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(j));
+        const float2 a = gp[0], b = gp[1], d = gp[2];
+        g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y; g4 += d.x; g5 += d.y;
+    }
+    float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (g0 != 0.f || g1 != 0.f || g2 != 0.f || g3 != 0.f || g4 != 0.f || g5 != 0.f) {
+        float q[4], mag, norm, R[9], cov[6];
+        unit_quat(quats + 4 * size_t(n), q, mag, norm);
+        quat_to_rot(q, R);
+        const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
+        covariance_from(R, s, cov);
+        const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
         Camera<float> cam;
         cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
         ProjState<float> st;
-        if (!project_state(cam, p, cov, Wf, Hf, st)) continue;
-        float c2[6];
-        projected_cov(st, c2);
-        double cd[6], P[3], mu;
+        if (project_state(cam, p, cov, float(W), float(H), st)) {
+            float c2[6];
+            projected_cov(st, c2);
+            double cd[6], P[3], mu;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
-        conic_mu_f64(cd, double(eps), P, mu);
-        // the forward stored the conic and mu in float32
-        double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
-        const double muf = double(float(mu));
-        vd += float(muf) * g5;
-        const double vP[3] = {double(g2), double(g3), double(g4)};
-        double gcov[6];
-        conic_mu_vjp_f64(cd, Pf, muf, vP, double(opac) * double(g5), gcov);
-        const float vm[2] = {g0, g1};
-        float vc[6];
+            for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
+            conic_mu_f64(cd, double(eps), P, mu);
+            // the forward stored the conic and mu in float32
+            const double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
+            const double muf = double(float(mu));
+            out[9] = float(muf) * g5;
+            const double vP[3] = {double(g2), double(g3), double(g4)};
+            double gcov[6];
+            conic_mu_vjp_f64(cd, Pf, muf, vP, double(dens[n]) * double(g5), gcov);
+            const float vm[2] = {g0, g1};
+            float vc[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) vc[k] = float(gcov[k]);
-        projection_vjp(cam, st, vm, vc, vp, vcov);
+            for (int k = 0; k < 6; ++k) vc[k] = float(gcov[k]);
+            projection_vjp(cam, st, vm, vc, out, out + 3);
+        }
     }
+    const size_t CN = size_t(C) * N;
+#pragma unroll
+    for (int f = 0; f < kParamFields; ++f) ws[f * CN + v] = out[f];
+}
+
+__global__ void __launch_bounds__(256)
+fused_param_reduce_kernel(const float* __restrict__ quats, const float* __restrict__ scales, uint32_t C,
+                          uint32_t N, const float* __restrict__ ws, float* __restrict__ v_means,
+                          float* __restrict__ v_quats, float* __restrict__ v_scales, float* __restrict__ v_dens) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const size_t CN = size_t(C) * N;
+    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t c = 0; c < C; ++c) {
+#pragma unroll
+        for (int f = 0; f < kParamFields; ++f) acc[f] += ws[f * CN + size_t(c) * N + n];
+    }
+    float q[4], mag, norm, R[9];
+    unit_quat(quats + 4 * size_t(n), q, mag, norm);
+    quat_to_rot(q, R);
+    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     float gq[4], vs[3];
-    covariance_vjp(q, R, s, vcov, gq, vs);
+    covariance_vjp(q, R, s, acc + 3, gq, vs);
     normalize_vjp(q, mag, norm, gq, v_quats + 4 * size_t(n));
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v_means[3 * size_t(n) + k] = vp[k];
+        v_means[3 * size_t(n) + k] = acc[k];
         v_scales[3 * size_t(n) + k] = vs[k];
     }
-    v_dens[n] = vd;
+    v_dens[n] = acc[9];
 }
 
 // ---------------------------------------------------------------------------
 // Forward scratch layout (released by the caller after the forward)
 // ---------------------------------------------------------------------------
 struct FusedScratch {
-    size_t recs, rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, bytes;
+    size_t recs, rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, tm, lengths, cm,
+        tile_scan, bytes;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -495,6 +564,10 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
     L.order_sort_bytes = sort_workspace_bytes(n_tiles);
     L.order_sort = o; o += al(L.order_sort_bytes);
     L.order_keys = o; o += al(size_t(n_tiles) * 4 + 4);
+    L.tm = o; o += al(size_t(n_tiles) * 4 + 4);
+    L.lengths = o; o += al(size_t(n_tiles) * 4 + 4);
+    L.cm = o; o += al(size_t(n_tiles) * 4 + 8);
+    L.tile_scan = o; o += al(scan_workspace_bytes(n_tiles));
     L.bytes = o;
     return L;
 }
@@ -597,26 +670,36 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     }
     uint32_t* keys = at<uint32_t>(scratch, L.keys);
     uint32_t* vals = at<uint32_t>(scratch, L.vals);
-    if (int rc = sort_pairs_u32(sort_ws, L.sort_bytes, capacity, counters, key_end_bit(n_tiles), keys, vals,
+    if (int rc = sort_pairs_u32(sort_ws, L.sort_bytes, capacity, counters, key_end_bit(ntx * nty - 1), keys, vals,
                                 stream))
         return rc;
-    fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, counters, capacity, n_tiles,
-                                                                     tile_offsets);
+    uint32_t* tm = at<uint32_t>(scratch, L.tm);
+    uint32_t* cm = at<uint32_t>(scratch, L.cm);
+    fused_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(keys, vals, counters, capacity, C, N, n_tiles,
+                                                                     reinterpret_cast<int32_t*>(tm));
     if (int rc = check_launch("fused_offsets")) return rc;
+    fused_list_lengths_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(tm, counters, capacity, C, ntx * nty,
+                                                                          at<uint32_t>(scratch, L.lengths));
+    if (int rc = check_launch("fused_list_lengths")) return rc;
+    if (int rc = scan_exclusive(at<uint32_t>(scratch, L.lengths), n_tiles, cm, nullptr, at<void>(scratch, L.tile_scan),
+                                stream))
+        return rc;
+    const uint32_t pack_blocks = (n_tiles + 7u) & ~7u;  // a multiple of the 8 XCDs
+    fused_pack_kernel<<<pack_blocks, 256, 0, stream>>>(at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C,
+                                                       ntx * nty, ntx, vals, tm, cm, counters, capacity,
+                                                       static_cast<Splat*>(packed), tile_offsets);
+    if (int rc = check_launch("fused_pack")) return rc;
     if (tile_order) {
         void* order_ws = at<void>(scratch, L.order_sort);
         fused_tile_order_keys_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(
-            tile_offsets, counters, capacity, n_tiles, sort_keys_buffer(order_ws), sort_vals_buffer(order_ws, n_tiles));
+            tile_offsets, counters, capacity, C, ntx * nty, sort_keys_buffer(order_ws),
+            sort_vals_buffer(order_ws, n_tiles));
         if (int rc = check_launch("fused_tile_order")) return rc;
         if (int rc = sort_pairs_u32(order_ws, L.order_sort_bytes, n_tiles, nullptr, 8, at<uint32_t>(scratch, L.order_keys),
                                     tile_order, stream))
             return rc;
     }
-    const uint32_t pack_blocks = (ceil_div(capacity, 256) + 7u) & ~7u;  // a multiple of the 8 XCDs
-    fused_pack_kernel<<<pack_blocks, 256, 0, stream>>>(at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, N,
-                                                       ntx, ntx * nty, keys, vals, counters, capacity,
-                                                       static_cast<Splat*>(packed));
-    return check_launch("fused_pack");
+    return R2S_OK;
 }
 
 extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
@@ -647,18 +730,29 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     return check_launch("fused_raster_bwd");
 }
 
+extern "C" size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N) {
+    return size_t(kParamFields) * C * N * sizeof(float);
+}
+
 extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                                    const float* densities, const float* viewmats, const float* Ks, uint32_t C,
                                    uint32_t N, uint32_t width, uint32_t height, float eps,
                                    const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
-                                   const float* isect_grads, float* v_means, float* v_quats, float* v_scales,
-                                   float* v_densities, r2s_stream_t stream) {
+                                   const float* isect_grads, void* workspace, size_t workspace_bytes,
+                                   float* v_means, float* v_quats, float* v_scales, float* v_densities,
+                                   r2s_stream_t stream) {
     if (N == 0) return R2S_OK;
     if (int rc = fused_check(C, N, 1, 1, capacity)) return rc;
-    fused_primitive_bwd_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
+    R2S_REQUIRE(workspace_bytes >= r2s_fused_param_workspace_bytes(C, N), "param workspace too small: %zu < %zu",
+                workspace_bytes, r2s_fused_param_workspace_bytes(C, N));
+    float* ws = static_cast<float*>(workspace);
+    fused_camera_vjp_kernel<<<dim3(ceil_div(N, 256), C), 256, 0, stream>>>(
         means, quats, scales, densities, viewmats, Ks, width, height, eps, C, N, record_offsets, counters, capacity,
-        isect_grads, v_means, v_quats, v_scales, v_densities);
-    return check_launch("fused_primitive_bwd");
+        isect_grads, ws);
+    if (int rc = check_launch("fused_camera_vjp")) return rc;
+    fused_param_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(quats, scales, C, N, ws, v_means, v_quats,
+                                                                    v_scales, v_densities);
+    return check_launch("fused_param_reduce");
 }
 
 extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
@@ -680,12 +774,13 @@ extern "C" int r2s_fused_backward(const float* means, const float* quats, const 
                                   uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                                   uint32_t capacity, const void* packed, const int32_t* tile_offsets,
                                   const uint32_t* tile_order, const uint32_t* record_offsets,
-                                  const uint32_t* counters, float* isect_grads, const float* v_image, float* v_means,
-                                  float* v_quats, float* v_scales, float* v_densities, r2s_stream_t stream) {
+                                  const uint32_t* counters, float* isect_grads, void* workspace,
+                                  size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
+                                  float* v_scales, float* v_densities, r2s_stream_t stream) {
     if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, tile_order, counters, capacity, C, width, height,
                                       tile_size, 0, v_image, isect_grads, stream))
         return rc;
     return r2s_fused_param_vjp(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
-                               record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,
-                               v_densities, stream);
+                               record_offsets, counters, capacity, isect_grads, workspace, workspace_bytes, v_means,
+                               v_quats, v_scales, v_densities, stream);
 }

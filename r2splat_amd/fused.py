@@ -146,6 +146,7 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
     grad = torch.empty((fwd.capacity * 6,), dtype=torch.float32, device=device)
+    ws = torch.empty((be.lib().r2s_fused_param_workspace_bytes(C, N),), dtype=torch.uint8, device=device)
     v_means = torch.empty_like(means)
     v_quats = torch.empty_like(quats)
     v_scales = torch.empty_like(scales)
@@ -159,5 +160,6 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
         with stage("param_bwd"):
             call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
                  ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps, ptr(fwd.record_offsets), ptr(fwd.counters),
-                 fwd.capacity, ptr(grad), ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens), stream)
+                 fwd.capacity, ptr(grad), ptr(ws), ws.numel(), ptr(v_means), ptr(v_quats), ptr(v_scales),
+                 ptr(v_dens), stream)
     return v_means, v_quats, v_scales, v_dens

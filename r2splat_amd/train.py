@@ -88,6 +88,8 @@ class GaussianFit:
         self.tile_order = torch.empty((self.C * nty * ntx,), dtype=torch.int32, device=device)
         self.record_offsets = torch.empty((self.C * N + 1,), dtype=torch.int32, device=device)
         self.counters = torch.zeros((2,), dtype=torch.int32, device=device)
+        self.param_ws = torch.empty((be.lib().r2s_fused_param_workspace_bytes(self.C, N),), dtype=torch.uint8,
+                                    device=device)
         self.graph = None
         self.steps_issued = 0
         self._alloc(int(capacity) if capacity else self._measure_capacity())
@@ -116,7 +118,7 @@ class GaussianFit:
     def memory_bytes(self) -> int:
         """Device bytes held by this job (parameters, state and intermediates)."""
         tensors = [self.params, self.grads, self.image, self.v_image, self.loss_partials, self.tile_offsets,
-                   self.tile_order, self.record_offsets, self.scratch, self.packed, self.isect_grads, self.target]
+                   self.tile_order, self.record_offsets, self.scratch, self.param_ws, self.packed, self.isect_grads, self.target]
         if self.adam is not None:
             tensors += [self.exp_avg, self.exp_avg_sq]
         return sum(t.numel() * t.element_size() for t in tensors)
@@ -139,8 +141,8 @@ class GaussianFit:
              ptr(self.isect_grads), s)
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.eps, ptr(self.record_offsets), ptr(self.counters),
-             self.capacity, ptr(self.isect_grads), ptr(self.v_means), ptr(self.v_quats), ptr(self.v_scales),
-             ptr(self.v_densities), s)
+             self.capacity, ptr(self.isect_grads), ptr(self.param_ws), self.param_ws.numel(), ptr(self.v_means),
+             ptr(self.v_quats), ptr(self.v_scales), ptr(self.v_densities), s)
         if self.adam is not None:
             a = self.adam
             call("r2s_adam_step", ptr(self.params), ptr(self.grads), ptr(self.exp_avg), ptr(self.exp_avg_sq),
