@@ -40,6 +40,15 @@ FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
 # ~1); the backward reduces each record to six per-row moments (dx 1, two
 # FMAs 4, exp 1, v*e 1, three moment updates 4, per-row terms ~0.5).
 EXECUTED_FLOP_PER_PAIR = {"rasterize_fwd": 9.0, "rasterize_bwd": 11.5}
+# HBM bytes per launch of the raster kernels on this workload, from two
+# rocprofv3 --pmc runs of `bench.py --steps 2 --warmup 1 --plain-steps 0
+# --no-cpu-baseline --no-voxel --no-graph --copy-calibration` (FETCH_SIZE,
+# then WRITE_SIZE; KiB per dispatch), each scaled by the ratio of the bytes a
+# plain 256 MiB device copy moves to what the same counter reads for it
+# (FETCH 131,089 KiB -> x2.0; WRITE 262,144 KiB -> x1.0). Summary and
+# commands: profiles/raster_100k_64x256_hbm_traffic.md.
+PMC_KIB = {"rasterize_fwd": (433_808, 16_446), "rasterize_bwd": (445_886, 860_511)}
+PMC_SCALE = (262_144 / 131_089, 262_144 / 262_144)
 
 
 def parse():
@@ -129,8 +138,11 @@ def main():
     dom_ms = kernel_stages[dominant][0]
     achieved = pairs * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
     executed = pairs * EXECUTED_FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
+    fetch, write = PMC_KIB[dominant]
+    traffic = int((fetch * PMC_SCALE[0] + write * PMC_SCALE[1]) * 1024)
     roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "traffic_gbs": round(traffic / (dom_ms * 1e-3) / 1e9, 1),
                 "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
                 "avg_launch_ms": round(dom_ms, 4),
                 "executed": {"flop_per_pair": EXECUTED_FLOP_PER_PAIR[dominant], "achieved": round(executed, 3),

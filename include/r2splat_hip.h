@@ -320,7 +320,7 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * counters uint32 [2] persist from the forward to the backward. counters[0]
  * = intersections, counters[1] = 1 when capacity was exceeded (that
  * forward's outputs are then incomplete and the call must be repeated with
- * a larger capacity). isect_grads: float [capacity * 6], one VJP partial per
+ * a larger capacity). isect_grads: float [capacity * 8], one VJP partial per
  * intersection, summed per primitive in a fixed order (the backward is
  * deterministic and uses no atomics). */
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,

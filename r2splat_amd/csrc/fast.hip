@@ -40,6 +40,7 @@ static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
 constexpr int kDefaultWaves = 8;                     // waves sharing one tile unless chosen otherwise
+constexpr int kPartialStride = 8;                    // floats per intersection partial (6 used; 32-byte aligned)
 
 __device__ __forceinline__ uint32_t qlim_of(float w) {
     const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
@@ -437,10 +438,9 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
         if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
             // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
             const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
-            float2* gp = reinterpret_cast<float2*>(isect_grads + 6 * size_t(s.g));
-            gp[0] = make_float2(s.w * (a * mx + b * my), s.w * (b * mx + c * my));
-            gp[1] = make_float2(-0.5f * s.w * mxx, -s.w * mxy);
-            gp[2] = make_float2(-0.5f * s.w * myy, m0);
+            float4* gp = reinterpret_cast<float4*>(isect_grads + kPartialStride * size_t(s.g));
+            gp[0] = make_float4(s.w * (a * mx + b * my), s.w * (b * mx + c * my), -0.5f * s.w * mxx, -s.w * mxy);
+            gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, 0.f);
         }
     }
 }
@@ -472,9 +472,10 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
     const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
     float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
     for (uint32_t j = lo; j < hi; ++j) {
-        const float2* gp = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(j));
-        const float2 a = gp[0], b = gp[1], d = gp[2];
-        g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y; g4 += d.x; g5 += d.y;
+        const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
+        const float4 a = gp[0];
+        const float2 b = *reinterpret_cast<const float2*>(gp + 1);
+        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w; g4 += b.x; g5 += b.y;
     }
     float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (g0 != 0.f || g1 != 0.f || g2 != 0.f || g3 != 0.f || g4 != 0.f || g5 != 0.f) {

@@ -145,7 +145,7 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities)."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
-    grad = torch.empty((fwd.capacity * 6,), dtype=torch.float32, device=device)
+    grad = torch.empty((fwd.capacity * 8,), dtype=torch.float32, device=device)
     ws = torch.empty((be.lib().r2s_fused_param_workspace_bytes(C, N),), dtype=torch.uint8, device=device)
     v_means = torch.empty_like(means)
     v_quats = torch.empty_like(quats)

@@ -107,7 +107,7 @@ class GaussianFit:
         self.scratch = torch.empty((nbytes,), dtype=torch.uint8, device=self.device)
         self.packed = torch.empty((self.capacity * lib.r2s_fused_record_bytes(),), dtype=torch.uint8,
                                   device=self.device)
-        self.isect_grads = torch.empty((self.capacity * 6,), dtype=torch.float32, device=self.device)
+        self.isect_grads = torch.empty((self.capacity * 8,), dtype=torch.float32, device=self.device)
         self.graph = None
 
     def _measure_capacity(self) -> int:
