@@ -1,0 +1,227 @@
+"""Batched fits: many independent reconstructions of uneven size stepped
+together from one hipGraph, with per-job kernel configurations chosen by an
+autotuner.
+
+``BatchedFit`` places each job (a ``GaussianFit``: its own Gaussians, views,
+image size and Adam state) on one of a few streams, longest jobs first onto
+the least-loaded stream, and captures one step of every job (fused forward,
+MSE, backward, Adam) into a single graph whose streams fork from and join
+one capture stream, so jobs on different streams overlap on the device and a
+whole batched step costs one graph launch. Jobs on the same stream run one
+after another and share one ``Workspace`` (the per-intersection buffers),
+sized for the largest of them: HBM holds every job's parameters and
+optimizer state plus one workspace per stream, not one per job.
+
+``Autotuner`` picks the tile edge (8, 16, 32) and the waves per tile of the
+forward and backward raster kernels (4, 8, 16) for a job by timing the
+candidates on that job's own data; results are cached by (views, image size,
+log2 N). Every configuration computes the same function (the tile edge only
+changes which pixels share a list, the waves only the forward's summation
+order).
+
+A job whose intersection count outgrows its capacity skips its own update on
+the device (see ``GaussianFit``); ``sync()`` grows that job's capacity,
+replays its skipped steps on its own and re-captures the batch, so every
+job's sequence of applied updates equals that of the job fitted alone.
+
+The reference fits one scene per call (r2splat/explicit.py); this layer is
+the batched-fit capability built on the same operators.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from . import fused
+from .hip import _backend as be
+from .hip._backend import call, ptr
+from .train import AdamConfig, GaussianFit, Workspace
+
+
+@dataclass(frozen=True)
+class KernelConfig:
+    tile_size: int = 16
+    fwd_waves: int = 8
+    bwd_waves: int = 8
+
+
+@dataclass
+class TuneRecord:
+    config: KernelConfig
+    ms: float          # prepare + render + raster VJP of the chosen configuration
+    timings: dict      # (tile, kind, waves) -> ms
+
+
+class Autotuner:
+    """Times tile edges and waves per tile on a job's own data and keeps the fastest."""
+
+    TILES = (8, 16, 32)
+    WAVES = (4, 8, 16)
+
+    def __init__(self, repeats: int = 2, tiles: Sequence[int] = TILES, waves: Sequence[int] = WAVES):
+        self.repeats = max(1, int(repeats))
+        self.tiles = tuple(tiles)
+        self.waves = tuple(waves)
+        self.cache: dict = {}
+
+    @staticmethod
+    def key(N: int, C: int, H: int, W: int) -> tuple:
+        return (C, H, W, int(round(math.log2(max(N, 1)))))
+
+    def _time(self, fn) -> float:
+        fn()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(self.repeats):
+            fn()
+        end.record()
+        end.synchronize()
+        return start.elapsed_time(end) / self.repeats
+
+    def tune(self, means, quats, scales, densities, viewmats, Ks, height: int, width: int, *, eps: float = 1e-8,
+             radius_clip: float = 3.0) -> TuneRecord:
+        C, N = int(viewmats.shape[0]), int(means.shape[0])
+        key = self.key(N, C, height, width)
+        if key in self.cache:
+            return self.cache[key]
+        device = means.device
+        v_image = torch.randn((C, height, width), device=device, dtype=torch.float32,
+                              generator=torch.Generator(device=device).manual_seed(0)) * 1e-6
+        timings, best = {}, None
+        for tile in self.tiles:
+            prep = lambda: fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,  # noqa
+                                         radius_clip, tile_size=tile)
+            f = prep()
+            t_prep = self._time(prep)
+            grads = torch.empty((f.capacity * 8,), dtype=torch.float32, device=device)
+            stream = be.stream_of(device)
+            fwd, bwd = {}, {}
+            for w in self.waves:
+                fwd[w] = self._time(lambda: fused.render(f, C, device, w))
+                bwd[w] = self._time(lambda: call(
+                    "r2s_fused_raster_vjp", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters),
+                    f.capacity, C, width, height, tile, w, ptr(v_image), ptr(grads), stream))
+                timings[(tile, "render", w)] = fwd[w]
+                timings[(tile, "raster_vjp", w)] = bwd[w]
+            timings[(tile, "prepare", 0)] = t_prep
+            wf, wb = min(fwd, key=fwd.get), min(bwd, key=bwd.get)
+            total = t_prep + fwd[wf] + bwd[wb]
+            if best is None or total < best.ms:
+                best = TuneRecord(KernelConfig(tile, wf, wb), total, timings)
+            del f, grads
+        best.timings = timings
+        self.cache[key] = best
+        return best
+
+
+class BatchedFit:
+    """Steps many independent ``GaussianFit`` jobs together from one hipGraph.
+
+    ``jobs``: sequence of dicts with means, quats, scales, densities, viewmats,
+    Ks, target (float32, on one device) and optionally a ``config``
+    (KernelConfig) and ``capacity``.
+    """
+
+    def __init__(self, jobs, *, adam: Optional[AdamConfig] = AdamConfig(), n_streams: int = 4,
+                 tuner: Optional[Autotuner] = None, eps: float = 1e-8, radius_clip: float = 3.0):
+        if not jobs:
+            raise ValueError("no jobs")
+        self.device = jobs[0]["means"].device
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, n_streams))]
+        self.workspaces = [Workspace(self.device) for _ in self.streams]
+        self.tune_records = []
+        load = [0.0] * len(self.streams)
+        order = sorted(range(len(jobs)), key=lambda i: -self._cost(jobs[i]))
+        self.fits: list = [None] * len(jobs)
+        for i in order:
+            job = jobs[i]
+            cfg = job.get("config")
+            if cfg is None and tuner is not None:
+                rec = tuner.tune(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"],
+                                 job["Ks"], int(job["target"].shape[-2]), int(job["target"].shape[-1]), eps=eps,
+                                 radius_clip=radius_clip)
+                cfg = rec.config
+                self.tune_records.append(rec)
+            cfg = cfg or KernelConfig()
+            s = min(range(len(self.streams)), key=lambda k: load[k])
+            load[s] += self._cost(job)
+            self.fits[i] = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"],
+                                       job["Ks"], job["target"], eps=eps, radius_clip=radius_clip, adam=adam,
+                                       capacity=job.get("capacity"), stream=self.streams[s],
+                                       tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves,
+                                       workspace=self.workspaces[s])
+        self.graph = None
+        self._versions = None
+        self.steps_issued = 0
+
+    @staticmethod
+    def _cost(job) -> float:
+        C, H, W = job["target"].shape
+        return float(job["means"].shape[0]) * C * math.sqrt(H * W)
+
+    def _stale(self) -> bool:
+        return self.graph is None or self._versions != [ws.version for ws in self.workspaces]
+
+    def capture(self):
+        """One graph: fork to the job streams, each job's step in order, join."""
+        main = torch.cuda.Stream(self.device)
+        main.wait_stream(torch.cuda.current_stream(self.device))
+        for s in self.streams:  # buffers written before capture are visible to every job stream
+            s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(main):
+            with torch.cuda.graph(g, stream=main):
+                for s in self.streams:
+                    s.wait_stream(main)
+                for fit in self.fits:
+                    with torch.cuda.stream(fit.stream):
+                        fit._issue()
+                for s in self.streams:
+                    main.wait_stream(s)
+        torch.cuda.current_stream(self.device).wait_stream(main)
+        self.graph = g
+        self._versions = [ws.version for ws in self.workspaces]
+        return g
+
+    def step(self, n: int = 1):
+        """Issues n batched steps (graph replays) without synchronising."""
+        if self._stale():
+            self.capture()
+        for _ in range(n):
+            self.graph.replay()
+        self.steps_issued += n
+
+    def sync(self) -> int:
+        """Waits, then repairs jobs that overflowed their capacity (grow,
+        replay their skipped steps alone) until every job has applied the same
+        number of updates. Returns that number."""
+        torch.cuda.synchronize(self.device)
+        while True:
+            pending = [(fit, fit.pending_skips()) for fit in self.fits]
+            pending = [(fit, k) for fit, k in pending if k > 0]
+            if not pending:
+                break
+            for fit, k in pending:
+                fit.regrow()
+                with torch.cuda.stream(fit.stream):
+                    for _ in range(k):
+                        fit._issue()
+            torch.cuda.synchronize(self.device)
+        applied = {int(f.adam_state[0].item()) for f in self.fits} if self.fits[0].adam is not None else {
+            self.steps_issued}
+        if len(applied) != 1:
+            raise RuntimeError(f"jobs applied different step counts: {sorted(applied)}")
+        return applied.pop()
+
+    def resident_bytes(self) -> int:
+        return sum(f.resident_bytes() for f in self.fits) + sum(ws.nbytes() for ws in self.workspaces)
+
+    def losses(self) -> list:
+        return [float(v) for v in torch.cat([f.loss for f in self.fits]).tolist()]
+
+
+__all__ = ["Autotuner", "BatchedFit", "KernelConfig", "TuneRecord"]

@@ -35,6 +35,42 @@ from .hip._backend import call, ptr
 TILE = fused.TILE
 
 
+class Workspace:
+    """Intermediates of the fused pipeline (projected records, sort scratch,
+    packed intersection records, intersection partials, parameter-VJP
+    contributions). Jobs issued on the same stream run one after another and
+    can share one workspace, sized for the largest of them; ``version``
+    changes whenever a buffer is reallocated (captured graphs then hold stale
+    pointers and must be re-captured)."""
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self.version = 0
+        self.scratch = self.packed = self.isect_grads = self.param_ws = None
+
+    @staticmethod
+    def _grow(t, nbytes, device):
+        if t is not None and t.numel() >= nbytes:
+            return t, False
+        return torch.empty((max(1, int(nbytes)),), dtype=torch.uint8, device=device), True
+
+    def ensure(self, C, N, W, H, tile_size, capacity):
+        lib = be.lib()
+        changed = False
+        for name, nbytes in (("scratch", lib.r2s_fused_scratch_bytes(C, N, W, H, tile_size, capacity)),
+                             ("packed", capacity * lib.r2s_fused_record_bytes()),
+                             ("isect_grads", capacity * 8 * 4),
+                             ("param_ws", lib.r2s_fused_param_workspace_bytes(C, N))):
+            t, grown = self._grow(getattr(self, name), nbytes, self.device)
+            setattr(self, name, t)
+            changed |= grown
+        if changed:
+            self.version += 1
+
+    def nbytes(self) -> int:
+        return sum(t.numel() for t in (self.scratch, self.packed, self.isect_grads, self.param_ws) if t is not None)
+
+
 @dataclass
 class AdamConfig:
     lr: float = 1e-2
@@ -49,7 +85,7 @@ class GaussianFit:
     def __init__(self, means, quats, scales, densities, viewmats, Ks, target, *, eps: float = 1e-8,
                  radius_clip: float = 3.0, adam: Optional[AdamConfig] = None, capacity: Optional[int] = None,
                  headroom: float = 1.15, stream: Optional[torch.cuda.Stream] = None, tile_size: int = TILE,
-                 fwd_waves: int = 0, bwd_waves: int = 0):
+                 fwd_waves: int = 0, bwd_waves: int = 0, workspace: Optional[Workspace] = None):
         device = means.device
         if means.dtype != torch.float32:
             raise TypeError("GaussianFit runs the float32 fused pipeline")
@@ -88,9 +124,9 @@ class GaussianFit:
         self.tile_order = torch.empty((self.C * nty * ntx,), dtype=torch.int32, device=device)
         self.record_offsets = torch.empty((self.C * N + 1,), dtype=torch.int32, device=device)
         self.counters = torch.zeros((2,), dtype=torch.int32, device=device)
-        self.param_ws = torch.empty((be.lib().r2s_fused_param_workspace_bytes(self.C, N),), dtype=torch.uint8,
-                                    device=device)
+        self.ws = workspace if workspace is not None else Workspace(device)
         self.graph = None
+        self._graph_version = -1
         self.steps_issued = 0
         self._alloc(int(capacity) if capacity else self._measure_capacity())
 
@@ -102,46 +138,50 @@ class GaussianFit:
 
     def _alloc(self, capacity: int):
         self.capacity = max(1024, int(capacity))
-        lib = be.lib()
-        nbytes = lib.r2s_fused_scratch_bytes(self.C, self.N, self.W, self.H, self.tile_size, self.capacity)
-        self.scratch = torch.empty((nbytes,), dtype=torch.uint8, device=self.device)
-        self.packed = torch.empty((self.capacity * lib.r2s_fused_record_bytes(),), dtype=torch.uint8,
-                                  device=self.device)
-        self.isect_grads = torch.empty((self.capacity * 8,), dtype=torch.float32, device=self.device)
-        self.graph = None
+        self.ws.ensure(self.C, self.N, self.W, self.H, self.tile_size, self.capacity)
+
+    @property
+    def graph_stale(self) -> bool:
+        return self.graph is None or self._graph_version != self.ws.version
 
     def _measure_capacity(self) -> int:
         fwd = fused.prepare(self.means, self.quats, self.scales, self.densities, self.viewmats, self.Ks, self.W,
                             self.H, self.eps, self.radius_clip, tile_size=self.tile_size)
         return int(math.ceil(fwd.n_isects * self.headroom)) + 1024
 
-    def memory_bytes(self) -> int:
-        """Device bytes held by this job (parameters, state and intermediates)."""
+    def resident_bytes(self) -> int:
+        """Device bytes this job holds between steps (parameters, optimizer
+        state, images, per-view offsets), excluding its workspace."""
         tensors = [self.params, self.grads, self.image, self.v_image, self.loss_partials, self.tile_offsets,
-                   self.tile_order, self.record_offsets, self.scratch, self.param_ws, self.packed, self.isect_grads, self.target]
+                   self.tile_order, self.record_offsets, self.target, self.counters, self.adam_state]
         if self.adam is not None:
             tensors += [self.exp_avg, self.exp_avg_sq]
         return sum(t.numel() * t.element_size() for t in tensors)
+
+    def memory_bytes(self) -> int:
+        """Resident bytes plus its (possibly shared) workspace."""
+        return self.resident_bytes() + self.ws.nbytes()
 
     # -- one step -------------------------------------------------------------
     def _issue(self, with_loss: bool = True):
         """Enqueues one step on the current stream (no host synchronisation)."""
         s = be.stream_of(self.device)
         C, N, W, H = self.C, self.N, self.W, self.H
+        ws = self.ws
         call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size, self.eps, self.radius_clip, self.capacity,
-             ptr(self.scratch), self.scratch.numel(), ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order),
+             ptr(ws.scratch), ws.scratch.numel(), ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.record_offsets), ptr(self.counters), s)
-        call("r2s_fused_render", ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
+        call("r2s_fused_render", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
              self.capacity, C, W, H, self.tile_size, self.fwd_waves, ptr(self.image), s)
         call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
              ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
-        call("r2s_fused_raster_vjp", ptr(self.packed), ptr(self.tile_offsets), ptr(self.tile_order),
+        call("r2s_fused_raster_vjp", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, ptr(self.v_image),
-             ptr(self.isect_grads), s)
+             ptr(ws.isect_grads), s)
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.eps, ptr(self.record_offsets), ptr(self.counters),
-             self.capacity, ptr(self.isect_grads), ptr(self.param_ws), self.param_ws.numel(), ptr(self.v_means),
+             self.capacity, ptr(ws.isect_grads), ptr(ws.param_ws), ws.param_ws.numel(), ptr(self.v_means),
              ptr(self.v_quats), ptr(self.v_scales), ptr(self.v_densities), s)
         if self.adam is not None:
             a = self.adam
@@ -163,6 +203,7 @@ class GaussianFit:
                     self._issue()
             torch.cuda.current_stream(self.device).wait_stream(side)
         self.graph = g
+        self._graph_version = self.ws.version
         return g
 
     def step(self, n: int = 1, graph: bool = True):
@@ -170,12 +211,26 @@ class GaussianFit:
         with self._ctx():
             for _ in range(n):
                 if graph:
-                    if self.graph is None:
-                        raise RuntimeError("capture() first")
+                    if self.graph_stale:
+                        raise RuntimeError("capture() first (the workspace changed since the last capture)")
                     self.graph.replay()
                 else:
                     self._issue()
         self.steps_issued += n
+
+    def pending_skips(self) -> int:
+        """Steps whose forward overflowed the capacity (after a synchronisation)."""
+        count, overflow = (int(v) for v in self.counters.tolist())
+        applied, skipped = (int(v) for v in self.adam_state.tolist())
+        if self.adam is None:
+            return 1 if overflow else 0
+        return skipped
+
+    def regrow(self):
+        """Grows the capacity past the last step's intersection count and clears the skip counter."""
+        count = int(self.counters[0].item())
+        self._alloc(int(math.ceil(max(count, self.capacity) * self.headroom)) + 1024)
+        self.adam_state[1] = 0
 
     def sync(self) -> int:
         """Waits for the issued steps; on an overflow, grows the capacity,
@@ -183,17 +238,11 @@ class GaussianFit:
         with self._ctx():
             torch.cuda.current_stream(self.device).synchronize()
             while True:
-                count, overflow = (int(v) for v in self.counters.tolist())
-                applied, skipped = (int(v) for v in self.adam_state.tolist())
-                if self.adam is None:
-                    if not overflow:
-                        return self.steps_issued
-                    skipped = 1
-                if skipped == 0 and not overflow:
-                    return applied
+                skipped = self.pending_skips()
+                if skipped == 0:
+                    return int(self.adam_state[0].item()) if self.adam is not None else self.steps_issued
                 recapture = self.graph is not None
-                self._alloc(int(math.ceil(max(count, self.capacity) * self.headroom)) + 1024)
-                self.adam_state[1] = 0
+                self.regrow()
                 if recapture:
                     self.capture()
                 for _ in range(skipped):
@@ -204,4 +253,4 @@ class GaussianFit:
         return float(self.loss.item())
 
 
-__all__ = ["AdamConfig", "GaussianFit"]
+__all__ = ["AdamConfig", "GaussianFit", "Workspace"]

@@ -93,3 +93,31 @@ def config0(device="cpu", dtype=torch.float32, seed=0) -> Scene:
 def config1(device="cpu", dtype=torch.float32, seed=0) -> Scene:
     """N = 100,000, C = 64 views of 256 x 256, scales log-uniform in [0.005, 0.05]."""
     return scene(100_000, 64, 256, 256, scale_range=(0.005, 0.05), seed=seed, dtype=dtype, device=device)
+
+
+def config3_specs(n_jobs: int = 32, seed: int = 0):
+    """Batched uneven reconstructions: per job N log-uniform in [10k, 400k],
+    C uniform in {16, 32, 64, 128} views and image side uniform in
+    {128, 256, 512}; returns [(N, C, side, job_seed)]."""
+    g = torch.Generator().manual_seed(seed)
+    specs = []
+    for j in range(n_jobs):
+        u = float(torch.rand((), generator=g, dtype=torch.float64))
+        n = int(round(math.exp(math.log(10_000) + u * (math.log(400_000) - math.log(10_000)))))
+        c = (16, 32, 64, 128)[int(torch.randint(0, 4, (), generator=g))]
+        side = (128, 256, 512)[int(torch.randint(0, 3, (), generator=g))]
+        specs.append((n, c, side, 1000 + 2 * j))
+    return specs
+
+
+def fit_job(n: int, views: int, side: int, job_seed: int, device="cuda", scale_range=(0.005, 0.05)):
+    """One reconstruction job as in config 1: parameters from `job_seed`, target
+    images rendered from a second Gaussian set (`job_seed + 1`)."""
+    from .explicit import rasterize_fwd
+
+    sc = scene(n, views, side, side, scale_range=scale_range, seed=job_seed, device=device)
+    tg = scene(n, views, side, side, scale_range=scale_range, seed=job_seed + 1, device=device)
+    with torch.no_grad():
+        target, _ = rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size)
+    return dict(means=sc.means, quats=sc.quats, scales=sc.scales, densities=sc.densities, viewmats=sc.viewmats,
+                Ks=sc.Ks, target=target)
