@@ -397,6 +397,10 @@ int r2s_adam_step(float* params, const float* grads, float* exp_avg, float* exp_
                   float lr, float beta1, float beta2, float eps, uint32_t* state,
                   const uint32_t* skip_flag, r2s_stream_t stream);
 
+/* x[i] = max(x[i], lo): keeps optimized parameters inside the explicit API's
+ * domain (positive scales, nonnegative densities) after an update. */
+int r2s_clamp_min(float* x, uint64_t n, float lo, r2s_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Losses                                                                    */
 /* ------------------------------------------------------------------------ */

@@ -85,9 +85,22 @@ __global__ void adam_finish_kernel(uint32_t* __restrict__ state, const uint32_t*
     }
 }
 
+__global__ void __launch_bounds__(kTrainThreads)
+clamp_min_kernel(float* __restrict__ x, uint64_t n, float lo) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = fmaxf(x[i], lo);
+}
+
 }  // namespace r2s
 
 using namespace r2s;
+
+extern "C" int r2s_clamp_min(float* x, uint64_t n, float lo, r2s_stream_t stream) {
+    if (n == 0) return R2S_OK;
+    R2S_REQUIRE(n / kTrainThreads < (1u << 31), "buffer too large");
+    clamp_min_kernel<<<ceil_div(n, kTrainThreads), kTrainThreads, 0, stream>>>(x, n, lo);
+    return check_launch("clamp_min");
+}
 
 extern "C" size_t r2s_mse_partials(uint64_t n) { return (n + kMseBlockElems - 1) / kMseBlockElems; }
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "r2s_mse_partials": [U64],
     "r2s_mse_adjoint": [P, P, U64, P, P, P, P],
     "r2s_adam_step": [P, P, P, P, U64, F, F, F, F, P, P, P],
+    "r2s_clamp_min": [P, U64, F, P],
     "r2s_volume_tv_fwd": [I32, P, U32, U32, U32, U32, P, P, P],
     "r2s_volume_tv_bwd": [I32, P, P, U32, U32, U32, U32, D, P, P],
     "r2s_image_l1_fwd": [I32, P, P, U64, P, P, P],

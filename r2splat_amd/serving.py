@@ -1,0 +1,265 @@
+"""Continuous-batching reconstruction server.
+
+Requests (one reconstruction job each: N Gaussians or Beta primitives, C
+views of side x side, a number of Adam iterations) arrive over time. The
+server admits a queued request as soon as its HBM footprint fits the budget
+(in arrival order, letting a later request that fits pass one that does
+not), keeps every admitted job's parameters, optimizer state, target images
+and intermediates resident on the device, and on every tick issues one
+iteration of every live job: Gaussian jobs replay their own captured
+hipGraph (fused fwd + MSE + bwd + Adam) on one of a few streams, so jobs
+overlap on the device; Beta jobs step through the explicit operator path.
+A job leaves, and frees its memory, when its last iteration has been
+applied; time-to-finished-fit is measured from its arrival.
+
+Updates are Adam followed by a clamp to the explicit API's domain (scales >=
+``min_scale``, densities >= 0, beta >= ``min_beta``) so that the same job can
+be stepped by the plain explicit path for comparison. A job's result does not
+depend on which other jobs share the device with it: the Gaussian path is
+bitwise deterministic, the Beta path equal up to the atomic summation order
+of its backward.
+
+The reference has no serving layer; this is the serving capability built on
+the same operators.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import torch
+
+from . import explicit, workloads
+from .train import AdamConfig, GaussianFit
+
+
+@dataclass
+class FitRequest:
+    rid: int
+    arrival: float          # seconds (or ticks, with a tick clock) from the start
+    n: int
+    views: int
+    side: int
+    seed: int
+    beta: bool = False
+    steps: int = 200
+
+
+def poisson_requests(rate: float = 4.0, duration: float = 120.0, *, seed: int = 0, steps: int = 200,
+                     p_beta: float = 0.5, n_range=(10_000, 400_000)) -> List[FitRequest]:
+    """Poisson arrivals at `rate` per second for `duration` seconds; job sizes as
+    in the batched workload (N log-uniform, C in {16, 32, 64, 128}, side in
+    {128, 256, 512}); Beta with probability `p_beta`."""
+    g = torch.Generator().manual_seed(seed)
+    out, t, rid = [], 0.0, 0
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    while True:
+        t += float(torch.empty(()).exponential_(rate, generator=g))
+        if t >= duration:
+            return out
+        n = int(round(math.exp(lo + float(torch.rand((), generator=g, dtype=torch.float64)) * (hi - lo))))
+        c = (16, 32, 64, 128)[int(torch.randint(0, 4, (), generator=g))]
+        side = (128, 256, 512)[int(torch.randint(0, 3, (), generator=g))]
+        beta = bool(torch.rand((), generator=g) < p_beta)
+        out.append(FitRequest(rid, t, n, c, side, 7000 + 2 * rid, beta, steps))
+        rid += 1
+
+
+def build_job(req: FitRequest, device) -> dict:
+    """Parameters (seed) and target images (second set, seed + 1) of a request."""
+    beta_range = (0.5, 4.0) if req.beta else None
+    sc = workloads.scene(req.n, req.views, req.side, req.side, scale_range=(0.005, 0.05), seed=req.seed,
+                         device=device, beta_range=beta_range)
+    tg = workloads.scene(req.n, req.views, req.side, req.side, scale_range=(0.005, 0.05), seed=req.seed + 1,
+                         device=device, beta_range=beta_range)
+    with torch.no_grad():
+        target, _ = explicit.rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size, beta=tg.beta)
+    return dict(means=sc.means, quats=sc.quats, scales=sc.scales, densities=sc.densities, beta=sc.beta,
+                viewmats=sc.viewmats, Ks=sc.Ks, target=target)
+
+
+class ExplicitFit:
+    """One job stepped through the explicit operator path (any primitive
+    kind): rasterize_fwd -> MSE -> rasterize_bwd -> torch Adam -> clamp."""
+
+    def __init__(self, job: dict, adam: AdamConfig, min_beta: float = 1e-3, stream=None):
+        self.job = job
+        self.names = [k for k in ("means", "quats", "scales", "densities", "beta") if job.get(k) is not None]
+        self.params = [job[k].detach().clone() for k in self.names]
+        self.opt = torch.optim.Adam(self.params, lr=adam.lr, betas=(adam.beta1, adam.beta2), eps=adam.eps,
+                                    foreach=False)
+        self.adam = adam
+        self.min_beta = min_beta
+        self.stream = stream
+        self.applied = 0
+        self.loss = None
+
+    def step(self):
+        with torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.device(self.params[0].device):
+            kw = dict(zip(self.names, self.params))
+            tgt = self.job["target"]
+            img, ctx = explicit.rasterize_fwd(kw["means"], kw["quats"], kw["scales"], kw["densities"],
+                                              self.job["viewmats"], self.job["Ks"], tuple(tgt.shape[-2:]),
+                                              beta=kw.get("beta"))
+            diff = img - tgt
+            self.loss = (diff * diff).mean()
+            grads = explicit.rasterize_bwd(ctx, (2.0 / img.numel()) * diff)
+            for p, name in zip(self.params, self.names):
+                p.grad = getattr(grads, name)
+            self.opt.step()
+            with torch.no_grad():
+                k = self.names.index("scales")
+                self.params[k].clamp_(min=self.adam.min_scale or 1e-6)
+                self.params[self.names.index("densities")].clamp_(min=0.0)
+                if "beta" in self.names:
+                    self.params[self.names.index("beta")].clamp_(min=self.min_beta)
+        self.applied += 1
+
+    def resident_bytes(self) -> int:
+        t = self.job["target"]
+        return 4 * sum(p.numel() for p in self.params) + t.numel() * t.element_size()
+
+
+@dataclass
+class _Live:
+    req: FitRequest
+    fit: object
+    fused: bool
+    admitted: float
+    done: int = 0
+    nbytes: int = 0
+
+
+@dataclass
+class ServeStats:
+    finished: list = field(default_factory=list)   # (rid, arrival, admitted, finished, final_loss, beta)
+    results: dict = field(default_factory=dict)    # rid -> final parameters (when kept)
+    ticks: int = 0
+    job_steps: int = 0
+    peak_bytes: int = 0
+    wall: float = 0.0
+
+    def latencies(self) -> list:
+        return sorted(f[3] - f[1] for f in self.finished)
+
+    def percentile(self, q: float) -> float:
+        lat = self.latencies()
+        if not lat:
+            return float("nan")
+        k = min(len(lat) - 1, max(0, int(math.ceil(q / 100.0 * len(lat))) - 1))
+        return lat[k]
+
+
+class FitServer:
+    def __init__(self, device="cuda", *, hbm_budget: Optional[int] = None, n_streams: int = 4,
+                 adam: AdamConfig = AdamConfig(lr=1e-2, min_scale=1e-6), headroom: float = 1.15):
+        self.device = torch.device(device)
+        if hbm_budget is None:
+            hbm_budget = int(torch.cuda.get_device_properties(self.device).total_memory * 0.85)
+        self.budget = int(hbm_budget)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, n_streams))]
+        self.adam = adam
+        self.headroom = headroom
+        self.live: List[_Live] = []
+        self._next_stream = 0
+
+    @staticmethod
+    def estimate_bytes(req: FitRequest) -> int:
+        """Footprint before building: parameters/state, images, and ~100 B per
+        expected intersection (records, sort scratch, partials) with the
+        expected intersections per (view, primitive) growing with the side."""
+        per = {128: 2.0, 256: 5.0, 512: 15.0}.get(req.side, 5.0 * (req.side / 256.0) ** 2)
+        isects = req.n * req.views * per
+        return int(16 * 11 * req.n + 12 * req.views * req.side * req.side + 100 * isects + 48 * req.views * req.n)
+
+    def used_bytes(self) -> int:
+        return sum(l.nbytes for l in self.live)
+
+    def _admit(self, req: FitRequest, now: float) -> bool:
+        if self.used_bytes() + self.estimate_bytes(req) > self.budget:
+            return False
+        job = build_job(req, self.device)
+        stream = self.streams[self._next_stream % len(self.streams)]
+        self._next_stream += 1
+        if req.beta:
+            fit = ExplicitFit(job, self.adam, stream=stream)
+            nbytes = self.estimate_bytes(req)
+            fused = False
+        else:
+            fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
+                              job["target"], adam=self.adam, stream=stream, headroom=self.headroom)
+            fit.capture()
+            nbytes = fit.memory_bytes()
+            fused = True
+        self.live.append(_Live(req, fit, fused, now, 0, nbytes))
+        return True
+
+    def run(self, requests: List[FitRequest], clock=None, max_wall: Optional[float] = None,
+            keep_results: bool = False) -> ServeStats:
+        """Serves `requests` (sorted by arrival) until all finished. `clock`
+        returns the current time in the requests' unit (default: wall seconds
+        since the call; "ticks": the number of ticks served so far);
+        ``max_wall`` stops early (unfinished jobs dropped); ``keep_results``
+        keeps each job's final parameters in ``stats.results``."""
+        t0 = time.perf_counter()
+        stats = ServeStats()
+        tick_clock = clock == "ticks"
+        if tick_clock:
+            clock = lambda: float(stats.ticks)  # noqa: E731
+        clock = clock or (lambda: time.perf_counter() - t0)
+        pending = sorted(requests, key=lambda r: r.arrival)
+        queue: List[FitRequest] = []
+        while pending or queue or self.live:
+            now = clock()
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.pop(0))
+            for req in list(queue):  # arrival order; a later request that fits may pass one that does not
+                if self._admit(req, now):
+                    queue.remove(req)
+            if not self.live:
+                if pending and not queue:
+                    if tick_clock:  # nothing to run: advance the tick clock to the next arrival
+                        stats.ticks = int(math.ceil(pending[0].arrival))
+                        continue
+                    wait = pending[0].arrival - clock()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                    continue
+                if queue and not pending and not self.live:
+                    raise RuntimeError(f"request {queue[0].rid} does not fit the HBM budget")
+                continue
+            stats.peak_bytes = max(stats.peak_bytes, self.used_bytes())
+            for l in self.live:   # one iteration of every live job
+                if l.fused:
+                    l.fit.step(1)
+                else:
+                    l.fit.step()
+            for l in self.live:
+                if l.fused:
+                    l.fit.sync()
+            torch.cuda.synchronize(self.device)
+            stats.ticks += 1
+            stats.job_steps += len(self.live)
+            now = clock()
+            still = []
+            for l in self.live:
+                l.done += 1
+                if l.done >= l.req.steps:
+                    loss = float(l.fit.loss.item()) if l.fit.loss is not None else float("nan")
+                    stats.finished.append((l.req.rid, l.req.arrival, l.admitted, now, loss, l.req.beta))
+                    if keep_results:
+                        stats.results[l.req.rid] = (l.fit.params.clone() if l.fused
+                                                    else [p.detach().clone() for p in l.fit.params])
+                else:
+                    still.append(l)
+            self.live = still
+            if max_wall is not None and time.perf_counter() - t0 > max_wall:
+                break
+        stats.wall = time.perf_counter() - t0
+        return stats
+
+
+__all__ = ["ExplicitFit", "FitRequest", "FitServer", "ServeStats", "build_job", "poisson_requests"]

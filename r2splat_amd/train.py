@@ -77,6 +77,9 @@ class AdamConfig:
     beta1: float = 0.9
     beta2: float = 0.999
     eps: float = 1e-8
+    # after each update, clamp scales to >= min_scale and densities to >= 0 (the
+    # explicit API's domain); None leaves the raw update
+    min_scale: Optional[float] = None
 
 
 class GaussianFit:
@@ -188,6 +191,9 @@ class GaussianFit:
             call("r2s_adam_step", ptr(self.params), ptr(self.grads), ptr(self.exp_avg), ptr(self.exp_avg_sq),
                  self.params.numel(), a.lr, a.beta1, a.beta2, a.eps, ptr(self.adam_state), ptr(self.counters[1:]),
                  s)
+            if a.min_scale is not None:
+                call("r2s_clamp_min", ptr(self.scales), self.scales.numel(), float(a.min_scale), s)
+                call("r2s_clamp_min", ptr(self.densities), self.densities.numel(), 0.0, s)
 
     def _ctx(self):
         return torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.device(self.device)
