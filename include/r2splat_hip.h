@@ -322,28 +322,39 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * forward's outputs are then incomplete and the call must be repeated with
  * a larger capacity). isect_grads: float [capacity * 8], one VJP partial per
  * intersection, summed per primitive in a fixed order (the backward is
- * deterministic and uses no atomics). */
+ * deterministic and uses no atomics).
+ *
+ * Primitive kind: betas == NULL selects the Gaussian profile (explicit.py
+ * with beta=None). Otherwise betas [N] is the per-primitive beta (projected
+ * exponent beta + 1/2), beta_ratio [N] = B(1/2, beta + 1) / sqrt(2 pi)
+ * (the line-integral factor, mus = mu * ratio; explicit.py:166-170) and
+ * beta_log_grad [N] = d log ratio / d beta (explicit.py:226-230), both as
+ * r2splat/_beta.py computes them. The same `beta` flag (0 Gaussian, 1 Beta)
+ * selects the record layout in r2s_fused_scratch_bytes / r2s_fused_record_bytes /
+ * r2s_fused_render / r2s_fused_raster_vjp and must match the forward. */
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
-                               uint32_t tile_size, uint32_t capacity);
-size_t r2s_fused_record_bytes(void);
+                               uint32_t tile_size, uint32_t capacity, int beta);
+size_t r2s_fused_record_bytes(int beta);
 
 int r2s_fused_forward(const float* means, const float* quats, const float* scales,
-                      const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                      const float* densities, const float* betas, const float* beta_ratio,
+                      const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
                       void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
                       uint32_t* counters, float* image, r2s_stream_t stream);
 
-/* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N]
- * (overwritten; camera sums in a fixed order). */
+/* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N],
+ * v_betas [N] (Beta only; overwritten; camera sums in a fixed order). */
 int r2s_fused_backward(const float* means, const float* quats, const float* scales,
-                       const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                       const float* densities, const float* betas, const float* beta_ratio,
+                       const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
                        uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                        uint32_t capacity, const void* packed, const int32_t* tile_offsets,
                        const uint32_t* tile_order, const uint32_t* record_offsets,
                        const uint32_t* counters, float* isect_grads, void* workspace,
                        size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
-                       float* v_scales, float* v_densities, r2s_stream_t stream);
+                       float* v_scales, float* v_densities, float* v_betas, r2s_stream_t stream);
 
 /* The same two calls split into stages: forward = prepare (projection,
  * binning, sort, offsets, record packing) + render (rasterization);
@@ -351,28 +362,30 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
  * param_vjp (per-primitive sums, conic/mu, projection and parameter-transform
  * VJPs, summed over cameras). */
 int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
-                      const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                      const float* densities, const float* betas, const float* beta_ratio,
+                      const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
                       void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
                       uint32_t* counters, r2s_stream_t stream);
 int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
-                     const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
-                     uint32_t waves, float* image, r2s_stream_t stream);
+                     const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+                     uint32_t tile_size, uint32_t waves, int beta, float* image, r2s_stream_t stream);
 int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
                          const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                         uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
+                         uint32_t tile_size, uint32_t waves, int beta, const float* v_image, float* isect_grads,
                          r2s_stream_t stream);
 /* workspace: r2s_fused_param_workspace_bytes(C, N) bytes (per-camera
  * contributions, summed over cameras in order in a second pass). */
 size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
-                        const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                        const float* densities, const float* betas, const float* beta_ratio,
+                        const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
                         uint32_t N, uint32_t width, uint32_t height, float eps,
                         const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
                         const float* isect_grads, void* workspace, size_t workspace_bytes,
                         float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                        r2s_stream_t stream);
+                        float* v_betas, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Training-step kernels (no counterpart in the reference, which leaves image */

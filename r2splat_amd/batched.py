@@ -83,18 +83,20 @@ class Autotuner:
         return start.elapsed_time(end) / self.repeats
 
     def tune(self, means, quats, scales, densities, viewmats, Ks, height: int, width: int, *, eps: float = 1e-8,
-             radius_clip: float = 3.0) -> TuneRecord:
+             radius_clip: float = 3.0, beta=None) -> TuneRecord:
         C, N = int(viewmats.shape[0]), int(means.shape[0])
-        key = self.key(N, C, height, width)
+        kind = int(beta is not None)
+        key = self.key(N, C, height, width) + (kind,)
         if key in self.cache:
             return self.cache[key]
         device = means.device
         v_image = torch.randn((C, height, width), device=device, dtype=torch.float32,
                               generator=torch.Generator(device=device).manual_seed(0)) * 1e-6
         timings, best = {}, None
+        factors = fused.beta_factors(beta) if kind else None
         for tile in self.tiles:
             prep = lambda: fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,  # noqa
-                                         radius_clip, tile_size=tile)
+                                         radius_clip, tile_size=tile, beta=beta, factors=factors)
             f = prep()
             t_prep = self._time(prep)
             grads = torch.empty((f.capacity * 8,), dtype=torch.float32, device=device)
@@ -104,7 +106,7 @@ class Autotuner:
                 fwd[w] = self._time(lambda: fused.render(f, C, device, w))
                 bwd[w] = self._time(lambda: call(
                     "r2s_fused_raster_vjp", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters),
-                    f.capacity, C, width, height, tile, w, ptr(v_image), ptr(grads), stream))
+                    f.capacity, C, width, height, tile, w, kind, ptr(v_image), ptr(grads), stream))
                 timings[(tile, "render", w)] = fwd[w]
                 timings[(tile, "raster_vjp", w)] = bwd[w]
             timings[(tile, "prepare", 0)] = t_prep
@@ -118,16 +120,24 @@ class Autotuner:
         return best
 
 
+def _clip(job, radius_clip):
+    """The explicit API's default footprint clip: 3 sigma (Gaussian), 1 (Beta support)."""
+    if radius_clip is not None:
+        return radius_clip
+    return 3.0 if job.get("beta") is None else 1.0
+
+
 class BatchedFit:
     """Steps many independent ``GaussianFit`` jobs together from one hipGraph.
 
     ``jobs``: sequence of dicts with means, quats, scales, densities, viewmats,
-    Ks, target (float32, on one device) and optionally a ``config``
-    (KernelConfig) and ``capacity``.
+    Ks, target (float32, on one device) and optionally ``beta`` (Beta
+    primitives), a ``config`` (KernelConfig) and ``capacity``. Gaussian and
+    Beta jobs mix freely in one batch.
     """
 
     def __init__(self, jobs, *, adam: Optional[AdamConfig] = AdamConfig(), n_streams: int = 4,
-                 tuner: Optional[Autotuner] = None, eps: float = 1e-8, radius_clip: float = 3.0):
+                 tuner: Optional[Autotuner] = None, eps: float = 1e-8, radius_clip: Optional[float] = None):
         if not jobs:
             raise ValueError("no jobs")
         self.device = jobs[0]["means"].device
@@ -143,17 +153,17 @@ class BatchedFit:
             if cfg is None and tuner is not None:
                 rec = tuner.tune(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"],
                                  job["Ks"], int(job["target"].shape[-2]), int(job["target"].shape[-1]), eps=eps,
-                                 radius_clip=radius_clip)
+                                 radius_clip=_clip(job, radius_clip), beta=job.get("beta"))
                 cfg = rec.config
                 self.tune_records.append(rec)
             cfg = cfg or KernelConfig()
             s = min(range(len(self.streams)), key=lambda k: load[k])
             load[s] += self._cost(job)
             self.fits[i] = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"],
-                                       job["Ks"], job["target"], eps=eps, radius_clip=radius_clip, adam=adam,
-                                       capacity=job.get("capacity"), stream=self.streams[s],
+                                       job["Ks"], job["target"], eps=eps, radius_clip=_clip(job, radius_clip),
+                                       adam=adam, capacity=job.get("capacity"), stream=self.streams[s],
                                        tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves,
-                                       workspace=self.workspaces[s])
+                                       workspace=self.workspaces[s], beta=job.get("beta"))
         self.graph = None
         self._versions = None
         self.steps_issued = 0

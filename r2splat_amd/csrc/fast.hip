@@ -1,4 +1,4 @@
-// Fused float32 Gaussian pipeline for 16x16 tiles (the working-size path).
+// Fused float32 pipeline for Gaussian and Beta primitives (the working-size path).
 //
 // Forward:  fused projection (parameter transform + projection + double
 //           precision conic/mu + radii + tile count, one 32-byte splat record
@@ -23,6 +23,11 @@
 // bit patterns and every negative float or NaN has a larger one, so the test
 // is the single unsigned compare bits(q) < qlim with qlim = bits(S) + 1, and
 // qlim = 0 (nothing passes) when S < 0, w = 0 or S is NaN.
+// Beta primitives (d^2 = delta^T P delta, alpha = w (1 - d^2)^b2d kept for
+// 0 <= d^2 < 1 and |alpha| >= 1e-5) use the same single compare on d^2 with
+// the bound D = min(largest float below 1, 1 - (1e-5 / |w|)^(1 / b2d)).
+#include <type_traits>
+
 #include "binning.hpp"
 #include "common.hpp"
 #include "geometry_common.hpp"
@@ -30,12 +35,26 @@
 
 namespace r2s {
 
+// Gaussian record: sigma log2 e = A dx^2 + B dx dy + C dy^2, alpha = w 2^-q.
 struct __align__(16) Splat {
     float mx, my, A, B, C, w;
     uint32_t qlim;  // bits(S) + 1, or 0 when the primitive contributes nowhere
-    int32_t g;
+    int32_t g;      // first emission index (projected) / emission index (packed)
 };
 static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
+
+// Beta record: d^2 = A dx^2 + B dx dy + C dy^2 (A = a, B = 2 b, C = c of the
+// conic), alpha = w (1 - d^2)^b2d with w = opacity mu ratio(beta), b2d = beta + 1/2.
+struct __align__(16) BetaSplat {
+    float mx, my, A, B, C, w;
+    uint32_t qlim;  // bits(D) + 1, or 0
+    int32_t g;
+    float b2d, pad0, pad1, pad2;
+};
+static_assert(sizeof(BetaSplat) == 48, "beta record must be 48 bytes");
+
+template <bool BETA>
+using Rec = typename std::conditional<BETA, BetaSplat, Splat>::type;
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
@@ -45,6 +64,13 @@ constexpr int kPartialStride = 8;                    // floats per intersection 
 __device__ __forceinline__ uint32_t qlim_of(float w) {
     const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
     return (w != 0.f && S >= 0.f) ? __float_as_uint(S) + 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t beta_qlim_of(float w, float b2d) {
+    const float S = __log2f(fabsf(w)) + kLog2Inv1e5;  // log2(|w| / 1e-5)
+    const float D = 1.f - __builtin_amdgcn_exp2f(-S / b2d);
+    if (!(w != 0.f && S >= 0.f && D >= 0.f)) return 0u;
+    return __float_as_uint(fminf(D, 0.99999994f)) + 1u;
 }
 
 __device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
@@ -59,12 +85,14 @@ __device__ __forceinline__ uint32_t live(const uint32_t* __restrict__ d_n, uint3
 // ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
 // ---------------------------------------------------------------------------
+template <bool BETA>
 __global__ void __launch_bounds__(256)
 fused_project_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                      const float* __restrict__ scales, const float* __restrict__ dens,
+                     const float* __restrict__ betas, const float* __restrict__ ratio,
                      const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
                      uint32_t H, float eps, float clip, uint32_t tile_size, uint32_t ntx, uint32_t nty,
-                     uint32_t N, Splat* __restrict__ recs, uint2* __restrict__ rects,
+                     uint32_t N, Rec<BETA>* __restrict__ recs, uint2* __restrict__ rects,
                      uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = blockIdx.y;
@@ -78,7 +106,8 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     covariance_from(R, s, cov);
     const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
-    Splat rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u, int32_t(idx)};
+    Rec<BETA> rec = {};
+    rec.g = int32_t(idx);
     uint2 rect = make_uint2(0, 0);
     uint32_t count = 0;
     ProjState<float> st;
@@ -94,14 +123,25 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
 #pragma unroll
         for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
         conic_mu_f64(cd, double(eps), P, mu);
-        const float w = dens[n] * float(mu);
         rec.mx = mx;
         rec.my = my;
-        rec.A = 0.5f * float(P[0]) * kLog2e;
-        rec.B = float(P[1]) * kLog2e;
-        rec.C = 0.5f * float(P[2]) * kLog2e;
-        rec.w = w;
-        rec.qlim = qlim_of(w);
+        if constexpr (BETA) {
+            const float mus = float(mu) * ratio[n];  // mu_beta = mu ratio(beta), in float32
+            const float w = dens[n] * mus;
+            rec.A = float(P[0]);
+            rec.B = 2.f * float(P[1]);
+            rec.C = float(P[2]);
+            rec.w = w;
+            rec.b2d = betas[n] + 0.5f;
+            rec.qlim = beta_qlim_of(w, rec.b2d);
+        } else {
+            const float w = dens[n] * float(mu);
+            rec.A = 0.5f * float(P[0]) * kLog2e;
+            rec.B = float(P[1]) * kLog2e;
+            rec.C = 0.5f * float(P[2]) * kLog2e;
+            rec.w = w;
+            rec.qlim = qlim_of(w);
+        }
         uint32_t x0, x1, y0, y1;
         if (rx > 0 && ry > 0 && tile_span(mx, rx, tile_size, ntx, x0, x1) &&
             tile_span(my, ry, tile_size, nty, y0, y1)) {
@@ -120,10 +160,11 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
 // orders the list by (tile, camera, primitive): one radix pass for up to 256
 // tiles per camera. The projected record's g field is set to its first
 // emission index.
+template <typename R>
 __global__ void __launch_bounds__(256)
 fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ cum, uint32_t total,
-                  uint32_t N, uint32_t ntx, uint32_t nt, uint32_t capacity, uint32_t* __restrict__ keys,
-                  uint32_t* __restrict__ vals, uint32_t* __restrict__ overflow, Splat* __restrict__ recs) {
+                  uint32_t ntx, uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                  uint32_t* __restrict__ overflow, R* __restrict__ recs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint32_t w0 = cum[i], cnt = cum[i + 1] - w0;
@@ -217,11 +258,12 @@ fused_list_lengths_kernel(const uint32_t* __restrict__ tm, const uint32_t* __res
 // is camera-major, tile, then primitive: the reference's order.
 constexpr int kPackItems = 4;
 
+template <typename R>
 __global__ void __launch_bounds__(256)
-fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t C,
+fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t C,
                   uint32_t nt, uint32_t ntx, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tm,
                   const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n, uint32_t capacity,
-                  Splat* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
+                  R* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x);
     if (k >= C * nt) return;
     const uint32_t n_live = live(d_n, capacity);
@@ -236,7 +278,7 @@ fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rect
             const uint32_t r = r0 + q * blockDim.x + threadIdx.x;
             v[q] = (r < len && src + r < n_live) ? vals[src + r] : n_recs;
         }
-        Splat rec[kPackItems];
+        R rec[kPackItems];
         uint2 rc[kPackItems];
 #pragma unroll
         for (int q = 0; q < kPackItems; ++q) {
@@ -268,16 +310,19 @@ fused_pack_kernel(const Splat* __restrict__ recs, const uint2* __restrict__ rect
 // partial images are summed in LDS in wave order, so the result does not
 // depend on scheduling.
 // ---------------------------------------------------------------------------
-template <int PX>
-__device__ __forceinline__ void splat_accumulate(const Splat& s, float px, const float (&py)[PX],
-                                                 float (&acc)[PX]) {
+template <int PX, typename R>
+__device__ __forceinline__ void splat_accumulate(const R& s, float px, const float (&py)[PX], float (&acc)[PX]) {
     const float dx = px - s.mx;
     const float adx2 = s.A * dx * dx, bdx = s.B * dx;
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         const float dy = py[k] - s.my;
-        const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);
-        const float e = __builtin_amdgcn_exp2f(-q);
+        const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);  // Gaussian: sigma log2 e; Beta: d^2
+        float e;
+        if constexpr (std::is_same<R, BetaSplat>::value)
+            e = __builtin_amdgcn_exp2f(s.b2d * __log2f(1.f - q));
+        else
+            e = __builtin_amdgcn_exp2f(-q);
         acc[k] = fmaf(contributes(q, s.qlim) ? e : 0.f, s.w, acc[k]);
     }
 }
@@ -288,8 +333,10 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ Splat load_splat(const Splat* __restrict__ packed, int32_t i, int32_t hi) {
-    Splat s = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u, -1};
+template <typename R>
+__device__ __forceinline__ R load_splat(const R* __restrict__ packed, int32_t i, int32_t hi) {
+    R s = {};
+    s.g = -1;
     if (i < hi) s = packed[i];
     return s;
 }
@@ -297,20 +344,23 @@ __device__ __forceinline__ Splat load_splat(const Splat* __restrict__ packed, in
 // Marks a record's registers as used here: the loads that produced them (a
 // whole chunk earlier) are waited for at this point, before the next
 // prefetch is issued, so later waits need not drain that prefetch.
-__device__ __forceinline__ void settle(const Splat& s) {
+template <typename R>
+__device__ __forceinline__ void settle(const R& s) {
     asm volatile("" ::"v"(s.mx), "v"(s.my), "v"(s.A), "v"(s.B), "v"(s.C), "v"(s.w), "v"(s.qlim), "v"(s.g));
+    if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d));
 }
 
-template <int T, int WAVES>
+template <int T, int WAVES, bool BETA>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
                         uint32_t H, float* __restrict__ out) {
     constexpr int PX = T * T / kWave, STEP = kWave / T;
     static_assert(PX * kWave == T * T && STEP * T == kWave, "tile must be 8, 16 or 32");
     __shared__ float part[WAVES][T * T];
-    __shared__ Splat stage[WAVES][kWave];
+    using R = Rec<BETA>;
+    __shared__ R stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
@@ -328,8 +378,8 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
         py[k] = float(ty * T + row0 + STEP * k) + 0.5f;
         acc[k] = 0.f;
     }
-    Splat* slot = stage[wave];
-    Splat next = load_splat(packed, lo + int32_t(lane), hi);
+    R* slot = stage[wave];
+    R next = load_splat(packed, lo + int32_t(lane), hi);
     for (int32_t base = lo; base < hi; base += kWave) {
         wave_lds_sync();  // every lane is done reading the previous batch
         slot[lane] = next;
@@ -337,9 +387,9 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
         next = load_splat(packed, base + kWave + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         // LDS reads pipelined one record ahead (the batch holds n >= 1 records)
-        Splat cur = slot[0];
+        R cur = slot[0];
         for (int32_t j = 1; j < n; ++j) {
-            const Splat nxt = slot[j];
+            const R nxt = slot[j];
             splat_accumulate<PX>(cur, px, py, acc);
             cur = nxt;
         }
@@ -369,10 +419,14 @@ fused_raster_fwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 // The VJP is
 //   v_mean = w (a Mx + b My, b Mx + c My), v_conic = -w (Mxx/2, Mxy, Myy/2),
 //   v_w = M0, stored (no atomics) as the partial of emission index g.
+// Beta: with h = 1 - d^2 the moments are of G = v h^(b2d - 1), plus
+// Se = sum v h^b2d and Sl = sum v h^b2d log2 h, and (delta = mean - pixel)
+//   v_mean = w b2d (2a Mx + 2b My, 2b Mx + 2c My),
+//   v_conic = -w b2d (Mxx, 2 Mxy, Myy), v_w = Se, v_b2d = w ln2 Sl.
 // ---------------------------------------------------------------------------
-template <int T, int WAVES>
+template <int T, int WAVES, bool BETA>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
                         uint32_t H, const float* __restrict__ v_img, float* __restrict__ isect_grads) {
@@ -393,16 +447,19 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
     __syncthreads();
     const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f;
     constexpr int32_t kStride = WAVES * kWave;
+    using R = Rec<BETA>;
     int32_t chunk = start + int32_t(wave) * kWave;
-    Splat next = load_splat(packed, chunk + int32_t(lane), end);
+    R next = load_splat(packed, chunk + int32_t(lane), end);
     for (; chunk < end; chunk += kStride) {
         const int32_t i = chunk + int32_t(lane);
         const bool active = i < end;
-        const Splat s = next;
+        const R s = next;
         settle(s);
         next = load_splat(packed, chunk + kStride + int32_t(lane), end);  // next chunk, a chunk ahead
         const float dx0 = x0 - s.mx;
-        float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f;
+        float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f;
+        float bm1 = 0.f;
+        if constexpr (BETA) bm1 = s.b2d - 1.f;
         for (uint32_t r = 0; r < uint32_t(T); ++r) {
             const float dy = y0 + float(r) - s.my;
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
@@ -421,8 +478,20 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
             for (int j = 0; j < T; ++j) {
                 const float dx = dx0 + float(j);
                 const float q = fmaf(dx, fmaf(s.A, dx, bdy), cdy2);
-                const float e = __builtin_amdgcn_exp2f(-q);
-                const float g = contributes(q, s.qlim) ? e * v[j] : 0.f;
+                float g;
+                if constexpr (BETA) {
+                    // h <= 0 outside the support: log2 h is -inf or NaN there, so it is
+                    // selected away (not multiplied by a zero weight)
+                    const bool in = contributes(q, s.qlim);
+                    const float h = 1.f - q, lg = __log2f(h);
+                    g = in ? v[j] * __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;
+                    const float ve = g * h;
+                    se += ve;
+                    sl = fmaf(ve, in ? lg : 0.f, sl);
+                } else {
+                    const float e = __builtin_amdgcn_exp2f(-q);
+                    g = contributes(q, s.qlim) ? e * v[j] : 0.f;
+                }
                 r0 += g;
                 const float gx = g * dx;
                 r1 += gx;
@@ -436,11 +505,19 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
             myy = fmaf(dy * dy, r0, myy);
         }
         if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
-            // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
-            const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
             float4* gp = reinterpret_cast<float4*>(isect_grads + kPartialStride * size_t(s.g));
-            gp[0] = make_float4(s.w * (a * mx + b * my), s.w * (b * mx + c * my), -0.5f * s.w * mxx, -s.w * mxy);
-            gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, 0.f);
+            if constexpr (BETA) {
+                const float a = s.A, b = 0.5f * s.B, c = s.C, k = s.w * s.b2d;
+                gp[0] = make_float4(k * (2.f * a * mx + 2.f * b * my), k * (2.f * b * mx + 2.f * c * my), -k * mxx,
+                                    -2.f * k * mxy);
+                gp[1] = make_float4(-k * myy, se, s.w * 0.6931471805599453f * sl, 0.f);
+            } else {
+                // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
+                const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
+                gp[0] = make_float4(s.w * (a * mx + b * my), s.w * (b * mx + c * my), -0.5f * s.w * mxx,
+                                    -s.w * mxy);
+                gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, 0.f);
+            }
         }
     }
 }
@@ -450,16 +527,21 @@ fused_raster_bwd_kernel(const Splat* __restrict__ packed, const int32_t* __restr
 // Pass 1, one thread per (camera, primitive): the record's intersection
 // partials [cum[v], cum[v+1]) summed in emission order -> conic/mu VJP
 // (double) -> projection VJP, giving this camera's contribution to v_mean (3),
-// v_covariance (6) and v_density (1), stored field-major ws[f][c][n].
+// v_covariance (6), v_density (1) and v_beta (1), stored field-major
+// ws[f][c][n].
 // Pass 2, one thread per primitive: the contributions summed over cameras in
 // order -> parameter-transform VJP. A partial is (v_mx, v_my, v_a, v_b, v_c,
-// v_w) with w = density * mu.
+// v_w[, v_b2d]) with w = density * mu [* ratio(beta)]; for Beta primitives
+// v_beta = v_b2d + v_mus mus dlog ratio/dbeta and the conic/mu VJP receives
+// v_mu = v_mus ratio (r2splat/explicit.py:227-231).
 // ---------------------------------------------------------------------------
-constexpr int kParamFields = 10;
+constexpr int kParamFields = 11;
 
+template <bool BETA>
 __global__ void __launch_bounds__(256)
 fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                         const float* __restrict__ scales, const float* __restrict__ dens,
+                        const float* __restrict__ ratio, const float* __restrict__ log_grad,
                         const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W, uint32_t H,
                         float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
                         const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
@@ -470,15 +552,21 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
     const size_t v = size_t(c) * N + n;
     const uint32_t n_live = live(d_n, capacity);
     const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f;
     for (uint32_t j = lo; j < hi; ++j) {
         const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
         const float4 a = gp[0];
-        const float2 b = *reinterpret_cast<const float2*>(gp + 1);
-        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w; g4 += b.x; g5 += b.y;
+        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
+        if constexpr (BETA) {
+            const float4 b = gp[1];
+            g4 += b.x; g5 += b.y; g6 += b.z;
+        } else {
+            const float2 b = *reinterpret_cast<const float2*>(gp + 1);
+            g4 += b.x; g5 += b.y;
+        }
     }
-    float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (g0 != 0.f || g1 != 0.f || g2 != 0.f || g3 != 0.f || g4 != 0.f || g5 != 0.f) {
+    float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (g0 != 0.f || g1 != 0.f || g2 != 0.f || g3 != 0.f || g4 != 0.f || g5 != 0.f || g6 != 0.f) {
         float q[4], mag, norm, R[9], cov[6];
         unit_quat(quats + 4 * size_t(n), q, mag, norm);
         quat_to_rot(q, R);
@@ -498,10 +586,20 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
             // the forward stored the conic and mu in float32
             const double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
             const double muf = double(float(mu));
-            out[9] = float(muf) * g5;
             const double vP[3] = {double(g2), double(g3), double(g4)};
+            double v_mu;
+            if constexpr (BETA) {
+                const float mus = float(mu) * ratio[n];
+                const float v_mus = g5 * dens[n];
+                out[9] = g5 * mus;
+                out[10] = g6 + v_mus * mus * log_grad[n];
+                v_mu = double(v_mus) * double(ratio[n]);
+            } else {
+                out[9] = float(muf) * g5;
+                v_mu = double(dens[n]) * double(g5);
+            }
             double gcov[6];
-            conic_mu_vjp_f64(cd, Pf, muf, vP, double(dens[n]) * double(g5), gcov);
+            conic_mu_vjp_f64(cd, Pf, muf, vP, v_mu, gcov);
             const float vm[2] = {g0, g1};
             float vc[6];
 #pragma unroll
@@ -517,11 +615,12 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
 __global__ void __launch_bounds__(256)
 fused_param_reduce_kernel(const float* __restrict__ quats, const float* __restrict__ scales, uint32_t C,
                           uint32_t N, const float* __restrict__ ws, float* __restrict__ v_means,
-                          float* __restrict__ v_quats, float* __restrict__ v_scales, float* __restrict__ v_dens) {
+                          float* __restrict__ v_quats, float* __restrict__ v_scales, float* __restrict__ v_dens,
+                          float* __restrict__ v_betas) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const size_t CN = size_t(C) * N;
-    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (uint32_t c = 0; c < C; ++c) {
 #pragma unroll
         for (int f = 0; f < kParamFields; ++f) acc[f] += ws[f * CN + size_t(c) * N + n];
@@ -539,6 +638,7 @@ fused_param_reduce_kernel(const float* __restrict__ quats, const float* __restri
         v_scales[3 * size_t(n) + k] = vs[k];
     }
     v_dens[n] = acc[9];
+    if (v_betas) v_betas[n] = acc[10];
 }
 
 // ---------------------------------------------------------------------------
@@ -551,11 +651,13 @@ struct FusedScratch {
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
-static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity) {
+static size_t record_bytes(int beta) { return beta ? sizeof(BetaSplat) : sizeof(Splat); }
+
+static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity, int beta) {
     FusedScratch L;
     const size_t CN = size_t(C) * N;
     size_t o = 0;
-    L.recs = o; o += al(CN * sizeof(Splat));
+    L.recs = o; o += al(CN * record_bytes(beta));
     L.rects = o; o += al(CN * sizeof(uint2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
     L.sort_bytes = sort_workspace_bytes(capacity);
@@ -581,15 +683,15 @@ static T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<cha
 using namespace r2s;
 
 extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
-                                          uint32_t tile_size, uint32_t capacity) {
+                                          uint32_t tile_size, uint32_t capacity, int beta) {
     if (tile_size == 0) return 0;
     const uint64_t n_tiles = uint64_t(C) * ((width + tile_size - 1) / tile_size) *
                              ((height + tile_size - 1) / tile_size);
     if (n_tiles >= (1ull << 32)) return 0;
-    return fused_scratch(C, N, uint32_t(n_tiles), capacity).bytes;
+    return fused_scratch(C, N, uint32_t(n_tiles), capacity, beta).bytes;
 }
 
-extern "C" size_t r2s_fused_record_bytes(void) { return sizeof(Splat); }
+extern "C" size_t r2s_fused_record_bytes(int beta) { return record_bytes(beta); }
 
 static int fused_check(uint32_t C, uint32_t N, uint32_t ntx, uint32_t nty, uint32_t capacity) {
     R2S_REQUIRE(uint64_t(C) * N < (1ull << 31), "C*N exceeds the int32 index range");
@@ -607,66 +709,72 @@ static int check_tile(uint32_t tile_size, uint32_t waves) {
     return R2S_OK;
 }
 
-// Launches KERNEL<T, WAVES> for the runtime tile size and wave count.
-template <template <int, int> class Launch, typename... Args>
-static void launch_tiled(uint32_t tile, uint32_t waves, Args&&... args) {
+// Launches Launch<T, WAVES, BETA>::run for the runtime tile size, wave count and primitive kind.
+template <template <int, int, bool> class Launch, typename... Args>
+static void launch_tiled(uint32_t tile, uint32_t waves, int beta, Args&&... args) {
     if (waves == 0) waves = kDefaultWaves;
-#define R2S_TW(T_, W_) \
-    if (tile == T_ && waves == W_) return Launch<T_, W_>::run(args...);
+#define R2S_TW(T_, W_)                                                   \
+    if (tile == T_ && waves == W_) {                                     \
+        if (beta) return Launch<T_, W_, true>::run(args...);             \
+        return Launch<T_, W_, false>::run(args...);                      \
+    }
     R2S_TW(8, 4) R2S_TW(8, 8) R2S_TW(8, 16)
     R2S_TW(16, 4) R2S_TW(16, 8) R2S_TW(16, 16)
     R2S_TW(32, 4) R2S_TW(32, 8) R2S_TW(32, 16)
 #undef R2S_TW
 }
 
-template <int T, int WAVES>
+template <int T, int WAVES, bool BETA>
 struct LaunchFwd {
-    static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
+    static void run(uint32_t n_tiles, hipStream_t stream, const void* packed, const int32_t* offsets,
                     const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
                     uint32_t W, uint32_t H, float* image) {
-        fused_raster_fwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            packed, offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, image);
+        fused_raster_fwd_kernel<T, WAVES, BETA><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            static_cast<const Rec<BETA>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W, H,
+            image);
     }
 };
 
-template <int T, int WAVES>
+template <int T, int WAVES, bool BETA>
 struct LaunchBwd {
-    static void run(uint32_t n_tiles, hipStream_t stream, const Splat* packed, const int32_t* offsets,
+    static void run(uint32_t n_tiles, hipStream_t stream, const void* packed, const int32_t* offsets,
                     const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
                     uint32_t W, uint32_t H, const float* v_image, float* isect_grads) {
-        fused_raster_bwd_kernel<T, WAVES><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            packed, offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, v_image, isect_grads);
+        fused_raster_bwd_kernel<T, WAVES, BETA><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            static_cast<const Rec<BETA>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W, H,
+            v_image, isect_grads);
     }
 };
 
-extern "C" int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
-                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
-                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
-                                 uint32_t* counters, r2s_stream_t stream) {
-    if (int rc = check_tile(tile_size, 0)) return rc;
+template <bool BETA>
+static int prepare_impl(const float* means, const float* quats, const float* scales, const float* densities,
+                        const float* betas, const float* beta_ratio, const float* viewmats, const float* Ks,
+                        uint32_t C, uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                        float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
+                        int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters,
+                        hipStream_t stream) {
+    using R = Rec<BETA>;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
-    const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity);
+    const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity, BETA);
     R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
     uint32_t* cum = record_offsets;
     if (CN > 0) {
         dim3 grid(ceil_div(N, 256), C);
-        fused_project_kernel<<<grid, 256, 0, stream>>>(means, quats, scales, densities, viewmats, Ks, width,
-                                                       height, eps, radius_clip, tile_size, ntx, nty, N,
-                                                       at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), cum);
+        fused_project_kernel<BETA><<<grid, 256, 0, stream>>>(means, quats, scales, densities, betas, beta_ratio,
+                                                             viewmats, Ks, width, height, eps, radius_clip, tile_size,
+                                                             ntx, nty, N, at<R>(scratch, L.recs),
+                                                             at<uint2>(scratch, L.rects), cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
     void* sort_ws = at<void>(scratch, L.sort);
     if (CN > 0) {
-        fused_emit_kernel<<<ceil_div(CN, 256), 256, 0, stream>>>(at<uint2>(scratch, L.rects), cum, CN, N, ntx,
-                                                                 ntx * nty, capacity, sort_keys_buffer(sort_ws),
-                                                                 sort_vals_buffer(sort_ws, capacity), counters + 1,
-                                                                 at<Splat>(scratch, L.recs));
+        fused_emit_kernel<R><<<ceil_div(CN, 256), 256, 0, stream>>>(
+            at<uint2>(scratch, L.rects), cum, CN, ntx, capacity, sort_keys_buffer(sort_ws),
+            sort_vals_buffer(sort_ws, capacity), counters + 1, at<R>(scratch, L.recs));
         if (int rc = check_launch("fused_emit")) return rc;
     }
     uint32_t* keys = at<uint32_t>(scratch, L.keys);
@@ -686,9 +794,9 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
                                 stream))
         return rc;
     const uint32_t pack_blocks = (n_tiles + 7u) & ~7u;  // a multiple of the 8 XCDs
-    fused_pack_kernel<<<pack_blocks, 256, 0, stream>>>(at<Splat>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C,
-                                                       ntx * nty, ntx, vals, tm, cm, counters, capacity,
-                                                       static_cast<Splat*>(packed), tile_offsets);
+    fused_pack_kernel<R><<<pack_blocks, 256, 0, stream>>>(at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C,
+                                                          ntx * nty, ntx, vals, tm, cm, counters, capacity,
+                                                          static_cast<R*>(packed), tile_offsets);
     if (int rc = check_launch("fused_pack")) return rc;
     if (tile_order) {
         void* order_ws = at<void>(scratch, L.order_sort);
@@ -703,31 +811,49 @@ extern "C" int r2s_fused_prepare(const float* means, const float* quats, const f
     return R2S_OK;
 }
 
+extern "C" int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
+                                 const float* densities, const float* betas, const float* beta_ratio,
+                                 const float* viewmats, const float* Ks, uint32_t C, uint32_t N, uint32_t width,
+                                 uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
+                                 void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
+                                 uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters,
+                                 r2s_stream_t stream) {
+    if (int rc = check_tile(tile_size, 0)) return rc;
+    R2S_REQUIRE((betas == nullptr) == (beta_ratio == nullptr), "betas and beta_ratio go together");
+    if (betas)
+        return prepare_impl<true>(means, quats, scales, densities, betas, beta_ratio, viewmats, Ks, C, N, width,
+                                  height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed,
+                                  tile_offsets, tile_order, record_offsets, counters, stream);
+    return prepare_impl<false>(means, quats, scales, densities, nullptr, nullptr, viewmats, Ks, C, N, width, height,
+                               tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
+                               tile_order, record_offsets, counters, stream);
+}
+
 extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
-                                const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
-                                uint32_t waves, float* image, r2s_stream_t stream) {
+                                const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width,
+                                uint32_t height, uint32_t tile_size, uint32_t waves, int beta, float* image,
+                                r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     const uint64_t n_tiles = uint64_t(C) * ntx * nty;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    launch_tiled<LaunchFwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
-                            tile_offsets, tile_order, counters, capacity, ntx, nty, width, height, image);
+    launch_tiled<LaunchFwd>(tile_size, waves, beta, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
+                            counters, capacity, ntx, nty, width, height, image);
     return check_launch("fused_raster_fwd");
 }
 
 extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
-                                    const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                                    uint32_t tile_size, uint32_t waves, const float* v_image, float* isect_grads,
-                                    r2s_stream_t stream) {
+                                    const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width,
+                                    uint32_t height, uint32_t tile_size, uint32_t waves, int beta,
+                                    const float* v_image, float* isect_grads, r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     const uint64_t n_tiles = uint64_t(C) * ntx * nty;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    launch_tiled<LaunchBwd>(tile_size, waves, uint32_t(n_tiles), stream, static_cast<const Splat*>(packed),
-                            tile_offsets, tile_order, counters, capacity, ntx, nty, width, height, v_image,
-                            isect_grads);
+    launch_tiled<LaunchBwd>(tile_size, waves, beta, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
+                            counters, capacity, ntx, nty, width, height, v_image, isect_grads);
     return check_launch("fused_raster_bwd");
 }
 
@@ -736,52 +862,64 @@ extern "C" size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N) {
 }
 
 extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
-                                   const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                   const float* densities, const float* betas, const float* beta_ratio,
+                                   const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
                                    uint32_t N, uint32_t width, uint32_t height, float eps,
                                    const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
                                    const float* isect_grads, void* workspace, size_t workspace_bytes,
                                    float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                                   r2s_stream_t stream) {
+                                   float* v_betas, r2s_stream_t stream) {
     if (N == 0) return R2S_OK;
     if (int rc = fused_check(C, N, 1, 1, capacity)) return rc;
     R2S_REQUIRE(workspace_bytes >= r2s_fused_param_workspace_bytes(C, N), "param workspace too small: %zu < %zu",
                 workspace_bytes, r2s_fused_param_workspace_bytes(C, N));
+    const bool beta = betas != nullptr;
+    R2S_REQUIRE(!beta || (beta_ratio && beta_log_grad && v_betas), "Beta primitives need ratio, log-gradient and "
+                "v_betas buffers");
     float* ws = static_cast<float*>(workspace);
-    fused_camera_vjp_kernel<<<dim3(ceil_div(N, 256), C), 256, 0, stream>>>(
-        means, quats, scales, densities, viewmats, Ks, width, height, eps, C, N, record_offsets, counters, capacity,
-        isect_grads, ws);
+    const dim3 grid(ceil_div(N, 256), C);
+    if (beta)
+        fused_camera_vjp_kernel<true><<<grid, 256, 0, stream>>>(
+            means, quats, scales, densities, beta_ratio, beta_log_grad, viewmats, Ks, width, height, eps, C, N,
+            record_offsets, counters, capacity, isect_grads, ws);
+    else
+        fused_camera_vjp_kernel<false><<<grid, 256, 0, stream>>>(
+            means, quats, scales, densities, nullptr, nullptr, viewmats, Ks, width, height, eps, C, N,
+            record_offsets, counters, capacity, isect_grads, ws);
     if (int rc = check_launch("fused_camera_vjp")) return rc;
     fused_param_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(quats, scales, C, N, ws, v_means, v_quats,
-                                                                    v_scales, v_densities);
+                                                                    v_scales, v_densities, beta ? v_betas : nullptr);
     return check_launch("fused_param_reduce");
 }
 
 extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
-                                 const float* densities, const float* viewmats, const float* Ks, uint32_t C,
-                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
-                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
-                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
-                                 uint32_t* counters, float* image, r2s_stream_t stream) {
-    if (int rc = r2s_fused_prepare(means, quats, scales, densities, viewmats, Ks, C, N, width, height, tile_size,
-                                   eps, radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
-                                   tile_order, record_offsets, counters, stream))
+                                 const float* densities, const float* betas, const float* beta_ratio,
+                                 const float* viewmats, const float* Ks, uint32_t C, uint32_t N, uint32_t width,
+                                 uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
+                                 void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
+                                 uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters, float* image,
+                                 r2s_stream_t stream) {
+    if (int rc = r2s_fused_prepare(means, quats, scales, densities, betas, beta_ratio, viewmats, Ks, C, N, width,
+                                   height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed,
+                                   tile_offsets, tile_order, record_offsets, counters, stream))
         return rc;
     return r2s_fused_render(packed, tile_offsets, tile_order, counters, capacity, C, width, height, tile_size, 0,
-                            image, stream);
+                            betas != nullptr, image, stream);
 }
 
 extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
-                                  const float* densities, const float* viewmats, const float* Ks, uint32_t C,
+                                  const float* densities, const float* betas, const float* beta_ratio,
+                                  const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
                                   uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                                   uint32_t capacity, const void* packed, const int32_t* tile_offsets,
                                   const uint32_t* tile_order, const uint32_t* record_offsets,
                                   const uint32_t* counters, float* isect_grads, void* workspace,
                                   size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
-                                  float* v_scales, float* v_densities, r2s_stream_t stream) {
+                                  float* v_scales, float* v_densities, float* v_betas, r2s_stream_t stream) {
     if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, tile_order, counters, capacity, C, width, height,
-                                      tile_size, 0, v_image, isect_grads, stream))
+                                      tile_size, 0, betas != nullptr, v_image, isect_grads, stream))
         return rc;
-    return r2s_fused_param_vjp(means, quats, scales, densities, viewmats, Ks, C, N, width, height, eps,
-                               record_offsets, counters, capacity, isect_grads, workspace, workspace_bytes, v_means,
-                               v_quats, v_scales, v_densities, stream);
+    return r2s_fused_param_vjp(means, quats, scales, densities, betas, beta_ratio, beta_log_grad, viewmats, Ks, C, N,
+                               width, height, eps, record_offsets, counters, capacity, isect_grads, workspace,
+                               workspace_bytes, v_means, v_quats, v_scales, v_densities, v_betas, stream);
 }

@@ -32,13 +32,14 @@ RASTER_TILE = 16
 VOXEL_TILE = 8
 _FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
 
-# "fused": float32 Gaussian mean-only projections run the fused pipeline
-# (r2splat_amd.fused); every other case, and "ops", runs one operator per stage.
+# "fused": float32 mean-only projections (Gaussian or Beta) run the fused
+# pipeline (r2splat_amd.fused); every other case, and "ops", runs one operator
+# per stage.
 BACKEND = os.environ.get("R2S_BACKEND", "fused")
 
 
 def _use_fused(means, beta, uncertainty) -> bool:
-    return BACKEND == "fused" and means.dtype == torch.float32 and beta is None and uncertainty is None
+    return BACKEND == "fused" and means.dtype == torch.float32 and uncertainty is None
 
 
 @dataclass
@@ -133,7 +134,7 @@ def _raster_state(prims, viewmats, Ks, width, height, eps, radius_clip) -> dict:
     C = viewmats.shape[0]
     if _use_fused(means, beta, uncertainty):
         return {"fused": fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,
-                                       radius_clip)}
+                                       radius_clip, beta=beta)}
     T = RASTER_TILE
     with stage("project"):
         covars, _ = ops.parameter_transform_fwd(quats, scales)
@@ -252,9 +253,8 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
         _, height, width = context.shape
         b.update(_raster_state(context.primitives, b["viewmats"], b["Ks"], width, height, *b.pop("recompute")))
     if "fused" in b:
-        v_means, v_quats, v_scales, v_dens = fused.backward(b["fused"], means, quats, scales, densities,
-                                                            b["viewmats"], b["Ks"], v_image)
-        return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_dens))
+        grads = fused.backward(b["fused"], means, quats, scales, densities, b["viewmats"], b["Ks"], v_image)
+        return _consume(context, ParameterGradients(*grads))
     _, height, width = context.shape
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""

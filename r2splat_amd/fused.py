@@ -1,4 +1,4 @@
-"""Fused float32 Gaussian rasterization (8, 16 or 32 pixel tiles) on the HIP library.
+"""Fused float32 Gaussian / Beta rasterization (8, 16 or 32 pixel tiles) on the HIP library.
 
 ``FusedForward`` holds what the backward needs (per-intersection records,
 tile offsets, per-(camera, primitive) intersection offsets, device counters)
@@ -7,15 +7,21 @@ sums them per primitive in a fixed order, so it is bitwise reproducible. Capacit
 intersection list is sized from the previous call with the same shapes
 (``CapacityCache``) and grown on overflow, so steady-state calls perform no
 host-visible sizing besides the overflow check.
+
+Beta primitives (``beta`` given) carry the per-primitive line-integral factor
+ratio(beta) and d log ratio / d beta, computed once per forward in float64
+as r2splat/_beta.py does and cast to float32 (explicit.py:166-170, 226-230).
 """
 
 from __future__ import annotations
 
 import math
 from dataclasses import dataclass
+from typing import Optional
 
 import torch
 
+from ._beta import beta_path_log_grad, beta_path_ratio
 from .hip import _backend as be
 from .hip._backend import call, ptr
 from .profiling import stage
@@ -66,24 +72,42 @@ class FusedForward:
     height: int
     eps: float
     tile_size: int = TILE
+    beta: Optional[torch.Tensor] = None       # [N] (Beta primitives) or None (Gaussian)
+    ratio: Optional[torch.Tensor] = None      # [N] B(1/2, beta + 1) / sqrt(2 pi)
+    log_grad: Optional[torch.Tensor] = None   # [N] d log ratio / d beta
+
+    @property
+    def is_beta(self) -> int:
+        return int(self.beta is not None)
 
 
-def _scratch(C, N, width, height, tile_size, capacity, device):
-    nbytes = be.lib().r2s_fused_scratch_bytes(C, N, width, height, tile_size, capacity)
+def _scratch(C, N, width, height, tile_size, capacity, beta, device):
+    nbytes = be.lib().r2s_fused_scratch_bytes(C, N, width, height, tile_size, capacity, beta)
     return torch.empty((nbytes,), dtype=torch.uint8, device=device)
 
 
+def beta_factors(beta: torch.Tensor):
+    """(ratio, log_grad) of float32 betas, as the explicit path computes them."""
+    return beta_path_ratio(beta).contiguous(), beta_path_log_grad(beta).contiguous()
+
+
 def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=None, tile_size: int = TILE) -> FusedForward:
+            check=None, tile_size: int = TILE, beta=None, factors=None) -> FusedForward:
     """Projection, binning, sort and record packing (everything but the image). float32, one device.
 
     ``check`` (default: unless the stream is being captured into a graph)
     reads the overflow flag and re-runs with more capacity. Under capture the
     capacity of the last eager call with the same shapes is used and the flag
-    is left on the device (``FusedForward.counters[1]``) for the caller."""
+    is left on the device (``FusedForward.counters[1]``) for the caller.
+    ``beta`` [N] selects Beta primitives; ``factors`` = beta_factors(beta)
+    may be passed in to reuse them."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
-    key = (device, C, N, height, width, tile_size)
+    kind = int(beta is not None)
+    ratio = log_grad = None
+    if kind:
+        ratio, log_grad = factors if factors is not None else beta_factors(beta)
+    key = (device, C, N, height, width, tile_size, kind)
     capturing = torch.cuda.is_current_stream_capturing()
     if check is None:
         check = not capturing
@@ -92,7 +116,7 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
                            "same shapes first")
     cap = int(capacity or CAPACITY.get(key, C, N))
     ntx, nty = -(-width // tile_size), -(-height // tile_size)
-    rec = be.lib().r2s_fused_record_bytes()
+    rec = be.lib().r2s_fused_record_bytes(kind)
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
     tile_order = torch.empty((C * nty * ntx,), dtype=torch.int32, device=device)
     record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
@@ -101,11 +125,11 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
         stream = be.stream_of(device)
         n_isects = -1
         while True:
-            scratch = _scratch(C, N, width, height, tile_size, cap, device)
+            scratch = _scratch(C, N, width, height, tile_size, cap, kind, device)
             packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
             with stage("project_bin"):
-                call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
-                     ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
+                call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(beta), ptr(ratio),
+                     ptr(viewmats), ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
                      scratch.numel(), ptr(packed), ptr(offsets), ptr(tile_order), ptr(record_offsets), ptr(counters),
                      stream)
             del scratch
@@ -119,7 +143,7 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     global LAST_COUNTERS
     LAST_COUNTERS = counters
     return FusedForward(packed, offsets, tile_order, record_offsets, counters, cap, n_isects, width, height,
-                        float(eps), tile_size)
+                        float(eps), tile_size, beta, ratio, log_grad)
 
 
 def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
@@ -129,20 +153,21 @@ def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
         with stage("rasterize_fwd"):
             call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
                  fwd.capacity, C,
-                 fwd.width, fwd.height, fwd.tile_size, waves, ptr(image), be.stream_of(device))
+                 fwd.width, fwd.height, fwd.tile_size, waves, fwd.is_beta, ptr(image), be.stream_of(device))
     return image
 
 
 def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=None, tile_size: int = TILE, waves: int = 0):
+            check=None, tile_size: int = TILE, waves: int = 0, beta=None):
     """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
     fwd = prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip,
-                  capacity=capacity, check=check, tile_size=tile_size)
+                  capacity=capacity, check=check, tile_size=tile_size, beta=beta)
     return render(fwd, viewmats.shape[0], means.device, waves), fwd
 
 
 def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image, waves: int = 0):
-    """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities)."""
+    """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities[, v_beta]);
+    v_beta only for Beta primitives."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
     grad = torch.empty((fwd.capacity * 8,), dtype=torch.float32, device=device)
@@ -151,15 +176,18 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     v_quats = torch.empty_like(quats)
     v_scales = torch.empty_like(scales)
     v_dens = torch.empty_like(densities)
+    v_beta = torch.empty_like(fwd.beta) if fwd.is_beta else None
     with torch.cuda.device(device):
         stream = be.stream_of(device)
         with stage("rasterize_bwd"):
             call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
                  fwd.capacity, C,
-                 fwd.width, fwd.height, fwd.tile_size, waves, ptr(v_image), ptr(grad), stream)
+                 fwd.width, fwd.height, fwd.tile_size, waves, fwd.is_beta, ptr(v_image), ptr(grad), stream)
         with stage("param_bwd"):
-            call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(viewmats),
-                 ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps, ptr(fwd.record_offsets), ptr(fwd.counters),
-                 fwd.capacity, ptr(grad), ptr(ws), ws.numel(), ptr(v_means), ptr(v_quats), ptr(v_scales),
-                 ptr(v_dens), stream)
+            call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(fwd.beta),
+                 ptr(fwd.ratio), ptr(fwd.log_grad), ptr(viewmats), ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps,
+                 ptr(fwd.record_offsets), ptr(fwd.counters), fwd.capacity, ptr(grad), ptr(ws), ws.numel(),
+                 ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens), ptr(v_beta), stream)
+    if fwd.is_beta:
+        return v_means, v_quats, v_scales, v_dens, v_beta
     return v_means, v_quats, v_scales, v_dens

@@ -6,18 +6,17 @@ server admits a queued request as soon as its HBM footprint fits the budget
 (in arrival order, letting a later request that fits pass one that does
 not), keeps every admitted job's parameters, optimizer state, target images
 and intermediates resident on the device, and on every tick issues one
-iteration of every live job: Gaussian jobs replay their own captured
-hipGraph (fused fwd + MSE + bwd + Adam) on one of a few streams, so jobs
-overlap on the device; Beta jobs step through the explicit operator path.
-A job leaves, and frees its memory, when its last iteration has been
-applied; time-to-finished-fit is measured from its arrival.
+iteration of every live job: each job (Gaussian or Beta primitives)
+replays its own captured hipGraph (fused fwd + MSE + bwd + Adam) on one of a
+few streams, so jobs overlap on the device. A job leaves, and frees its
+memory, when its last iteration has been applied; time-to-finished-fit is
+measured from its arrival.
 
 Updates are Adam followed by a clamp to the explicit API's domain (scales >=
 ``min_scale``, densities >= 0, beta >= ``min_beta``) so that the same job can
-be stepped by the plain explicit path for comparison. A job's result does not
-depend on which other jobs share the device with it: the Gaussian path is
-bitwise deterministic, the Beta path equal up to the atomic summation order
-of its backward.
+be stepped by the plain explicit path (``ExplicitFit``) for comparison. A
+job's result does not depend on which other jobs share the device with it:
+the fused path is bitwise deterministic for both primitive kinds.
 
 The reference has no serving layer; this is the serving capability built on
 the same operators.
@@ -82,17 +81,17 @@ def build_job(req: FitRequest, device) -> dict:
 
 
 class ExplicitFit:
-    """One job stepped through the explicit operator path (any primitive
-    kind): rasterize_fwd -> MSE -> rasterize_bwd -> torch Adam -> clamp."""
+    """One job stepped through the explicit API eagerly (any primitive kind):
+    rasterize_fwd -> MSE -> rasterize_bwd -> torch Adam -> clamp. The
+    un-captured, one-call-per-stage baseline of the server's steps."""
 
-    def __init__(self, job: dict, adam: AdamConfig, min_beta: float = 1e-3, stream=None):
+    def __init__(self, job: dict, adam: AdamConfig, stream=None):
         self.job = job
         self.names = [k for k in ("means", "quats", "scales", "densities", "beta") if job.get(k) is not None]
         self.params = [job[k].detach().clone() for k in self.names]
         self.opt = torch.optim.Adam(self.params, lr=adam.lr, betas=(adam.beta1, adam.beta2), eps=adam.eps,
                                     foreach=False)
         self.adam = adam
-        self.min_beta = min_beta
         self.stream = stream
         self.applied = 0
         self.loss = None
@@ -115,7 +114,7 @@ class ExplicitFit:
                 self.params[k].clamp_(min=self.adam.min_scale or 1e-6)
                 self.params[self.names.index("densities")].clamp_(min=0.0)
                 if "beta" in self.names:
-                    self.params[self.names.index("beta")].clamp_(min=self.min_beta)
+                    self.params[self.names.index("beta")].clamp_(min=self.adam.min_beta)
         self.applied += 1
 
     def resident_bytes(self) -> int:
@@ -184,17 +183,10 @@ class FitServer:
         job = build_job(req, self.device)
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
-        if req.beta:
-            fit = ExplicitFit(job, self.adam, stream=stream)
-            nbytes = self.estimate_bytes(req)
-            fused = False
-        else:
-            fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
-                              job["target"], adam=self.adam, stream=stream, headroom=self.headroom)
-            fit.capture()
-            nbytes = fit.memory_bytes()
-            fused = True
-        self.live.append(_Live(req, fit, fused, now, 0, nbytes))
+        fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
+                          job["target"], adam=self.adam, stream=stream, headroom=self.headroom, beta=job["beta"])
+        fit.capture()
+        self.live.append(_Live(req, fit, True, now, 0, fit.memory_bytes()))
         return True
 
     def run(self, requests: List[FitRequest], clock=None, max_wall: Optional[float] = None,

@@ -1,8 +1,8 @@
-"""Graph-captured fits of float32 Gaussian scenes to target images.
+"""Graph-captured fits of float32 Gaussian or Beta scenes to target images.
 
 ``GaussianFit`` owns one reconstruction job: a flat parameter buffer (means,
-quats, scales, densities as views), its gradient buffer, Adam state and every
-intermediate of the fused pipeline, all allocated once. One step is
+quats, scales, densities[, betas] as views), its gradient buffer, Adam state
+and every intermediate of the fused pipeline, all allocated once. One step is
 
     fused prepare -> render -> MSE adjoint -> raster VJP -> parameter VJP
     [-> Adam]
@@ -29,6 +29,7 @@ from typing import Optional
 import torch
 
 from . import fused
+from ._beta import beta_path_log_grad, beta_path_ratio
 from .hip import _backend as be
 from .hip._backend import call, ptr
 
@@ -54,11 +55,11 @@ class Workspace:
             return t, False
         return torch.empty((max(1, int(nbytes)),), dtype=torch.uint8, device=device), True
 
-    def ensure(self, C, N, W, H, tile_size, capacity):
+    def ensure(self, C, N, W, H, tile_size, capacity, beta: int = 0):
         lib = be.lib()
         changed = False
-        for name, nbytes in (("scratch", lib.r2s_fused_scratch_bytes(C, N, W, H, tile_size, capacity)),
-                             ("packed", capacity * lib.r2s_fused_record_bytes()),
+        for name, nbytes in (("scratch", lib.r2s_fused_scratch_bytes(C, N, W, H, tile_size, capacity, beta)),
+                             ("packed", capacity * lib.r2s_fused_record_bytes(beta)),
                              ("isect_grads", capacity * 8 * 4),
                              ("param_ws", lib.r2s_fused_param_workspace_bytes(C, N))):
             t, grown = self._grow(getattr(self, name), nbytes, self.device)
@@ -77,18 +78,20 @@ class AdamConfig:
     beta1: float = 0.9
     beta2: float = 0.999
     eps: float = 1e-8
-    # after each update, clamp scales to >= min_scale and densities to >= 0 (the
-    # explicit API's domain); None leaves the raw update
+    # after each update, clamp scales to >= min_scale, densities to >= 0 and
+    # betas to >= min_beta (the explicit API's domain); None leaves the raw update
     min_scale: Optional[float] = None
+    min_beta: float = 1e-3
 
 
 class GaussianFit:
-    """One job: N Gaussians, C views of H x W, MSE against ``target`` [C, H, W]."""
+    """One job: N Gaussians (or Beta primitives, when ``beta`` [N] is given),
+    C views of H x W, MSE against ``target`` [C, H, W]."""
 
     def __init__(self, means, quats, scales, densities, viewmats, Ks, target, *, eps: float = 1e-8,
-                 radius_clip: float = 3.0, adam: Optional[AdamConfig] = None, capacity: Optional[int] = None,
+                 radius_clip: Optional[float] = None, adam: Optional[AdamConfig] = None, capacity: Optional[int] = None,
                  headroom: float = 1.15, stream: Optional[torch.cuda.Stream] = None, tile_size: int = TILE,
-                 fwd_waves: int = 0, bwd_waves: int = 0, workspace: Optional[Workspace] = None):
+                 fwd_waves: int = 0, bwd_waves: int = 0, workspace: Optional[Workspace] = None, beta=None):
         device = means.device
         if means.dtype != torch.float32:
             raise TypeError("GaussianFit runs the float32 fused pipeline")
@@ -98,18 +101,27 @@ class GaussianFit:
         self.H, self.W = int(target.shape[-2]), int(target.shape[-1])
         if tuple(target.shape) != (self.C, self.H, self.W):
             raise ValueError("target must be [C, H, W]")
+        if radius_clip is None:  # the explicit API's default: 3 sigma (Gaussian), the unit support (Beta)
+            radius_clip = 3.0 if beta is None else 1.0
         self.eps, self.radius_clip = float(eps), float(radius_clip)
         self.adam = adam
         self.headroom = headroom
         self.tile_size, self.fwd_waves, self.bwd_waves = int(tile_size), int(fwd_waves), int(bwd_waves)
         self.stream = stream
         N = self.N
-        self.params = torch.empty((11 * N,), dtype=torch.float32, device=device)
+        self.kind = int(beta is not None)
+        self.params = torch.empty(((11 + self.kind) * N,), dtype=torch.float32, device=device)
         self.grads = torch.zeros_like(self.params)
-        self.means, self.quats, self.scales, self.densities = self._views(self.params)
-        self.v_means, self.v_quats, self.v_scales, self.v_densities = self._views(self.grads)
-        for dst, src in zip((self.means, self.quats, self.scales, self.densities), (means, quats, scales, densities)):
-            dst.copy_(src)
+        self.means, self.quats, self.scales, self.densities, self.beta = self._views(self.params)
+        self.v_means, self.v_quats, self.v_scales, self.v_densities, self.v_beta = self._views(self.grads)
+        for dst, src in zip((self.means, self.quats, self.scales, self.densities, self.beta),
+                            (means, quats, scales, densities, beta)):
+            if dst is not None:
+                dst.copy_(src)
+        self.beta_ratio = self.beta_log_grad = None
+        if self.kind:
+            self.beta_ratio = torch.empty((N,), dtype=torch.float32, device=device)
+            self.beta_log_grad = torch.empty((N,), dtype=torch.float32, device=device)
         self.viewmats = viewmats.contiguous().to(device, torch.float32)
         self.Ks = Ks.contiguous().to(device, torch.float32)
         self.target = target.contiguous().to(device, torch.float32)
@@ -137,11 +149,11 @@ class GaussianFit:
     def _views(self, flat):
         N = self.N
         return (flat[: 3 * N].view(N, 3), flat[3 * N: 7 * N].view(N, 4), flat[7 * N: 10 * N].view(N, 3),
-                flat[10 * N:])
+                flat[10 * N: 11 * N], flat[11 * N:] if self.kind else None)
 
     def _alloc(self, capacity: int):
         self.capacity = max(1024, int(capacity))
-        self.ws.ensure(self.C, self.N, self.W, self.H, self.tile_size, self.capacity)
+        self.ws.ensure(self.C, self.N, self.W, self.H, self.tile_size, self.capacity, self.kind)
 
     @property
     def graph_stale(self) -> bool:
@@ -149,7 +161,7 @@ class GaussianFit:
 
     def _measure_capacity(self) -> int:
         fwd = fused.prepare(self.means, self.quats, self.scales, self.densities, self.viewmats, self.Ks, self.W,
-                            self.H, self.eps, self.radius_clip, tile_size=self.tile_size)
+                            self.H, self.eps, self.radius_clip, tile_size=self.tile_size, beta=self.beta)
         return int(math.ceil(fwd.n_isects * self.headroom)) + 1024
 
     def resident_bytes(self) -> int:
@@ -159,6 +171,8 @@ class GaussianFit:
                    self.tile_order, self.record_offsets, self.target, self.counters, self.adam_state]
         if self.adam is not None:
             tensors += [self.exp_avg, self.exp_avg_sq]
+        if self.kind:
+            tensors += [self.beta_ratio, self.beta_log_grad]
         return sum(t.numel() * t.element_size() for t in tensors)
 
     def memory_bytes(self) -> int:
@@ -171,21 +185,26 @@ class GaussianFit:
         s = be.stream_of(self.device)
         C, N, W, H = self.C, self.N, self.W, self.H
         ws = self.ws
+        if self.kind:  # line-integral factors of the current betas (float64 math, as the explicit path)
+            self.beta_ratio.copy_(beta_path_ratio(self.beta))
+            self.beta_log_grad.copy_(beta_path_log_grad(self.beta))
         call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
-             ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size, self.eps, self.radius_clip, self.capacity,
+             ptr(self.beta), ptr(self.beta_ratio), ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size,
+             self.eps, self.radius_clip, self.capacity,
              ptr(ws.scratch), ws.scratch.numel(), ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.record_offsets), ptr(self.counters), s)
         call("r2s_fused_render", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
-             self.capacity, C, W, H, self.tile_size, self.fwd_waves, ptr(self.image), s)
+             self.capacity, C, W, H, self.tile_size, self.fwd_waves, self.kind, ptr(self.image), s)
         call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
              ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
         call("r2s_fused_raster_vjp", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
-             ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, ptr(self.v_image),
+             ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, self.kind, ptr(self.v_image),
              ptr(ws.isect_grads), s)
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
-             ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.eps, ptr(self.record_offsets), ptr(self.counters),
-             self.capacity, ptr(ws.isect_grads), ptr(ws.param_ws), ws.param_ws.numel(), ptr(self.v_means),
-             ptr(self.v_quats), ptr(self.v_scales), ptr(self.v_densities), s)
+             ptr(self.beta), ptr(self.beta_ratio), ptr(self.beta_log_grad), ptr(self.viewmats), ptr(self.Ks), C, N,
+             W, H, self.eps, ptr(self.record_offsets), ptr(self.counters), self.capacity, ptr(ws.isect_grads),
+             ptr(ws.param_ws), ws.param_ws.numel(), ptr(self.v_means), ptr(self.v_quats), ptr(self.v_scales),
+             ptr(self.v_densities), ptr(self.v_beta), s)
         if self.adam is not None:
             a = self.adam
             call("r2s_adam_step", ptr(self.params), ptr(self.grads), ptr(self.exp_avg), ptr(self.exp_avg_sq),
@@ -194,6 +213,8 @@ class GaussianFit:
             if a.min_scale is not None:
                 call("r2s_clamp_min", ptr(self.scales), self.scales.numel(), float(a.min_scale), s)
                 call("r2s_clamp_min", ptr(self.densities), self.densities.numel(), 0.0, s)
+                if self.kind:
+                    call("r2s_clamp_min", ptr(self.beta), self.beta.numel(), float(a.min_beta), s)
 
     def _ctx(self):
         return torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.device(self.device)

@@ -110,14 +110,17 @@ def config3_specs(n_jobs: int = 32, seed: int = 0):
     return specs
 
 
-def fit_job(n: int, views: int, side: int, job_seed: int, device="cuda", scale_range=(0.005, 0.05)):
+def fit_job(n: int, views: int, side: int, job_seed: int, device="cuda", scale_range=(0.005, 0.05),
+            beta_range=None):
     """One reconstruction job as in config 1: parameters from `job_seed`, target
-    images rendered from a second Gaussian set (`job_seed + 1`)."""
+    images rendered from a second primitive set (`job_seed + 1`); Beta
+    primitives (betas uniform in `beta_range`) when it is given."""
     from .explicit import rasterize_fwd
 
-    sc = scene(n, views, side, side, scale_range=scale_range, seed=job_seed, device=device)
-    tg = scene(n, views, side, side, scale_range=scale_range, seed=job_seed + 1, device=device)
+    sc = scene(n, views, side, side, scale_range=scale_range, seed=job_seed, device=device, beta_range=beta_range)
+    tg = scene(n, views, side, side, scale_range=scale_range, seed=job_seed + 1, device=device,
+               beta_range=beta_range)
     with torch.no_grad():
-        target, _ = rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size)
-    return dict(means=sc.means, quats=sc.quats, scales=sc.scales, densities=sc.densities, viewmats=sc.viewmats,
-                Ks=sc.Ks, target=target)
+        target, _ = rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size, beta=tg.beta)
+    return dict(means=sc.means, quats=sc.quats, scales=sc.scales, densities=sc.densities, beta=sc.beta,
+                viewmats=sc.viewmats, Ks=sc.Ks, target=target)

@@ -81,3 +81,25 @@ def test_autotuner_times_every_candidate_and_caches(device):
     assert batch.fits[0].tile_size == rec.config.tile_size
     batch.step(1)
     assert batch.sync() == 1
+
+
+def test_mixed_gaussian_and_beta_batch_equals_jobs_alone(device):
+    adam = AdamConfig(lr=1e-3, min_scale=1e-6)
+    jobs = []
+    for k, (n, c, side, seed) in enumerate(SPECS[:3]):
+        job = workloads.fit_job(n, c, side, seed, device=device, scale_range=(0.01, 0.08),
+                                beta_range=(0.5, 4.0) if k != 1 else None)
+        job["config"] = CONFIGS[k]
+        jobs.append(job)
+    batch = BatchedFit(jobs, adam=adam, n_streams=2)
+    batch.step(3)
+    assert batch.sync() == 3
+    for job, fit in zip(jobs, batch.fits):
+        cfg = job["config"]
+        ref = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
+                          job["target"], adam=adam, tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves,
+                          bwd_waves=cfg.bwd_waves, beta=job["beta"])
+        ref.capture()
+        ref.step(3)
+        assert ref.sync() == 3
+        assert torch.equal(fit.params, ref.params)

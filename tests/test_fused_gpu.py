@@ -161,3 +161,101 @@ def test_fused_tile_sizes_and_wave_counts_match_oracle(device, tile, waves):
     grads8 = fused.backward(f8, *sc.params(), sc.viewmats, sc.Ks, v, waves=8)
     for a, b in zip(grads, grads8):
         assert torch.equal(a, b)
+
+
+# --------------------------------------------------------------------------- Beta primitives
+def _beta_scene(device, n=700, c=3, H=56, W=72, seed=6):
+    return workloads.scene(n, c, H, W, scale_range=(0.02, 0.1), seed=seed, device=device, beta_range=(0.5, 4.0))
+
+
+def _beta_oracle(sc, v, tile, clip=1.0):
+    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
+    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
+                                                     1e-8, clip)
+    mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), tile, sc.width, sc.height)
+    leaves = [t.detach().cpu().double().requires_grad_() for t in sc.params() + (sc.beta,)]
+    ref = O.rasterize_scene(*leaves[:4], sc.viewmats.cpu().double(), sc.Ks.cpu().double(), sc.image_size, eps=1e-8,
+                            beta=leaves[4], mask=mask)
+    return ref, torch.autograd.grad((ref * v.cpu().double()).sum(), leaves)
+
+
+@pytest.mark.parametrize("tile,waves", [(16, 8), (8, 4), (32, 16)])
+def test_fused_beta_matches_oracle(device, tile, waves):
+    """Beta primitives (alpha = w (1 - d^2)^(beta + 1/2), w = density mu ratio(beta)):
+    image to 1e-5 and all five gradients, beta included, to 1e-4 of the fp64 oracle."""
+    sc = _beta_scene(device)
+    W, H = sc.width, sc.height
+    img, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 1.0, tile_size=tile, waves=waves,
+                           beta=sc.beta)
+    _, ids, _ = ops.isect_tiles(*_proj_clip(sc, 1.0), tile, -(-W // tile), -(-H // tile))
+    assert f.n_isects == ids.numel()
+    v = (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
+    grads = fused.backward(f, *sc.params(), sc.viewmats, sc.Ks, v, waves=waves)
+    assert len(grads) == 5
+    ref, exp = _beta_oracle(sc, v, tile)
+    assert O.relative_error(img, ref) <= 1e-5
+    for name, g, e in zip(("means", "quats", "scales", "densities", "beta"), grads, exp):
+        err = O.relative_error(g, e)
+        assert err <= 1e-4, f"{name}: {err:.2e}"
+    # the backward does not depend on the wave count
+    img8, f8 = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 1.0, tile_size=tile, waves=8,
+                             beta=sc.beta)
+    grads8 = fused.backward(f8, *sc.params(), sc.viewmats, sc.Ks, v, waves=8)
+    for a, b in zip(grads, grads8):
+        assert torch.equal(a, b)
+
+
+def _proj_clip(sc, clip):
+    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
+    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
+                                                     1e-8, clip)
+    return m2, radii
+
+
+def _run_beta(sc, backend, v_fn=None):
+    old = explicit.BACKEND
+    explicit.BACKEND = backend
+    try:
+        img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
+        v = v_fn(img) if v_fn else (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
+        return img, r2splat_amd.rasterize_bwd(ctx, v), v, ctx
+    finally:
+        explicit.BACKEND = old
+
+
+@pytest.mark.parametrize("n,views,side,seed", [(1000, 4, 64, 0), (20_000, 8, 128, 3)])
+def test_fused_beta_matches_staged_ops_path(device, n, views, side, seed):
+    """The explicit API's Beta path through the fused pipeline agrees with the
+    staged operator path (each operator pinned to the oracle separately)."""
+    sc = workloads.scene(n, views, side, side, scale_range=(0.005, 0.08), seed=seed, device=device,
+                         beta_range=(0.5, 4.0))
+    a_img, a, _, ctx = _run_beta(sc, "fused")
+    b_img, b, _, _ = _run_beta(sc, "ops")
+    assert O.relative_error(a_img, b_img) <= 2e-6
+    for name in ("means", "quats", "scales", "densities", "beta"):
+        err = O.relative_error(getattr(a, name), getattr(b, name))
+        assert err <= 2e-5, f"{name}: {err:.2e}"
+    # bitwise reproducible backward
+    img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
+    a2 = r2splat_amd.rasterize_bwd(ctx, (2.0 * (img - 0.8 * img) / img.numel()).contiguous())
+    assert torch.equal(img, a_img)
+    for name in ("means", "quats", "scales", "densities", "beta"):
+        assert torch.equal(getattr(a2, name), getattr(a, name)), name
+
+
+def test_fused_beta_large_exponent_centre_pixel(device):
+    """One Beta primitive with a huge exponent seen at its centre: alpha = w
+    exactly (h = 1), d alpha / d beta from the line-integral factor only."""
+    gamma, ratio, dlog = 1e5, 0.002236059592288547, -4.999962500249999e-6
+    means = torch.tensor([[0., 0., 1.]], device=device)
+    quats = means.new_tensor([[1., 0., 0., 0.]])
+    scales = means.new_full((1, 3), 0.2)
+    view = torch.eye(4, device=device)[None]
+    K = means.new_tensor([[[1., 0., .5], [0., 1., .5], [0., 0., 1.]]])
+    beta = means.new_tensor([gamma])
+    img, f = fused.forward(means, quats, scales, means.new_tensor([1.]), view, K, 1, 1, 0.0, 1.0, beta=beta)
+    g = fused.backward(f, means, quats, scales, means.new_tensor([1.]), view, K, torch.ones_like(img))
+    expected = 0.2 * (2 * np.pi) ** 0.5 * ratio
+    torch.testing.assert_close(img.cpu(), torch.full((1, 1, 1), expected), atol=0, rtol=4e-6)
+    torch.testing.assert_close(g[3].cpu(), torch.tensor([expected]), atol=0, rtol=4e-6)
+    torch.testing.assert_close(g[4].cpu(), torch.tensor([expected * dlog]), atol=0, rtol=4e-6)

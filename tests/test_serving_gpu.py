@@ -1,7 +1,7 @@
 """Continuous-batching server (r2splat_amd.serving): jobs admitted as they
-arrive, stepped together, each job's result the same alone and in the mix
-(bitwise for the fused Gaussian path, within 1e-5 for the Beta explicit path,
-whose backward sums with atomics); admission respects the HBM budget."""
+arrive, stepped together, each job's result bitwise the same alone and in
+the mix (Gaussian and Beta jobs alike); the served steps agree with the
+eager explicit-API loop (torch Adam); admission respects the HBM budget."""
 
 import pytest
 import torch
@@ -22,13 +22,8 @@ def _requests():
 
 def _alone(req, device):
     job = build_job(req, device)
-    if req.beta:
-        fit = ExplicitFit(job, ADAM)
-        for _ in range(req.steps):
-            fit.step()
-        return [p.detach().clone() for p in fit.params]
     fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
-                      job["target"], adam=ADAM)
+                      job["target"], adam=ADAM, beta=job["beta"])
     fit.capture()
     fit.step(req.steps)
     fit.sync()
@@ -41,12 +36,19 @@ def test_each_result_is_the_same_alone_and_in_the_mix(device):
     assert sorted(f[0] for f in stats.finished) == [0, 1, 2, 3]
     assert max(f[3] - f[1] for f in stats.finished) > 0
     for req in _requests():
-        got, exp = stats.results[req.rid], _alone(req, device)
-        if req.beta:
-            for a, b in zip(got, exp):
-                assert O.relative_error(a, b) <= 1e-5
-        else:
-            assert torch.equal(got, exp)
+        assert torch.equal(stats.results[req.rid], _alone(req, device)), req.rid
+
+
+@pytest.mark.parametrize("beta", [False, True])
+def test_served_steps_agree_with_eager_explicit_loop(device, beta):
+    req = FitRequest(0, 0.0, 700, 3, 48, 109, beta, 4)
+    server = FitServer(device, n_streams=1, adam=ADAM)
+    got = server.run([req], clock="ticks", keep_results=True).results[0]
+    ref = ExplicitFit(build_job(req, device), ADAM)
+    for _ in range(req.steps):
+        ref.step()
+    exp = torch.cat([p.detach().reshape(-1) for p in ref.params])
+    assert O.relative_error(got, exp) <= 2e-6
 
 
 def test_admission_waits_for_memory(device):

@@ -124,3 +124,43 @@ def test_captured_autograd_wrapper_step_equals_eager(device):
         torch.cuda.synchronize()
         for a, b in zip(out, eager):
             assert torch.equal(a, b), f"replay {k}"
+
+
+def test_beta_fit_matches_explicit_path_and_torch_adam(device):
+    """Beta primitives: one captured step equals the explicit API bitwise
+    (image and all five gradients), and captured Adam steps follow
+    torch.optim.Adam on the explicit path's gradients."""
+    sc = workloads.scene(900, 3, 56, 64, scale_range=(0.02, 0.1), seed=13, device=device, beta_range=(0.5, 4.0))
+    tg = workloads.scene(900, 3, 56, 64, scale_range=(0.02, 0.1), seed=14, device=device, beta_range=(0.5, 4.0))
+    target, _ = r2splat_amd.rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size, beta=tg.beta)
+    fit = GaussianFit(*sc.params(), sc.viewmats, sc.Ks, target, beta=sc.beta)
+    fit.capture()
+    fit.step(1)
+    fit.sync()
+    img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
+    ref = r2splat_amd.rasterize_bwd(ctx, (2.0 / img.numel()) * (img - target))
+    assert torch.equal(fit.image, img)
+    for name, got in zip(("means", "quats", "scales", "densities", "beta"),
+                         (fit.v_means, fit.v_quats, fit.v_scales, fit.v_densities, fit.v_beta)):
+        assert torch.equal(got, getattr(ref, name)), name
+
+    cfg = AdamConfig(lr=1e-3, min_scale=1e-6)
+    fit = GaussianFit(*sc.params(), sc.viewmats, sc.Ks, target, beta=sc.beta, adam=cfg)
+    fit.capture()
+    fit.step(4)
+    assert fit.sync() == 4
+    params = [t.detach().clone() for t in sc.params() + (sc.beta,)]
+    opt = torch.optim.Adam(params, lr=cfg.lr, betas=(cfg.beta1, cfg.beta2), eps=cfg.eps, foreach=False)
+    for _ in range(4):
+        img, ctx = r2splat_amd.rasterize_fwd(*params[:4], sc.viewmats, sc.Ks, sc.image_size, beta=params[4])
+        g = r2splat_amd.rasterize_bwd(ctx, (2.0 / img.numel()) * (img - target))
+        for p, gp in zip(params, (g.means, g.quats, g.scales, g.densities, g.beta)):
+            p.grad = gp
+        opt.step()
+        with torch.no_grad():
+            params[2].clamp_(min=1e-6)
+            params[3].clamp_(min=0.0)
+            params[4].clamp_(min=cfg.min_beta)
+    for got, exp in zip((fit.means, fit.quats, fit.scales, fit.densities, fit.beta), params):
+        err = float((got - exp).abs().max() / exp.abs().max())
+        assert err <= 2e-6, err
