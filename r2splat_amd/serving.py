@@ -125,10 +125,9 @@ class ExplicitFit:
 @dataclass
 class _Live:
     req: FitRequest
-    fit: object
-    fused: bool
+    fit: GaussianFit
     admitted: float
-    done: int = 0
+    issued: int = 0
     nbytes: int = 0
 
 
@@ -139,6 +138,7 @@ class ServeStats:
     ticks: int = 0
     job_steps: int = 0
     peak_bytes: int = 0
+    peak_live: int = 0
     wall: float = 0.0
 
     def latencies(self) -> list:
@@ -154,7 +154,8 @@ class ServeStats:
 
 class FitServer:
     def __init__(self, device="cuda", *, hbm_budget: Optional[int] = None, n_streams: int = 4,
-                 adam: AdamConfig = AdamConfig(lr=1e-2, min_scale=1e-6), headroom: float = 1.15):
+                 adam: AdamConfig = AdamConfig(lr=1e-2, min_scale=1e-6), headroom: float = 1.15,
+                 sync_every: int = 8):
         self.device = torch.device(device)
         if hbm_budget is None:
             hbm_budget = int(torch.cuda.get_device_properties(self.device).total_memory * 0.85)
@@ -162,6 +163,7 @@ class FitServer:
         self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, n_streams))]
         self.adam = adam
         self.headroom = headroom
+        self.sync_every = max(1, int(sync_every))
         self.live: List[_Live] = []
         self._next_stream = 0
 
@@ -186,16 +188,24 @@ class FitServer:
         fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
                           job["target"], adam=self.adam, stream=stream, headroom=self.headroom, beta=job["beta"])
         fit.capture()
-        self.live.append(_Live(req, fit, True, now, 0, fit.memory_bytes()))
+        self.live.append(_Live(req, fit, now, 0, fit.memory_bytes()))
         return True
 
     def run(self, requests: List[FitRequest], clock=None, max_wall: Optional[float] = None,
-            keep_results: bool = False) -> ServeStats:
+            keep_results: bool = False, progress: Optional[Callable[[ServeStats, int, int], None]] = None,
+            progress_every: float = 30.0) -> ServeStats:
         """Serves `requests` (sorted by arrival) until all finished. `clock`
         returns the current time in the requests' unit (default: wall seconds
         since the call; "ticks": the number of ticks served so far);
         ``max_wall`` stops early (unfinished jobs dropped); ``keep_results``
-        keeps each job's final parameters in ``stats.results``."""
+        keeps each job's final parameters in ``stats.results``; ``progress``
+        (stats, live jobs, queued requests) is called every
+        ``progress_every`` wall seconds.
+
+        A tick issues one captured step of every live job without waiting;
+        the host synchronises every ``sync_every`` ticks (repairing capacity
+        overflows) and on the tick that issues some job's last step, which
+        is when that job is finished."""
         t0 = time.perf_counter()
         stats = ServeStats()
         tick_clock = clock == "ticks"
@@ -204,6 +214,7 @@ class FitServer:
         clock = clock or (lambda: time.perf_counter() - t0)
         pending = sorted(requests, key=lambda r: r.arrival)
         queue: List[FitRequest] = []
+        last_report = t0
         while pending or queue or self.live:
             now = clock()
             while pending and pending[0].arrival <= now:
@@ -211,6 +222,9 @@ class FitServer:
             for req in list(queue):  # arrival order; a later request that fits may pass one that does not
                 if self._admit(req, now):
                     queue.remove(req)
+            if progress is not None and time.perf_counter() - last_report >= progress_every:
+                last_report = time.perf_counter()
+                progress(stats, len(self.live), len(queue))
             if not self.live:
                 if pending and not queue:
                     if tick_clock:  # nothing to run: advance the tick clock to the next arrival
@@ -224,27 +238,26 @@ class FitServer:
                     raise RuntimeError(f"request {queue[0].rid} does not fit the HBM budget")
                 continue
             stats.peak_bytes = max(stats.peak_bytes, self.used_bytes())
+            stats.peak_live = max(stats.peak_live, len(self.live))
             for l in self.live:   # one iteration of every live job
-                if l.fused:
-                    l.fit.step(1)
-                else:
-                    l.fit.step()
-            for l in self.live:
-                if l.fused:
-                    l.fit.sync()
-            torch.cuda.synchronize(self.device)
+                l.fit.step(1)
+                l.issued += 1
             stats.ticks += 1
             stats.job_steps += len(self.live)
+            finishing = any(l.issued >= l.req.steps for l in self.live)
+            if not finishing and stats.ticks % self.sync_every:
+                continue
+            for l in self.live:
+                l.fit.sync()
+            torch.cuda.synchronize(self.device)
             now = clock()
             still = []
             for l in self.live:
-                l.done += 1
-                if l.done >= l.req.steps:
-                    loss = float(l.fit.loss.item()) if l.fit.loss is not None else float("nan")
+                if l.issued >= l.req.steps:
+                    loss = float(l.fit.loss.item())
                     stats.finished.append((l.req.rid, l.req.arrival, l.admitted, now, loss, l.req.beta))
                     if keep_results:
-                        stats.results[l.req.rid] = (l.fit.params.clone() if l.fused
-                                                    else [p.detach().clone() for p in l.fit.params])
+                        stats.results[l.req.rid] = l.fit.params.clone()
                 else:
                     still.append(l)
             self.live = still
