@@ -82,6 +82,33 @@ __device__ __forceinline__ uint32_t live(const uint32_t* __restrict__ d_n, uint3
     return n <= capacity ? n : 0u;
 }
 
+// Rows [lo, hi) of a T-row tile (pixel centres y_base + r + 1/2) that a record
+// can reach. For every dx, q = A dx^2 + B dx dy + C dy^2 >= dy^2 (4AC - B^2) /
+// (4A), so a row with dy^2 > 4 A S / (4AC - B^2) (S the record's bound: the
+// Gaussian's S or the Beta's D, stored as qlim - 1) holds no contributing pixel.
+// The extent is widened (1e-3 relative, 0.05 px) to cover the rounding of q;
+// forms that are not positive definite get the whole tile, records that
+// contribute nowhere (qlim = 0) none.
+template <typename R>
+__device__ __forceinline__ void row_span(const R& s, float y_base, int T, int& lo, int& hi) {
+    lo = 0;
+    hi = T;
+    if (s.qlim == 0u) {
+        hi = 0;
+        return;
+    }
+    const float S = __uint_as_float(s.qlim - 1u);
+    const float den = 4.f * s.A * s.C - s.B * s.B;
+    const float e2 = 4.f * s.A * S / den;
+    if (!(s.A > 0.f) || !(den > 0.f) || !(e2 < 1e30f)) return;
+    const float ext = __builtin_sqrtf(e2) * 1.001f + 0.05f;
+    const float c = s.my - y_base - 0.5f;  // the mean in row units of this tile
+    const float fl = fminf(fmaxf(ceilf(c - ext), 0.f), float(T));
+    const float fh = fminf(fmaxf(floorf(c + ext) + 1.f, 0.f), float(T));
+    lo = int(fl);
+    hi = max(lo, int(fh));
+}
+
 // ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
 // ---------------------------------------------------------------------------
@@ -251,50 +278,111 @@ fused_list_lengths_kernel(const uint32_t* __restrict__ tm, const uint32_t* __res
 }
 
 // One workgroup per camera-major list k = cam * nt + t (the lists of one XCD
-// are a contiguous band of cameras): packed[cm[k] + r] = record of the r-th
-// intersection of the list, with g = its emission index w0 + (ty - y0)
-// (x1 - x0) + (tx - x0) (row-major within the record's tile rectangle, the
-// order fused_emit_kernel wrote it in; w0 is the record's g). The packed list
-// is camera-major, tile, then primitive: the reference's order.
+// are a contiguous band of cameras): the records of the list's intersections
+// with g = their emission index w0 + (ty - y0) (x1 - x0) + (tx - x0)
+// (row-major within the record's tile rectangle, the order fused_emit_kernel
+// wrote it in; w0 is the record's g). The packed lists are camera-major, the
+// reference's list order. Within each chunk of kPackChunk intersections the
+// records are ordered by the number of tile rows they reach (descending; ties
+// keep the primitive order), a stable counting sort in LDS: the backward's
+// waves then take 64 records of similar height and loop over no more rows
+// than their tallest record needs.
 constexpr int kPackItems = 4;
+constexpr int kPackThreads = 256;
+constexpr int kPackChunk = kPackItems * kPackThreads;
+constexpr int kPackWaves = kPackThreads / kWave;
+constexpr int kPackClasses = 34;  // rows 0..32 plus "not written"
+constexpr int kPackSlots = kPackItems * kPackWaves;
 
 template <typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kPackThreads)
 fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t C,
-                  uint32_t nt, uint32_t ntx, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tm,
-                  const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n, uint32_t capacity,
-                  R* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
+                  uint32_t nt, uint32_t ntx, uint32_t T, const uint32_t* __restrict__ vals,
+                  const uint32_t* __restrict__ tm, const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n,
+                  uint32_t capacity, R* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
+    // cnt[class][item][wave] -> exclusive offsets in that order (class-major:
+    // taller records first; within a class the chunk's own order)
+    __shared__ uint32_t cnt[kPackClasses * kPackSlots];
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x);
-    if (k >= C * nt) return;
+    if (k >= C * nt) return;  // uniform over the workgroup
     const uint32_t n_live = live(d_n, capacity);
     const uint32_t cam = k / nt, t = k - cam * nt;
     const uint32_t dst = cm[k], len = cm[k + 1] - dst, src = tm[t * C + cam];
     if (threadIdx.x == 0) tile_offsets[k] = int32_t(dst);
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
-    for (uint32_t r0 = 0; r0 < len; r0 += blockDim.x * kPackItems) {
-        uint32_t v[kPackItems];
+    const float y_base = float(ty * T);
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    const uint32_t skip_cls = T + 1u;
+    for (uint32_t r0 = 0; r0 < len; r0 += kPackChunk) {
+        uint32_t v[kPackItems], cls[kPackItems], rank[kPackItems];
 #pragma unroll
         for (int q = 0; q < kPackItems; ++q) {
-            const uint32_t r = r0 + q * blockDim.x + threadIdx.x;
+            const uint32_t r = r0 + q * kPackThreads + threadIdx.x;
             v[q] = (r < len && src + r < n_live) ? vals[src + r] : n_recs;
         }
         R rec[kPackItems];
         uint2 rc[kPackItems];
 #pragma unroll
         for (int q = 0; q < kPackItems; ++q) {
+            cls[q] = skip_cls;
             if (v[q] < n_recs) {
                 rec[q] = recs[v[q]];
                 rc[q] = rects[v[q]];
+                int lo, hi;
+                row_span(rec[q], y_base, int(T), lo, hi);
+                cls[q] = T - uint32_t(hi - lo);
             }
         }
+        for (uint32_t i = threadIdx.x; i < uint32_t(kPackClasses * kPackSlots); i += kPackThreads) cnt[i] = 0u;
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < kPackItems; ++q) {
-            const uint32_t r = r0 + q * blockDim.x + threadIdx.x;
-            if (v[q] >= n_recs || dst + r >= capacity) continue;
+            // stable rank among this wave's items of the same class: one ballot per distinct class
+            uint64_t pending = __ballot(1);
+            while (pending) {
+                const uint32_t c = __shfl(cls[q], __builtin_ctzll(pending), kWave);
+                const uint64_t m = __ballot(cls[q] == c);
+                if (cls[q] == c) rank[q] = uint32_t(__popcll(m & below));
+                if (lane == 0) cnt[c * kPackSlots + q * kPackWaves + wave] = uint32_t(__popcll(m));
+                pending &= ~m;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {  // exclusive scan of the counts, 9 per lane
+            constexpr int kPer = (kPackClasses * kPackSlots + kWave - 1) / kWave;
+            uint32_t loc[kPer], sum = 0;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = lane * kPer + j;
+                loc[j] = i < uint32_t(kPackClasses * kPackSlots) ? cnt[i] : 0u;
+                sum += loc[j];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t y = __shfl_up(incl, off, kWave);
+                if (lane >= uint32_t(off)) incl += y;
+            }
+            uint32_t run = incl - sum;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = lane * kPer + j;
+                if (i < uint32_t(kPackClasses * kPackSlots)) cnt[i] = run;
+                run += loc[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPackItems; ++q) {
+            if (cls[q] == skip_cls) continue;
+            const uint32_t pos = r0 + cnt[cls[q] * kPackSlots + q * kPackWaves + wave] + rank[q];
+            if (dst + pos >= capacity) continue;
             const uint32_t x0 = rc[q].x & 0xffffu, x1 = rc[q].x >> 16, y0 = rc[q].y & 0xffffu;
             rec[q].g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
-            packed[dst + r] = rec[q];
+            packed[dst + pos] = rec[q];
         }
+        __syncthreads();  // cnt is reused by the next chunk
     }
 }
 
@@ -410,8 +498,9 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
 // ---------------------------------------------------------------------------
 // Backward: one workgroup of WAVES waves per T x T tile; wave k takes chunks
 // k, k + WAVES, ... of 64 records, one lane per record. The tile's adjoint
-// image sits in LDS and is read with wave-uniform (broadcast) loads, row by
-// row, so each lane sums its own primitive's moments over the T^2 pixels
+// image sits in LDS and is read row by row; each lane walks only the rows its
+// record can reach (row_span; the pack step groups records of similar height
+// into the same chunks) and sums its own primitive's moments over them
 // without any cross-lane reduction:
 //   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
 // with dx = px - mx (the moments are summed directly in dx: expanding them
@@ -460,7 +549,11 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f;
         float bm1 = 0.f;
         if constexpr (BETA) bm1 = s.b2d - 1.f;
-        for (uint32_t r = 0; r < uint32_t(T); ++r) {
+        // only the rows this lane's record can reach (the others add exact zeros);
+        // the wave runs as many rows as its tallest record needs
+        int r_lo = 0, r_hi = 0;
+        if (active) row_span(s, y0 - 0.5f, T, r_lo, r_hi);
+        for (int r = r_lo; r < r_hi; ++r) {
             const float dy = y0 + float(r) - s.my;
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
             const float4* row = reinterpret_cast<const float4*>(sv + r * T);
@@ -794,9 +887,9 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
                                 stream))
         return rc;
     const uint32_t pack_blocks = (n_tiles + 7u) & ~7u;  // a multiple of the 8 XCDs
-    fused_pack_kernel<R><<<pack_blocks, 256, 0, stream>>>(at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C,
-                                                          ntx * nty, ntx, vals, tm, cm, counters, capacity,
-                                                          static_cast<R*>(packed), tile_offsets);
+    fused_pack_kernel<R><<<pack_blocks, kPackThreads, 0, stream>>>(
+        at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C, ntx * nty, ntx, tile_size, vals, tm, cm, counters,
+        capacity, static_cast<R*>(packed), tile_offsets);
     if (int rc = check_launch("fused_pack")) return rc;
     if (tile_order) {
         void* order_ws = at<void>(scratch, L.order_sort);
