@@ -99,9 +99,10 @@ __device__ __forceinline__ void row_span(const R& s, float y_base, int T, int& l
     }
     const float S = __uint_as_float(s.qlim - 1u);
     const float den = 4.f * s.A * s.C - s.B * s.B;
-    const float e2 = 4.f * s.A * S / den;
+    // hardware reciprocal and square root (about 1 ulp): covered by the margin
+    const float e2 = 4.f * s.A * S * __builtin_amdgcn_rcpf(den);
     if (!(s.A > 0.f) || !(den > 0.f) || !(e2 < 1e30f)) return;
-    const float ext = __builtin_sqrtf(e2) * 1.001f + 0.05f;
+    const float ext = __builtin_amdgcn_sqrtf(e2) * 1.001f + 0.05f;
     const float c = s.my - y_base - 0.5f;  // the mean in row units of this tile
     const float fl = fminf(fmaxf(ceilf(c - ext), 0.f), float(T));
     const float fh = fminf(fmaxf(floorf(c + ext) + 1.f, 0.f), float(T));
@@ -398,12 +399,20 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
 // partial images are summed in LDS in wave order, so the result does not
 // depend on scheduling.
 // ---------------------------------------------------------------------------
+// Adds one record to a lane's PX pixels. Bit k of the wave-uniform `rows` is
+// set when row group k can hold contributing pixels; the others are skipped
+// by scalar branches (one bit test each; none when every group is set).
 template <int PX, typename R>
-__device__ __forceinline__ void splat_accumulate(const R& s, float px, const float (&py)[PX], float (&acc)[PX]) {
+__device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, float px, const float (&py)[PX],
+                                                 float (&acc)[PX]) {
+    constexpr uint32_t kAll = PX >= 32 ? 0xffffffffu : (1u << PX) - 1u;
+    if (rows == 0u) return;
+    const bool all = rows == kAll;
     const float dx = px - s.mx;
     const float adx2 = s.A * dx * dx, bdx = s.B * dx;
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
+        if (!all && !((rows >> k) & 1u)) continue;
         const float dy = py[k] - s.my;
         const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);  // Gaussian: sigma log2 e; Beta: d^2
         float e;
@@ -468,7 +477,15 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
     }
     R* slot = stage[wave];
     R next = load_splat(packed, lo + int32_t(lane), hi);
+    const float y_base = float(ty * T);
     for (int32_t base = lo; base < hi; base += kWave) {
+        // row groups of the record this lane stages (lane rows are row0 + STEP k,
+        // row0 < STEP); record j's span is read from lane j's register
+        int r_lo, r_hi;
+        row_span(next, y_base, T, r_lo, r_hi);
+        const int k_lo = r_lo / STEP, k_hi = (r_hi + STEP - 1) / STEP;  // groups [k_lo, k_hi)
+        const uint32_t spans = r_lo < r_hi ? ((k_hi >= 32 ? 0xffffffffu : (1u << k_hi) - 1u) & ~((1u << k_lo) - 1u))
+                                           : 0u;
         wave_lds_sync();  // every lane is done reading the previous batch
         slot[lane] = next;
         wave_lds_sync();
@@ -478,10 +495,10 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         R cur = slot[0];
         for (int32_t j = 1; j < n; ++j) {
             const R nxt = slot[j];
-            splat_accumulate<PX>(cur, px, py, acc);
+            splat_accumulate<PX>(cur, __builtin_amdgcn_readlane(spans, j - 1), px, py, acc);
             cur = nxt;
         }
-        splat_accumulate<PX>(cur, px, py, acc);
+        splat_accumulate<PX>(cur, __builtin_amdgcn_readlane(spans, n - 1), px, py, acc);
     }
 #pragma unroll
     for (int k = 0; k < PX; ++k) part[wave][(row0 + STEP * k) * T + col] = acc[k];
