@@ -288,6 +288,19 @@ fused_list_lengths_kernel(const uint32_t* __restrict__ tm, const uint32_t* __res
 // keep the primitive order), a stable counting sort in LDS: the backward's
 // waves then take 64 records of similar height and loop over no more rows
 // than their tallest record needs.
+// Streaming (non-temporal) store of one record: the packed list is written
+// once and read by the next kernel, so it should not evict the projected
+// records the pack kernel is gathering from L2.
+template <typename R>
+__device__ __forceinline__ void store_stream(R* dst, const R& v) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(R) % sizeof(f4) == 0, "record size must be a multiple of 16 bytes");
+    const f4* src = reinterpret_cast<const f4*>(&v);
+    f4* out = reinterpret_cast<f4*>(dst);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(R) / sizeof(f4); ++i) __builtin_nontemporal_store(src[i], out + i);
+}
+
 constexpr int kPackItems = 4;
 constexpr int kPackThreads = 256;
 constexpr int kPackChunk = kPackItems * kPackThreads;
@@ -381,7 +394,7 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
             if (dst + pos >= capacity) continue;
             const uint32_t x0 = rc[q].x & 0xffffu, x1 = rc[q].x >> 16, y0 = rc[q].y & 0xffffu;
             rec[q].g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
-            packed[dst + pos] = rec[q];
+            store_stream(packed + dst + pos, rec[q]);
         }
         __syncthreads();  // cnt is reused by the next chunk
     }
