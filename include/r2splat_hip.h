@@ -430,6 +430,19 @@ int r2s_volume_tv_bwd(int dtype, const void* volume, const void* grad_output, ui
                       uint32_t D, uint32_t H, uint32_t W, double lambda_tv,
                       void* grad_volume, r2s_stream_t stream);
 
+/* Isotropic TV over volume [B,D,H,W]: tv_map[v] = sqrt(dz^2 + dy^2 + dx^2 + eps)
+ * with forward differences (zero past the last index of an axis) and
+ * tv_sum[0] = sum of tv_map (overwritten); the loss is lambda * tv_sum / (B D H W).
+ * No counterpart in the reference (whose TV is anisotropic, Loss.cpp:115-153);
+ * the isotropic option of the CT objective. eps > 0. */
+int r2s_volume_tv_iso_fwd(int dtype, const void* volume, uint32_t B, uint32_t D, uint32_t H,
+                          uint32_t W, double eps, void* tv_map, void* tv_sum, r2s_stream_t stream);
+
+/* grad_volume = lambda * grad_output[0] / (B D H W) * d(sum of tv_map)/d volume. */
+int r2s_volume_tv_iso_bwd(int dtype, const void* volume, const void* grad_output, uint32_t B,
+                          uint32_t D, uint32_t H, uint32_t W, double eps, double lambda_tv,
+                          void* grad_volume, r2s_stream_t stream);
+
 /* Mean-L1 image loss map |p - t| and its sum (overwritten); gradient
  * lambda * g[0] * sign(p - t) / n. Replaces image_loss_l1_fwd/bwd
  * (Ops.h:14-34, r2splat/cuda/csrc/ImageLossFwd.cu:20-31, ImageLossBwd.cu:14-30). */

@@ -387,6 +387,20 @@ def volume_tv(volume: Tensor) -> Tensor:
     return sum(d.abs().sum() for d in diffs) / edges
 
 
+def volume_tv_iso(volume: Tensor, eps: float = 1e-8) -> Tensor:
+    """Isotropic TV (this build's option beside the reference's anisotropic
+    loss, which has none): mean over voxels of sqrt(dz^2 + dy^2 + dx^2 + eps),
+    forward differences, zero past the last index of each axis."""
+    v = volume
+    dz = torch.zeros_like(v)
+    dy = torch.zeros_like(v)
+    dx = torch.zeros_like(v)
+    dz[:, :-1] = v[:, 1:] - v[:, :-1]
+    dy[:, :, :-1] = v[:, :, 1:] - v[:, :, :-1]
+    dx[:, :, :, :-1] = v[:, :, :, 1:] - v[:, :, :, :-1]
+    return torch.sqrt(dz * dz + dy * dy + dx * dx + eps).sum() / max(v.numel(), 1)
+
+
 def relative_error(actual: Tensor, expected: Tensor) -> float:
     """max |a - b| / max |b| (0 when both vanish)."""
     a = actual.detach().double().cpu()

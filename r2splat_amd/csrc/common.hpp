@@ -19,6 +19,12 @@ namespace r2s {
 
 constexpr int kWave = 64;
 
+// A dispatch's grid size counts work-items in 32 bits, so blocks * threads
+// must stay below 2^32. Kernels with one wave per element (intersections can
+// number in the hundreds of millions) loop over a grid capped at this many
+// blocks instead of launching one block per few elements.
+constexpr uint32_t kIsectBlocks = 1u << 16;
+
 // ---------------------------------------------------------------------------
 // Error reporting: thread-local message + status code.
 // ---------------------------------------------------------------------------

@@ -1,4 +1,5 @@
-// Volume TV loss (anisotropic, edge-mean normalised) and mean-L1 image loss.
+// Volume TV loss (anisotropic, edge-mean normalised; isotropic, voxel-mean
+// normalised) and mean-L1 image loss.
 // Semantics: r2splat/cuda/csrc/VolumeLossFwd.cu:39-153, VolumeLossBwd.cu:46-115,
 // Loss.cpp:115-188, ImageLossFwd.cu:20-31, ImageLossBwd.cu:14-30.
 #include "common.hpp"
@@ -63,6 +64,63 @@ tv_bwd_kernel(const S* __restrict__ vol, const S* __restrict__ grad_out, uint32_
     grad[idx] = g * (grad_out[0] * scale);
 }
 
+// Isotropic TV: t(v) = sqrt(dz^2 + dy^2 + dx^2 + eps) with forward differences
+// (zero past the last index of an axis); tv_map = t, tv_sum = sum of t.
+template <typename S>
+__device__ __forceinline__ S iso_term(const S* __restrict__ vol, size_t idx, uint32_t d, uint32_t h, uint32_t w,
+                                      uint32_t D, uint32_t H, uint32_t W, S eps, S& dz, S& dy, S& dx) {
+    const S v = vol[idx];
+    dz = d + 1 < D ? vol[idx + size_t(H) * W] - v : S(0);
+    dy = h + 1 < H ? vol[idx + W] - v : S(0);
+    dx = w + 1 < W ? vol[idx + 1] - v : S(0);
+    return sqrt(dz * dz + dy * dy + dx * dx + eps);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kLossThreads)
+tv_iso_fwd_kernel(const S* __restrict__ vol, uint32_t B, uint32_t D, uint32_t H, uint32_t W, S eps,
+                  S* __restrict__ tv_map, S* __restrict__ tv_sum) {
+    const size_t total = size_t(B) * D * H * W;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    S t = S(0);
+    if (idx < total) {
+        const uint32_t w = uint32_t(idx % W), h = uint32_t((idx / W) % H), d = uint32_t((idx / (size_t(W) * H)) % D);
+        S dz, dy, dx;
+        t = iso_term(vol, idx, d, h, w, D, H, W, eps, dz, dy, dx);
+        tv_map[idx] = t;
+    }
+    block_sum_atomic(t, tv_sum);
+}
+
+// d/dV(v) of sum_u t(u): -(dz + dy + dx)(v) / t(v) plus dz(v - ez) / t(v - ez),
+// dy(v - ey) / t(v - ey) and dx(v - ex) / t(v - ex) where those voxels exist.
+template <typename S>
+__global__ void __launch_bounds__(kLossThreads)
+tv_iso_bwd_kernel(const S* __restrict__ vol, const S* __restrict__ grad_out, uint32_t B, uint32_t D, uint32_t H,
+                  uint32_t W, S eps, S scale, S* __restrict__ grad) {
+    const size_t total = size_t(B) * D * H * W;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint32_t w = uint32_t(idx % W), h = uint32_t((idx / W) % H), d = uint32_t((idx / (size_t(W) * H)) % D);
+    const size_t sd = size_t(H) * W, sh = W;
+    S dz, dy, dx;
+    S t = iso_term(vol, idx, d, h, w, D, H, W, eps, dz, dy, dx);
+    S g = -(dz + dy + dx) / t;
+    if (d > 0) {
+        t = iso_term(vol, idx - sd, d - 1, h, w, D, H, W, eps, dz, dy, dx);
+        g += dz / t;
+    }
+    if (h > 0) {
+        t = iso_term(vol, idx - sh, d, h - 1, w, D, H, W, eps, dz, dy, dx);
+        g += dy / t;
+    }
+    if (w > 0) {
+        t = iso_term(vol, idx - 1, d, h, w - 1, D, H, W, eps, dz, dy, dx);
+        g += dx / t;
+    }
+    grad[idx] = g * (grad_out[0] * scale);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(kLossThreads)
 l1_fwd_kernel(const S* __restrict__ p, const S* __restrict__ t, uint64_t n, S* __restrict__ map,
@@ -115,6 +173,35 @@ extern "C" int r2s_volume_tv_bwd(int dtype, const void* volume, const void* grad
         tv_bwd_kernel<S><<<ceil_div(total, kLossThreads), kLossThreads, 0, stream>>>(
             (const S*)volume, (const S*)grad_output, B, D, H, W, S(lambda_tv / edges), (S*)grad_volume);
         return check_launch("volume_tv_bwd");
+    });
+}
+
+extern "C" int r2s_volume_tv_iso_fwd(int dtype, const void* volume, uint32_t B, uint32_t D, uint32_t H,
+                                     uint32_t W, double eps, void* tv_map, void* tv_sum, r2s_stream_t stream) {
+    R2S_REQUIRE(eps > 0.0, "isotropic TV needs eps > 0 (got %g)", eps);
+    const size_t total = size_t(B) * D * H * W;
+    return dispatch_dtype(dtype, [&](auto tag) {
+        using S = decltype(tag);
+        if (int rc = fill_zero<S>((S*)tv_sum, 1, stream)) return rc;
+        if (total == 0) return R2S_OK;
+        tv_iso_fwd_kernel<S><<<ceil_div(total, kLossThreads), kLossThreads, 0, stream>>>(
+            (const S*)volume, B, D, H, W, S(eps), (S*)tv_map, (S*)tv_sum);
+        return check_launch("volume_tv_iso_fwd");
+    });
+}
+
+extern "C" int r2s_volume_tv_iso_bwd(int dtype, const void* volume, const void* grad_output, uint32_t B,
+                                     uint32_t D, uint32_t H, uint32_t W, double eps, double lambda_tv,
+                                     void* grad_volume, r2s_stream_t stream) {
+    R2S_REQUIRE(eps > 0.0, "isotropic TV needs eps > 0 (got %g)", eps);
+    const size_t total = size_t(B) * D * H * W;
+    if (total == 0) return R2S_OK;
+    return dispatch_dtype(dtype, [&](auto tag) {
+        using S = decltype(tag);
+        tv_iso_bwd_kernel<S><<<ceil_div(total, kLossThreads), kLossThreads, 0, stream>>>(
+            (const S*)volume, (const S*)grad_output, B, D, H, W, S(eps), S(lambda_tv / double(total)),
+            (S*)grad_volume);
+        return check_launch("volume_tv_iso_bwd");
     });
 }
 

@@ -180,52 +180,56 @@ raster_bwd_isect_kernel(RasterArgs<S> A, const int64_t* __restrict__ isect_ids,
                         const S* __restrict__ unc, S* __restrict__ v_abs, S* __restrict__ v_xy,
                         S* __restrict__ v_conic, S* __restrict__ v_opac, S* __restrict__ v_mu,
                         S* __restrict__ v_beta, S* __restrict__ v_q) {
-    const uint32_t e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (e >= A.n_isects) return;
-    const uint32_t nt = A.ntx * A.nty;
-    const int64_t key = isect_ids[e];
-    const uint32_t cam = uint32_t(key / nt);
-    if (cam >= A.C) return;
-    const uint32_t tile = uint32_t(key - int64_t(cam) * nt);
-    const uint32_t ty = tile / A.ntx, tx = tile - (tile / A.ntx) * A.ntx;
-    const int32_t g = A.flatten[e];
-    if (g < 0 || uint32_t(g) >= A.C * A.N) return;
-    Prim2D<S> p;
-    p.mx = A.means2d[2 * size_t(g)];
-    p.my = A.means2d[2 * size_t(g) + 1];
-    p.a = A.conics[3 * size_t(g)];
-    p.b = A.conics[3 * size_t(g) + 1];
-    p.c = A.conics[3 * size_t(g) + 2];
-    p.opac = A.opac[g];
-    p.mu = A.mus[g];
-    p.beta = BETA ? A.betas[g] : S(0);
-    const uint32_t n = uint32_t(g) % A.N;
-    const S q = unc ? unc[n] : S(0);
-    const size_t img = size_t(cam) * A.H * A.W;
-    Grad2D<S> gr;
-    S vq = S(0);
-    const uint32_t npix = A.T * A.T;
-    for (uint32_t k = lane_id(); k < npix; k += kWave) {
-        const uint32_t ly = k / A.T, lx = k - ly * A.T;
-        const uint32_t i = ty * A.T + ly, j = tx * A.T + lx;
-        if (i >= A.H || j >= A.W) continue;
-        const S px = S(j) + S(0.5), py = S(i) + S(0.5);
-        S vis, alpha;
-        if (!eval_profile<S, BETA>(p, px, py, vis, alpha)) continue;
-        const size_t o = img + size_t(i) * A.W + j;
-        S v = v_render[o];
-        if (v_var) {
-            const S vv = v_var[o];
-            vq += vv * alpha * alpha;
-            if (unc) v += S(2) * q * alpha * vv;
+    // grid-stride over intersections (one wave each): a launch covers at most
+    // kIsectBlocks blocks, whatever the intersection count
+    const uint32_t waves = blockDim.x / kWave;
+    const uint64_t stride = uint64_t(gridDim.x) * waves;
+    for (uint64_t e = uint64_t(blockIdx.x) * waves + threadIdx.x / kWave; e < A.n_isects; e += stride) {
+        const uint32_t nt = A.ntx * A.nty;
+        const int64_t key = isect_ids[e];
+        const uint32_t cam = uint32_t(key / nt);
+        if (cam >= A.C) continue;
+        const uint32_t tile = uint32_t(key - int64_t(cam) * nt);
+        const uint32_t ty = tile / A.ntx, tx = tile - (tile / A.ntx) * A.ntx;
+        const int32_t g = A.flatten[e];
+        if (g < 0 || uint32_t(g) >= A.C * A.N) continue;
+        Prim2D<S> p;
+        p.mx = A.means2d[2 * size_t(g)];
+        p.my = A.means2d[2 * size_t(g) + 1];
+        p.a = A.conics[3 * size_t(g)];
+        p.b = A.conics[3 * size_t(g) + 1];
+        p.c = A.conics[3 * size_t(g) + 2];
+        p.opac = A.opac[g];
+        p.mu = A.mus[g];
+        p.beta = BETA ? A.betas[g] : S(0);
+        const uint32_t n = uint32_t(g) % A.N;
+        const S q = unc ? unc[n] : S(0);
+        const size_t img = size_t(cam) * A.H * A.W;
+        Grad2D<S> gr;
+        S vq = S(0);
+        const uint32_t npix = A.T * A.T;
+        for (uint32_t k = lane_id(); k < npix; k += kWave) {
+            const uint32_t ly = k / A.T, lx = k - ly * A.T;
+            const uint32_t i = ty * A.T + ly, j = tx * A.T + lx;
+            if (i >= A.H || j >= A.W) continue;
+            const S px = S(j) + S(0.5), py = S(i) + S(0.5);
+            S vis, alpha;
+            if (!eval_profile<S, BETA>(p, px, py, vis, alpha)) continue;
+            const size_t o = img + size_t(i) * A.W + j;
+            S v = v_render[o];
+            if (v_var) {
+                const S vv = v_var[o];
+                vq += vv * alpha * alpha;
+                if (unc) v += S(2) * q * alpha * vv;
+            }
+            pixel_vjp<S, BETA>(p, px, py, vis, alpha, v, gr);
         }
-        pixel_vjp<S, BETA>(p, px, py, vis, alpha, v, gr);
-    }
-    gr.reduce_wave(v_abs != nullptr);
-    if (v_q) vq = wave_sum(vq);
-    if (lane_id() == 0) {
-        gr.commit(g, v_abs, v_xy, v_conic, v_opac, v_mu, v_beta);
-        if (v_q) atomic_add(v_q + n, vq);
+        gr.reduce_wave(v_abs != nullptr);
+        if (v_q) vq = wave_sum(vq);
+        if (lane_id() == 0) {
+            gr.commit(g, v_abs, v_xy, v_conic, v_opac, v_mu, v_beta);
+            if (v_q) atomic_add(v_q + n, vq);
+        }
     }
 }
 
@@ -402,7 +406,7 @@ extern "C" int r2s_rasterize_to_pixels_bwd_per_gaussian(
                               n_isects, C, N, width, height, tile_size, n_tiles_x, n_tiles_y);
         constexpr uint32_t waves = 4;
         auto k = beta ? raster_bwd_isect_kernel<S, true> : raster_bwd_isect_kernel<S, false>;
-        k<<<ceil_div(n_isects, waves), waves * kWave, 0, stream>>>(
+        k<<<std::min<uint32_t>(ceil_div(n_isects, waves), kIsectBlocks), waves * kWave, 0, stream>>>(
             A, isect_ids, (const S*)v_render, (const S*)v_render_var,
             v_render_var ? (const S*)uncertainties : nullptr, (S*)v_means2d_abs, (S*)v_means2d,
             (S*)v_conics, (S*)v_opacities, (S*)v_mus, (S*)v_betas,

@@ -240,49 +240,52 @@ vox_bwd_isect_kernel(VoxArgs<S> A, const int64_t* __restrict__ isect_ids, const 
                      const S* __restrict__ v_var, const S* __restrict__ unc, S* __restrict__ v_xyz,
                      S* __restrict__ v_c, S* __restrict__ v_opac, S* __restrict__ v_beta,
                      S* __restrict__ v_q) {
-    const uint32_t e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (e >= A.n_isects) return;
-    const uint32_t nt = A.ntx * A.nty * A.ntz;
-    const int64_t key = isect_ids[e];
-    if (key < 0 || key >= int64_t(nt)) return;
-    const uint32_t tile = uint32_t(key);
-    const uint32_t tx = tile % A.ntx, ty = (tile / A.ntx) % A.nty, tz = tile / (A.ntx * A.nty);
-    const int32_t g = A.flatten[e];
-    if (g < 0 || uint32_t(g) >= A.N) return;
-    const size_t G = size_t(g);
-    Prim3D<S> p;
-    p.x = A.means3d[3 * G];
-    p.y = A.means3d[3 * G + 1];
-    p.z = A.means3d[3 * G + 2];
+    // grid-stride over intersections (one wave each), at most kIsectBlocks blocks
+    const uint32_t waves = blockDim.x / kWave;
+    const uint64_t stride = uint64_t(gridDim.x) * waves;
+    for (uint64_t e = uint64_t(blockIdx.x) * waves + threadIdx.x / kWave; e < A.n_isects; e += stride) {
+        const uint32_t nt = A.ntx * A.nty * A.ntz;
+        const int64_t key = isect_ids[e];
+        if (key < 0 || key >= int64_t(nt)) continue;
+        const uint32_t tile = uint32_t(key);
+        const uint32_t tx = tile % A.ntx, ty = (tile / A.ntx) % A.nty, tz = tile / (A.ntx * A.nty);
+        const int32_t g = A.flatten[e];
+        if (g < 0 || uint32_t(g) >= A.N) continue;
+        const size_t G = size_t(g);
+        Prim3D<S> p;
+        p.x = A.means3d[3 * G];
+        p.y = A.means3d[3 * G + 1];
+        p.z = A.means3d[3 * G + 2];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) p.c[k] = A.conics[6 * G + k];
-    p.opac = A.opac[g];
-    p.beta = BETA ? A.betas[g] : S(0);
-    const S q = unc ? unc[g] : S(0);
-    Grad3D<S> gr;
-    S vq = S(0);
-    const uint32_t T = A.T, nvox = T * T * T;
-    for (uint32_t t = lane_id(); t < nvox; t += kWave) {
-        const uint32_t lx = t % T, ly = (t / T) % T, lz = t / (T * T);
-        const uint32_t k = tx * T + lx, j = ty * T + ly, i = tz * T + lz;
-        if (i >= A.dz || j >= A.dy || k >= A.dx) continue;
-        const S px = S(k) + S(0.5), py = S(j) + S(0.5), pz = S(i) + S(0.5);
-        S vis, alpha;
-        if (!eval_profile3<S, BETA>(p, px, py, pz, vis, alpha)) continue;
-        const size_t o = (size_t(i) * A.dy + j) * A.dx + k;
-        S v = v_render[o];
-        if (v_var) {
-            const S vv = v_var[o];
-            vq += vv * alpha * alpha;
-            if (unc) v += S(2) * q * alpha * vv;
+        for (int k = 0; k < 6; ++k) p.c[k] = A.conics[6 * G + k];
+        p.opac = A.opac[g];
+        p.beta = BETA ? A.betas[g] : S(0);
+        const S q = unc ? unc[g] : S(0);
+        Grad3D<S> gr;
+        S vq = S(0);
+        const uint32_t T = A.T, nvox = T * T * T;
+        for (uint32_t t = lane_id(); t < nvox; t += kWave) {
+            const uint32_t lx = t % T, ly = (t / T) % T, lz = t / (T * T);
+            const uint32_t k = tx * T + lx, j = ty * T + ly, i = tz * T + lz;
+            if (i >= A.dz || j >= A.dy || k >= A.dx) continue;
+            const S px = S(k) + S(0.5), py = S(j) + S(0.5), pz = S(i) + S(0.5);
+            S vis, alpha;
+            if (!eval_profile3<S, BETA>(p, px, py, pz, vis, alpha)) continue;
+            const size_t o = (size_t(i) * A.dy + j) * A.dx + k;
+            S v = v_render[o];
+            if (v_var) {
+                const S vv = v_var[o];
+                vq += vv * alpha * alpha;
+                if (unc) v += S(2) * q * alpha * vv;
+            }
+            voxel_vjp<S, BETA>(p, px, py, pz, vis, alpha, v, gr);
         }
-        voxel_vjp<S, BETA>(p, px, py, pz, vis, alpha, v, gr);
-    }
-    gr.reduce_wave();
-    if (v_q) vq = wave_sum(vq);
-    if (lane_id() == 0) {
-        gr.commit(g, v_xyz, v_c, v_opac, BETA ? v_beta : nullptr);
-        if (v_q) atomic_add(v_q + G, vq);
+        gr.reduce_wave();
+        if (v_q) vq = wave_sum(vq);
+        if (lane_id() == 0) {
+            gr.commit(g, v_xyz, v_c, v_opac, BETA ? v_beta : nullptr);
+            if (v_q) atomic_add(v_q + G, vq);
+        }
     }
 }
 
@@ -397,7 +400,7 @@ extern "C" int r2s_voxelize_to_voxels_bwd_per_gaussian(
                                   n_tiles_y, n_tiles_z);
         constexpr uint32_t waves = 4;
         auto k = beta ? vox_bwd_isect_kernel<S, true> : vox_bwd_isect_kernel<S, false>;
-        k<<<ceil_div(n_isects, waves), waves * kWave, 0, stream>>>(
+        k<<<std::min<uint32_t>(ceil_div(n_isects, waves), kIsectBlocks), waves * kWave, 0, stream>>>(
             A, isect_ids, (const S*)v_render, (const S*)v_render_var,
             v_render_var ? (const S*)uncertainties : nullptr, (S*)v_means3d, (S*)v_conics,
             (S*)v_opacities, (S*)v_betas, v_render_var ? (S*)v_uncertainties : nullptr);

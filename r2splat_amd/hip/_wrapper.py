@@ -711,6 +711,38 @@ def volume_loss_bwd(volume, grad_output, lambda_tv):
     return grad
 
 
+def volume_loss_iso_fwd(volume, lambda_tv, eps=1e-8):
+    """Isotropic TV over (B,D,H,W): tv = mean over voxels of sqrt(dz^2 + dy^2 +
+    dx^2 + eps) (forward differences, zero past the last index) -> (lambda * tv,
+    tv, tv_map). An option beside the reference's anisotropic volume_loss_fwd."""
+    _check("volume", volume, 4)
+    dt = _floating(volume)
+    _require(float(eps) > 0.0, "eps must be positive")
+    B, Dd, H, W = volume.shape
+    tv_map = torch.empty_like(volume)
+    tv_sum = volume.new_empty((1,))
+    with _Launch(volume) as L:
+        call("r2s_volume_tv_iso_fwd", dt, ptr(volume), B, Dd, H, W, float(eps), ptr(tv_map), ptr(tv_sum), L.stream)
+    tv = (tv_sum / max(volume.numel(), 1)).squeeze()
+    return (float(lambda_tv) * tv), tv, tv_map
+
+
+def volume_loss_iso_bwd(volume, grad_output, lambda_tv, eps=1e-8):
+    """-> lambda * grad_output * d tv / d volume of volume_loss_iso_fwd."""
+    _check("volume", volume, 4)
+    _check("grad_output", grad_output)
+    _require(grad_output.device == volume.device, "grad_output must be on the same CUDA device as volume")
+    dt = _floating(volume)
+    _require(grad_output.dtype == volume.dtype and grad_output.numel() >= 1, "grad_output must be a scalar tensor")
+    _require(float(eps) > 0.0, "eps must be positive")
+    B, Dd, H, W = volume.shape
+    grad = torch.empty_like(volume)
+    with _Launch(volume) as L:
+        call("r2s_volume_tv_iso_bwd", dt, ptr(volume), ptr(grad_output), B, Dd, H, W, float(eps), float(lambda_tv),
+             ptr(grad), L.stream)
+    return grad
+
+
 def image_loss_l1_fwd(prediction, target, lambda_l1):
     """Mean L1 over (B,H,W) -> (lambda * l1, l1, l1_map)."""
     _check("prediction", prediction, 3)

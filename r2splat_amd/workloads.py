@@ -95,6 +95,14 @@ def config1(device="cpu", dtype=torch.float32, seed=0) -> Scene:
     return scene(100_000, 64, 256, 256, scale_range=(0.005, 0.05), seed=seed, dtype=dtype, device=device)
 
 
+def config2(device="cpu", dtype=torch.float32, seed=0) -> Scene:
+    """N = 500,000 Beta primitives (beta log-uniform in [0.5, 4]) with
+    uncertainty uniform in [0, 0.1], scales log-uniform in [0.003, 0.03],
+    C = 256 views of 512 x 512."""
+    return scene(500_000, 256, 512, 512, scale_range=(0.003, 0.03), seed=seed, dtype=dtype, device=device,
+                 beta_range=(0.5, 4.0), uncertainty_max=0.1)
+
+
 def config3_specs(n_jobs: int = 32, seed: int = 0):
     """Batched uneven reconstructions: per job N log-uniform in [10k, 400k],
     C uniform in {16, 32, 64, 128} views and image side uniform in

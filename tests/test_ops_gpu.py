@@ -489,6 +489,23 @@ def test_volume_tv_loss(device, dtype):
     torch.testing.assert_close(gd.cpu().double(), grad, atol=t, rtol=t)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(2, 4, 5, 6), (1, 1, 1, 7), (1, 33, 17, 65)])
+def test_volume_tv_iso_loss(device, dtype, shape):
+    g = torch.Generator().manual_seed(1)
+    vol = torch.randn(*shape, generator=g, dtype=torch.float64)
+    vr = vol.clone().requires_grad_()
+    tv = O.volume_tv_iso(vr, 1e-6)
+    (grad,) = torch.autograd.grad(1e-3 * tv, vr)
+    loss, tv_d, tv_map = ops.volume_loss_iso_fwd(vol.to(device, dtype), 1e-3, eps=1e-6)
+    t = tol(dtype, 2e-6, 1e-12)
+    assert abs(tv_d.item() - tv.item()) <= t * abs(tv.item())
+    assert abs(loss.item() - 1e-3 * tv.item()) <= t * abs(tv.item())
+    assert abs(float(tv_map.double().sum().cpu()) / vol.numel() - tv.item()) <= t * abs(tv.item())
+    gd = ops.volume_loss_iso_bwd(vol.to(device, dtype), torch.ones(1, device=device, dtype=dtype), 1e-3, eps=1e-6)
+    assert O.relative_error(gd, grad) <= tol(dtype, 1e-5, 1e-12)
+
+
 def test_image_l1_loss(device):
     p = torch.tensor([[[-2.0, 1.0], [3.0, -4.0]]], device=device)
     t = torch.zeros_like(p)

@@ -343,3 +343,41 @@ def test_volume_tv_matches_loops():
                             tot += abs(float(v[b, d + dd, h + dh, w + dw] - v[b, d, h, w]))
                             edges += 1
     assert abs(O.volume_tv(v).item() - tot / edges) < 1e-13
+
+
+def test_volume_tv_iso_known_answers():
+    """Isotropic TV: a constant volume gives sqrt(eps) and a zero gradient; a
+    linear ramp a x + b y + c z gives sqrt(a^2 + b^2 + c^2 + eps) wherever all
+    three forward differences exist; a single row reduces to the mean of
+    sqrt(dx^2 + eps); the gradient matches central differences."""
+    eps = 1e-8
+    const = torch.full((1, 3, 4, 5), 2.5, dtype=torch.float64, requires_grad=True)
+    tv = O.volume_tv_iso(const, eps)
+    assert abs(tv.item() - eps ** 0.5) < 1e-15
+    (g,) = torch.autograd.grad(tv, const)
+    assert float(g.abs().max()) == 0.0
+    d, h, w = torch.meshgrid(torch.arange(4.0), torch.arange(5.0), torch.arange(6.0), indexing="ij")
+    a, b, c = 0.3, -1.2, 0.7
+    ramp = (a * w + b * h + c * d).double()[None]
+    t_interior = (a * a + b * b + c * c + eps) ** 0.5
+    expect = 0.0
+    for dd in range(4):
+        for hh in range(5):
+            for ww in range(6):
+                sq = (c * c if dd < 3 else 0) + (b * b if hh < 4 else 0) + (a * a if ww < 5 else 0)
+                expect += (sq + eps) ** 0.5
+    assert abs(O.volume_tv_iso(ramp, eps).item() - expect / ramp.numel()) < 1e-14
+    assert abs(expect / ramp.numel() - t_interior) > 1e-3  # the boundary terms matter
+    row = torch.tensor([[[[0.0, 1.0, -2.0, 0.5]]]], dtype=torch.float64)
+    dxs = [1.0, -3.0, 2.5, 0.0]
+    assert abs(O.volume_tv_iso(row, eps).item() - sum((x * x + eps) ** 0.5 for x in dxs) / 4) < 1e-15
+    gen = torch.Generator().manual_seed(9)
+    v = torch.randn(2, 3, 4, 3, generator=gen, dtype=torch.float64, requires_grad=True)
+    (g,) = torch.autograd.grad(O.volume_tv_iso(v, 1e-6), v)
+    h_ = 1e-6
+    for idx in [(0, 0, 0, 0), (1, 2, 3, 2), (0, 1, 2, 1), (1, 0, 3, 0)]:
+        vp, vm = v.detach().clone(), v.detach().clone()
+        vp[idx] += h_
+        vm[idx] -= h_
+        fd = (O.volume_tv_iso(vp, 1e-6) - O.volume_tv_iso(vm, 1e-6)).item() / (2 * h_)
+        assert abs(fd - g[idx].item()) < 1e-7
