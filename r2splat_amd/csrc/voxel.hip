@@ -292,11 +292,12 @@ vox_bwd_isect_kernel(VoxArgs<S> A, const int64_t* __restrict__ isect_ids, const 
 }
 
 // ---------------------------------------------------------------------------
-// Moments form of the per-primitive VJP (float32, mean only, 8^3 tiles): one
-// workgroup per tile holds the tile's adjoint in LDS; each lane takes one
-// intersection of the tile and sums its primitive's moments over the 512
-// voxels row by row (every lane reads the same row: broadcast loads, and no
-// cross-lane reduction):
+// Moments form of the per-primitive VJP (float32, mean only, 8^3 tiles): each
+// workgroup takes a fixed chunk of the tile-sorted intersections (equal work
+// per workgroup, whatever the tile populations), one per lane, and visits the
+// chunk's tiles in order with each tile's adjoint staged in LDS; a lane sums its
+// primitive's moments over the 512 voxels row by row (every lane of the tile
+// reads the same row: broadcast loads, and no cross-lane reduction):
 //   Gaussian, delta = voxel - mean, G = v e^-sigma (terms that pass the cut-off):
 //     M0 = sum G, M_k = sum G delta_k, M_kl = sum G delta_k delta_l,
 //     v_mean = opac P M, v_conic = -opac (M_xx/2, M_xy, M_xz, M_yy/2, M_yz, M_zz/2),
@@ -307,8 +308,8 @@ vox_bwd_isect_kernel(VoxArgs<S> A, const int64_t* __restrict__ isect_ids, const 
 // the same VJP as voxel_vjp summed over the tile. sigma is evaluated in log2
 // units (P scaled by log2 e) and the cut-off "sigma >= 0 and |opac e^-sigma| >=
 // 1e-6" (Beta: "0 <= q < 1 and |opac h^beta| >= 1e-5") is one unsigned compare
-// of the bits of q against qlim. The tile's range is found by binary search in
-// the sorted tile ids; gradients are accumulated with one atomic per field.
+// of the bits of q against qlim. Gradients are accumulated with one atomic per
+// field.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMomT = 8;
 constexpr uint32_t kMomThreads = 256;
@@ -325,15 +326,6 @@ __device__ __forceinline__ uint32_t vox_qlim(float opac, float beta, bool is_bet
     return __float_as_uint(fminf(D, 0.99999994f)) + 1u;
 }
 
-__device__ __forceinline__ uint32_t lower_bound_i64(const int64_t* __restrict__ ids, uint32_t n, int64_t key) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ids[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 template <bool BETA>
 __global__ void __launch_bounds__(kMomThreads)
 vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, const float* __restrict__ v_render,
@@ -341,107 +333,125 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                        float* __restrict__ v_beta) {
     constexpr uint32_t T = kMomT, T3 = T * T * T;
     __shared__ __align__(16) float sv[T3];
-    __shared__ uint32_t range[2];
-    const uint32_t blk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        range[0] = lower_bound_i64(isect_ids, A.n_isects, int64_t(blk));
-        range[1] = lower_bound_i64(isect_ids, A.n_isects, int64_t(blk) + 1);
-    }
-    const uint32_t tx = blk % A.ntx, rest = blk / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
-    for (uint32_t p = threadIdx.x; p < T3; p += blockDim.x) {
-        const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
-        sv[p] = (i < A.dz && j < A.dy && k < A.dx) ? v_render[(size_t(i) * A.dy + j) * A.dx + k] : 0.f;
-    }
+    __shared__ int64_t ids[kMomThreads];
+    // a chunk of kMomThreads consecutive (tile-sorted) intersections, one per thread; the chunk's
+    // tiles are visited in order, each one's adjoint staged in LDS in turn
+    const uint32_t e0 = blockIdx.x * kMomThreads, e1 = min(e0 + kMomThreads, A.n_isects);
+    const uint32_t e = e0 + threadIdx.x;
+    const int64_t mine = e < e1 ? isect_ids[e] : INT64_MAX;
+    ids[threadIdx.x] = mine;
     __syncthreads();
-    const uint32_t start = range[0], end = range[1];
-    const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f, z0 = float(tz * T) + 0.5f;
-    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-        const int32_t g = A.flatten[e];
-        if (g < 0 || uint32_t(g) >= A.N) continue;
-        const size_t G = size_t(g);
-        const float mx = A.means3d[3 * G], my = A.means3d[3 * G + 1], mz = A.means3d[3 * G + 2];
-        float P[6];
+    const uint32_t n_here = e1 - e0;
+    const int64_t nt = int64_t(A.ntx) * A.nty * A.ntz;
+    uint32_t pos = 0;  // first chunk entry of the current tile
+    while (pos < n_here) {
+        const int64_t tile = ids[pos];
+        uint32_t lo = pos, hi = n_here;  // first entry of the next tile
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ids[mid] <= tile) lo = mid + 1; else hi = mid;
+        }
+        const bool valid_tile = tile >= 0 && tile < nt;
+        const uint32_t blk = valid_tile ? uint32_t(tile) : 0u;
+        const uint32_t tx = blk % A.ntx, rest = blk / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
+        if (valid_tile)
+            for (uint32_t p = threadIdx.x; p < T3; p += blockDim.x) {
+                const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
+                sv[p] = (i < A.dz && j < A.dy && k < A.dx) ? v_render[(size_t(i) * A.dy + j) * A.dx + k] : 0.f;
+            }
+        __syncthreads();
+        if (valid_tile && mine == tile) {
+            const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f, z0 = float(tz * T) + 0.5f;
+            const int32_t g = A.flatten[e];
+            if (g >= 0 && uint32_t(g) < A.N) {
+                const size_t G = size_t(g);
+                const float mx = A.means3d[3 * G], my = A.means3d[3 * G + 1], mz = A.means3d[3 * G + 2];
+                float P[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) P[k] = A.conics[6 * G + k];
-        const float opac = A.opac[g], beta = BETA ? A.betas[g] : 0.f;
-        const uint32_t qlim = vox_qlim(opac, beta, BETA);
-        // q = a dx^2 + bxy dx dy + bxz dx dz + cyy dy^2 + cyz dy dz + czz dz^2
-        const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
-        const float a = s * P[0], bxy = t2 * P[1], bxz = t2 * P[2], cyy = s * P[3], cyz = t2 * P[4], czz = s * P[5];
-        const float bm1 = beta - 1.f;
-        float m0 = 0.f, m_x = 0.f, m_y = 0.f, m_z = 0.f, mxx = 0.f, myy = 0.f, mzz = 0.f, mxy = 0.f, mxz = 0.f,
-              myz = 0.f, se = 0.f, sl = 0.f;
-        const float dx0 = x0 - mx;
-        if (qlim != 0u) {
-            for (uint32_t z = 0; z < T; ++z) {
-                const float dz = z0 + float(z) - mz;
-                for (uint32_t y = 0; y < T; ++y) {
-                    const float dy = y0 + float(y) - my;
-                    const float4* row = reinterpret_cast<const float4*>(sv + (z * T + y) * T);
-                    const float4 va = row[0], vb = row[1];
-                    const float v[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-                    const float c1 = fmaf(bxy, dy, bxz * dz);
-                    const float c0 = fmaf(dy, fmaf(cyy, dy, cyz * dz), czz * dz * dz);
-                    float r0 = 0.f, r1 = 0.f, r2 = 0.f, re = 0.f, rl = 0.f;
+                for (int k = 0; k < 6; ++k) P[k] = A.conics[6 * G + k];
+                const float opac = A.opac[g], beta = BETA ? A.betas[g] : 0.f;
+                const uint32_t qlim = vox_qlim(opac, beta, BETA);
+                // q = a dx^2 + bxy dx dy + bxz dx dz + cyy dy^2 + cyz dy dz + czz dz^2
+                const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
+                const float a = s * P[0], bxy = t2 * P[1], bxz = t2 * P[2], cyy = s * P[3], cyz = t2 * P[4], czz = s * P[5];
+                const float bm1 = beta - 1.f;
+                float m0 = 0.f, m_x = 0.f, m_y = 0.f, m_z = 0.f, mxx = 0.f, myy = 0.f, mzz = 0.f, mxy = 0.f, mxz = 0.f,
+                      myz = 0.f, se = 0.f, sl = 0.f;
+                const float dx0 = x0 - mx;
+                if (qlim != 0u) {
+                    for (uint32_t z = 0; z < T; ++z) {
+                        const float dz = z0 + float(z) - mz;
+                        for (uint32_t y = 0; y < T; ++y) {
+                            const float dy = y0 + float(y) - my;
+                            const float4* row = reinterpret_cast<const float4*>(sv + (z * T + y) * T);
+                            const float4 va = row[0], vb = row[1];
+                            const float v[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+                            const float c1 = fmaf(bxy, dy, bxz * dz);
+                            const float c0 = fmaf(dy, fmaf(cyy, dy, cyz * dz), czz * dz * dz);
+                            float r0 = 0.f, r1 = 0.f, r2 = 0.f, re = 0.f, rl = 0.f;
 #pragma unroll
-                    for (int x = 0; x < 8; ++x) {
-                        const float dx = dx0 + float(x);
-                        const float q = fmaf(dx, fmaf(a, dx, c1), c0);
-                        const bool in = __float_as_uint(q) < qlim;
-                        float gv;
-                        if constexpr (BETA) {
-                            const float h = 1.f - q, lg = __log2f(h);
-                            gv = in ? v[x] * __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;
-                            const float ve = gv * h;
-                            re += ve;
-                            rl = fmaf(ve, in ? lg : 0.f, rl);
-                        } else {
-                            gv = in ? v[x] * __builtin_amdgcn_exp2f(-q) : 0.f;
+                            for (int x = 0; x < 8; ++x) {
+                                const float dx = dx0 + float(x);
+                                const float q = fmaf(dx, fmaf(a, dx, c1), c0);
+                                const bool in = __float_as_uint(q) < qlim;
+                                float gv;
+                                if constexpr (BETA) {
+                                    const float h = 1.f - q, lg = __log2f(h);
+                                    gv = in ? v[x] * __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;
+                                    const float ve = gv * h;
+                                    re += ve;
+                                    rl = fmaf(ve, in ? lg : 0.f, rl);
+                                } else {
+                                    gv = in ? v[x] * __builtin_amdgcn_exp2f(-q) : 0.f;
+                                }
+                                r0 += gv;
+                                const float gx = gv * dx;
+                                r1 += gx;
+                                r2 = fmaf(gx, dx, r2);
+                            }
+                            m0 += r0;
+                            m_x += r1;
+                            mxx += r2;
+                            m_y = fmaf(dy, r0, m_y);
+                            m_z = fmaf(dz, r0, m_z);
+                            myy = fmaf(dy * dy, r0, myy);
+                            mzz = fmaf(dz * dz, r0, mzz);
+                            myz = fmaf(dy * dz, r0, myz);
+                            mxy = fmaf(dy, r1, mxy);
+                            mxz = fmaf(dz, r1, mxz);
+                            if constexpr (BETA) {
+                                se += re;
+                                sl += rl;
+                            }
                         }
-                        r0 += gv;
-                        const float gx = gv * dx;
-                        r1 += gx;
-                        r2 = fmaf(gx, dx, r2);
                     }
-                    m0 += r0;
-                    m_x += r1;
-                    mxx += r2;
-                    m_y = fmaf(dy, r0, m_y);
-                    m_z = fmaf(dz, r0, m_z);
-                    myy = fmaf(dy * dy, r0, myy);
-                    mzz = fmaf(dz * dz, r0, mzz);
-                    myz = fmaf(dy * dz, r0, myz);
-                    mxy = fmaf(dy, r1, mxy);
-                    mxz = fmaf(dz, r1, mxz);
-                    if constexpr (BETA) {
-                        se += re;
-                        sl += rl;
-                    }
+                }
+                const float k = BETA ? 2.f * opac * beta : opac;
+                const float gm[3] = {k * (P[0] * m_x + P[1] * m_y + P[2] * m_z), k * (P[1] * m_x + P[3] * m_y + P[4] * m_z),
+                                     k * (P[2] * m_x + P[4] * m_y + P[5] * m_z)};
+                float gc[6];
+                if constexpr (BETA) {
+                    const float kb = -opac * beta;
+                    gc[0] = kb * mxx; gc[1] = 2.f * kb * mxy; gc[2] = 2.f * kb * mxz;
+                    gc[3] = kb * myy; gc[4] = 2.f * kb * myz; gc[5] = kb * mzz;
+                } else {
+                    gc[0] = -0.5f * opac * mxx; gc[1] = -opac * mxy; gc[2] = -opac * mxz;
+                    gc[3] = -0.5f * opac * myy; gc[4] = -opac * myz; gc[5] = -0.5f * opac * mzz;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) atomic_add(v_xyz + 3 * G + q, gm[q]);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) atomic_add(v_c + 6 * G + q, gc[q]);
+                if constexpr (BETA) {
+                    atomic_add(v_opac + G, se);
+                    atomic_add(v_beta + G, opac * 0.6931471805599453f * sl);
+                } else {
+                    atomic_add(v_opac + G, m0);
                 }
             }
         }
-        const float k = BETA ? 2.f * opac * beta : opac;
-        const float gm[3] = {k * (P[0] * m_x + P[1] * m_y + P[2] * m_z), k * (P[1] * m_x + P[3] * m_y + P[4] * m_z),
-                             k * (P[2] * m_x + P[4] * m_y + P[5] * m_z)};
-        float gc[6];
-        if constexpr (BETA) {
-            const float kb = -opac * beta;
-            gc[0] = kb * mxx; gc[1] = 2.f * kb * mxy; gc[2] = 2.f * kb * mxz;
-            gc[3] = kb * myy; gc[4] = 2.f * kb * myz; gc[5] = kb * mzz;
-        } else {
-            gc[0] = -0.5f * opac * mxx; gc[1] = -opac * mxy; gc[2] = -opac * mxz;
-            gc[3] = -0.5f * opac * myy; gc[4] = -opac * myz; gc[5] = -0.5f * opac * mzz;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) atomic_add(v_xyz + 3 * G + q, gm[q]);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) atomic_add(v_c + 6 * G + q, gc[q]);
-        if constexpr (BETA) {
-            atomic_add(v_opac + G, se);
-            atomic_add(v_beta + G, opac * 0.6931471805599453f * sl);
-        } else {
-            atomic_add(v_opac + G, m0);
-        }
+        __syncthreads();  // every thread is done with this tile's adjoint
+        pos = lo;
     }
 }
 
@@ -558,7 +568,7 @@ extern "C" int r2s_voxelize_to_voxels_bwd_per_gaussian(
             const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
             if (tile_size == kMomT && !v_render_var && n_tiles < (1ull << 31)) {  // moments form
                 auto km = beta ? vox_bwd_moments_kernel<true> : vox_bwd_moments_kernel<false>;
-                km<<<uint32_t(n_tiles), kMomThreads, 0, stream>>>(A, isect_ids, (const float*)v_render,
+                km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(A, isect_ids, (const float*)v_render,
                                                                   (float*)v_means3d, (float*)v_conics,
                                                                   (float*)v_opacities, (float*)v_betas);
                 return check_launch("voxelize_to_voxels_bwd_per_gaussian");
