@@ -387,6 +387,25 @@ int r2s_fused_param_vjp(const float* means, const float* quats, const float* sca
                         float* v_means, float* v_quats, float* v_scales, float* v_densities,
                         float* v_betas, r2s_stream_t stream);
 
+/* Fused float32 voxel VJP (mean only, 8^3 tiles): the whole per-primitive
+ * voxelize backward (VoxelizeToVoxelsBwd.cu per-Gaussian kernel, as called by
+ * explicit.py:327-350) without atomics. radii int32 [N,3] and tile_cum uint32
+ * [N+1] (exclusive scan of the per-primitive tile counts) are the binning's
+ * own (r2s_isect_tiles_3d_count); isect_ids / flatten_ids its sorted output.
+ * One partial per intersection is written at its emission index (partials:
+ * r2s_voxel_partials_bytes(n_isects) bytes) and summed per primitive in a
+ * fixed order, so the result is bitwise reproducible. Outputs overwritten:
+ * v_means3d [N,3], v_conics [N,6], v_opacities [N], v_betas [N] (iff betas). */
+size_t r2s_voxel_partials_bytes(uint32_t n_isects);
+int r2s_voxelize_fused_vjp(const float* means3d, const float* conics, const float* opacities,
+                           const float* betas, const int32_t* radii, const uint32_t* tile_cum,
+                           const int64_t* isect_ids, const int32_t* flatten_ids, uint32_t n_isects,
+                           uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, uint32_t tile_size,
+                           uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                           const float* v_render, float* partials, size_t partials_bytes,
+                           float* v_means3d, float* v_conics, float* v_opacities, float* v_betas,
+                           r2s_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Training-step kernels (no counterpart in the reference, which leaves image */
 /* losses beyond L1 and the optimizer to the caller; used by graph-captured   */

@@ -34,19 +34,6 @@ __device__ __forceinline__ bool rect_2d(const S* __restrict__ means2d, const int
 }
 
 template <typename S>
-__device__ __forceinline__ bool box_3d(const S* __restrict__ means3d, const int32_t* __restrict__ radii,
-                                       size_t n, uint32_t tile, uint32_t ntx, uint32_t nty,
-                                       uint32_t ntz, uint32_t lo[3], uint32_t hi[3]) {
-    const int32_t r[3] = {radii[3 * n], radii[3 * n + 1], radii[3 * n + 2]};
-    if (r[0] <= 0 || r[1] <= 0 || r[2] <= 0) return false;
-    const uint32_t nt[3] = {ntx, nty, ntz};
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) ok = tile_span(means3d[3 * n + k], r[k], tile, nt[k], lo[k], hi[k]) && ok;
-    return ok;
-}
-
-template <typename S>
 __global__ void __launch_bounds__(256)
 count_2d_kernel(const S* __restrict__ means2d, const int32_t* __restrict__ radii, bool rect,
                 uint32_t total, uint32_t tile, uint32_t ntx, uint32_t nty,

@@ -25,6 +25,22 @@ __device__ __forceinline__ bool tile_span(S mean, int32_t radius, uint32_t tile,
     return hi > lo;
 }
 
+// Tile box of one 3D primitive (IsectTiles3DKernel.cu:36-45): per axis the
+// tile_span of mean +/- radius; false when a radius is not positive or a span
+// is empty. Emission order within the box is z, then y, then x.
+template <typename S>
+__device__ __forceinline__ bool box_3d(const S* __restrict__ means3d, const int32_t* __restrict__ radii,
+                                       size_t n, uint32_t tile, uint32_t ntx, uint32_t nty,
+                                       uint32_t ntz, uint32_t lo[3], uint32_t hi[3]) {
+    const int32_t r[3] = {radii[3 * n], radii[3 * n + 1], radii[3 * n + 2]};
+    if (r[0] <= 0 || r[1] <= 0 || r[2] <= 0) return false;
+    const uint32_t nt[3] = {ntx, nty, ntz};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ok = tile_span(means3d[3 * n + k], r[k], tile, nt[k], lo[k], hi[k]) && ok;
+    return ok;
+}
+
 size_t scan_workspace_bytes(uint64_t n);
 // Exclusive scan of n uint32 values: out[0..n], out[n] = total (also written to
 // *d_total when non-null). `out` may alias `in`; `out` must hold n + 1 values.

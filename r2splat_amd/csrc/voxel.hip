@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "binning.hpp"
 #include "raster_common.hpp"
 
 namespace r2s {
@@ -312,6 +313,7 @@ vox_bwd_isect_kernel(VoxArgs<S> A, const int64_t* __restrict__ isect_ids, const 
 // field.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMomT = 8;
+constexpr uint32_t kVoxPartial = 12;  // floats per intersection partial (11 used, 16-byte aligned)
 constexpr uint32_t kMomThreads = 256;
 constexpr float kL2E = 1.4426950408889634f;
 
@@ -330,7 +332,8 @@ template <bool BETA>
 __global__ void __launch_bounds__(kMomThreads)
 vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, const float* __restrict__ v_render,
                        float* __restrict__ v_xyz, float* __restrict__ v_c, float* __restrict__ v_opac,
-                       float* __restrict__ v_beta) {
+                       float* __restrict__ v_beta, const int32_t* __restrict__ radii,
+                       const uint32_t* __restrict__ cum, float* __restrict__ partials) {
     constexpr uint32_t T = kMomT, T3 = T * T * T;
     __shared__ __align__(16) float sv[T3];
     __shared__ int64_t ids[kMomThreads];
@@ -438,21 +441,60 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                     gc[0] = -0.5f * opac * mxx; gc[1] = -opac * mxy; gc[2] = -opac * mxz;
                     gc[3] = -0.5f * opac * myy; gc[4] = -opac * myz; gc[5] = -0.5f * opac * mzz;
                 }
-#pragma unroll
-                for (int q = 0; q < 3; ++q) atomic_add(v_xyz + 3 * G + q, gm[q]);
-#pragma unroll
-                for (int q = 0; q < 6; ++q) atomic_add(v_c + 6 * G + q, gc[q]);
-                if constexpr (BETA) {
-                    atomic_add(v_opac + G, se);
-                    atomic_add(v_beta + G, opac * 0.6931471805599453f * sl);
+                const float go = BETA ? se : m0, gb = BETA ? opac * 0.6931471805599453f * sl : 0.f;
+                if (partials) {
+                    // the partial of this intersection at its emission index: the binning wrote
+                    // the primitive's tiles from cum[n] on, in z, y, x order over its tile box
+                    uint32_t lo[3], hi[3];
+                    if (box_3d(A.means3d, radii, G, T, A.ntx, A.nty, A.ntz, lo, hi) && tx >= lo[0] && tx < hi[0] &&
+                        ty >= lo[1] && ty < hi[1] && tz >= lo[2] && tz < hi[2]) {
+                        const uint32_t w = cum[G] + ((tz - lo[2]) * (hi[1] - lo[1]) + (ty - lo[1])) * (hi[0] - lo[0]) +
+                                           (tx - lo[0]);
+                        if (w < A.n_isects) {
+                            float4* out = reinterpret_cast<float4*>(partials + kVoxPartial * size_t(w));
+                            out[0] = make_float4(gm[0], gm[1], gm[2], gc[0]);
+                            out[1] = make_float4(gc[1], gc[2], gc[3], gc[4]);
+                            out[2] = make_float4(gc[5], go, gb, 0.f);
+                        }
+                    }
                 } else {
-                    atomic_add(v_opac + G, m0);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) atomic_add(v_xyz + 3 * G + q, gm[q]);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) atomic_add(v_c + 6 * G + q, gc[q]);
+                    atomic_add(v_opac + G, go);
+                    if constexpr (BETA) atomic_add(v_beta + G, gb);
                 }
             }
         }
         __syncthreads();  // every thread is done with this tile's adjoint
         pos = lo;
     }
+}
+
+// Per-primitive sums of the intersection partials [cum[n], cum[n+1]) in
+// emission order (deterministic; no atomics).
+__global__ void __launch_bounds__(256)
+vox_partials_reduce_kernel(const uint32_t* __restrict__ cum, const float* __restrict__ partials, uint32_t N,
+                           uint32_t n_isects, float* __restrict__ v_xyz, float* __restrict__ v_c,
+                           float* __restrict__ v_opac, float* __restrict__ v_beta) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float acc[11] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const uint32_t lo = min(cum[n], n_isects), hi = min(cum[n + 1], n_isects);
+    for (uint32_t j = lo; j < hi; ++j) {
+        const float4* p = reinterpret_cast<const float4*>(partials + kVoxPartial * size_t(j));
+        const float4 a = p[0], b = p[1], c = p[2];
+        acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+        acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+        acc[8] += c.x; acc[9] += c.y; acc[10] += c.z;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v_xyz[3 * size_t(n) + k] = acc[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v_c[6 * size_t(n) + k] = acc[3 + k];
+    v_opac[n] = acc[9];
+    if (v_beta) v_beta[n] = acc[10];
 }
 
 inline int check_vox_tile(uint32_t T, uint32_t ntx, uint32_t nty, uint32_t ntz, uint32_t dx,
@@ -568,9 +610,9 @@ extern "C" int r2s_voxelize_to_voxels_bwd_per_gaussian(
             const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
             if (tile_size == kMomT && !v_render_var && n_tiles < (1ull << 31)) {  // moments form
                 auto km = beta ? vox_bwd_moments_kernel<true> : vox_bwd_moments_kernel<false>;
-                km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(A, isect_ids, (const float*)v_render,
-                                                                  (float*)v_means3d, (float*)v_conics,
-                                                                  (float*)v_opacities, (float*)v_betas);
+                km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(
+                    A, isect_ids, (const float*)v_render, (float*)v_means3d, (float*)v_conics, (float*)v_opacities,
+                    (float*)v_betas, nullptr, nullptr, nullptr);
                 return check_launch("voxelize_to_voxels_bwd_per_gaussian");
             }
         }
@@ -582,4 +624,37 @@ extern "C" int r2s_voxelize_to_voxels_bwd_per_gaussian(
             (S*)v_opacities, (S*)v_betas, v_render_var ? (S*)v_uncertainties : nullptr);
         return check_launch("voxelize_to_voxels_bwd_per_gaussian");
     });
+}
+
+extern "C" size_t r2s_voxel_partials_bytes(uint32_t n_isects) {
+    return size_t(n_isects) * kVoxPartial * sizeof(float);
+}
+
+extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* conics, const float* opacities,
+                                      const float* betas, const int32_t* radii, const uint32_t* tile_cum,
+                                      const int64_t* isect_ids, const int32_t* flatten_ids, uint32_t n_isects,
+                                      uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, uint32_t tile_size,
+                                      uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                                      const float* v_render, float* partials, size_t partials_bytes,
+                                      float* v_means3d, float* v_conics, float* v_opacities, float* v_betas,
+                                      r2s_stream_t stream) {
+    if (int rc = check_vox_tile(tile_size, n_tiles_x, n_tiles_y, n_tiles_z, dim_x, dim_y, dim_z)) return rc;
+    R2S_REQUIRE(tile_size == kMomT, "the fused voxel VJP takes %u^3 tiles (got %u)", kMomT, tile_size);
+    R2S_REQUIRE(uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z < (1ull << 31), "too many tiles");
+    R2S_REQUIRE(partials_bytes >= r2s_voxel_partials_bytes(n_isects), "partials buffer too small: %zu < %zu",
+                partials_bytes, r2s_voxel_partials_bytes(n_isects));
+    const bool beta = betas != nullptr;
+    R2S_REQUIRE(!beta || v_betas != nullptr, "v_betas is required with betas");
+    if (N == 0) return R2S_OK;
+    if (n_isects > 0) {
+        auto A = make_vox_args<float>(means3d, conics, opacities, betas, nullptr, nullptr, flatten_ids, n_isects, N,
+                                      dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
+        auto km = beta ? vox_bwd_moments_kernel<true> : vox_bwd_moments_kernel<false>;
+        km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(A, isect_ids, v_render, nullptr, nullptr,
+                                                                         nullptr, nullptr, radii, tile_cum, partials);
+        if (int rc = check_launch("voxelize_fused_vjp")) return rc;
+    }
+    vox_partials_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(tile_cum, partials, N, n_isects, v_means3d,
+                                                                     v_conics, v_opacities, beta ? v_betas : nullptr);
+    return check_launch("voxelize_fused_vjp_reduce");
 }

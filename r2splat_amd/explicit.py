@@ -21,6 +21,7 @@ import torch
 
 from . import fused
 from . import hip as ops
+from .hip import _wrapper as _ops_impl
 from ._beta import beta_path_log_grad, beta_path_ratio
 from ._validation import _constant, _options, _primitives, _shape, _tensor, _value_checks, _vector
 from .profiling import stage
@@ -40,6 +41,11 @@ BACKEND = os.environ.get("R2S_BACKEND", "fused")
 
 def _use_fused(means, beta, uncertainty) -> bool:
     return BACKEND == "fused" and means.dtype == torch.float32 and uncertainty is None
+
+
+def _use_fused_voxels(means) -> bool:
+    """float32 mean-adjoint voxel VJPs (8^3 tiles) run without atomics (fused.voxel_vjp)."""
+    return BACKEND == "fused" and means.dtype == torch.float32 and VOXEL_TILE == 8
 
 
 @dataclass
@@ -303,10 +309,10 @@ def _voxel_state(prims, extent, center, shape, eps, radius_clip) -> dict:
                                                                 height, depth, eps, radius_clip)
     tx, ty, tz = (width + T - 1) // T, (height + T - 1) // T, (depth + T - 1) // T
     with stage("voxel_bin"):
-        _, isect_ids, flatten_ids = ops.isect_tiles_3d(mapped, radii, T, tx, ty, tz)
+        _, isect_ids, flatten_ids, tile_cum = _ops_impl._isect_tiles_3d(mapped, radii, T, tx, ty, tz)
         offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
-    return dict(covars=covars, mapped=mapped, conics=conics, offsets=offsets, isect_ids=isect_ids,
-                flatten_ids=flatten_ids)
+    return dict(covars=covars, mapped=mapped, conics=conics, radii=radii, tile_cum=tile_cum, offsets=offsets,
+                isect_ids=isect_ids, flatten_ids=flatten_ids)
 
 
 def _voxel_render(b, prims, shape):
@@ -403,6 +409,9 @@ def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Opti
                 *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, v_variance, uncertainty,
                 context.detach_variance, width, height, depth, T)
             v_uncertainty = grads[-1]
+        elif _use_fused_voxels(means):
+            grads = fused.voxel_vjp(b, densities, beta, v_volume, (width, height, depth), T)
+            v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
         else:
             if context.backward_mode == "primitive":
                 grads = getattr(ops, prefix + "voxelize_to_voxels_bwd_per_gaussian")(

@@ -191,3 +191,30 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     if fwd.is_beta:
         return v_means, v_quats, v_scales, v_dens, v_beta
     return v_means, v_quats, v_scales, v_dens
+
+
+def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8):
+    """Per-primitive voxelize VJP (float32, 8^3 tiles) from a voxel binning
+    state (mapped, conics, radii, tile_cum, isect_ids, flatten_ids): one partial
+    per intersection, summed per primitive in a fixed order (no atomics).
+    -> (v_mapped [N,3], v_conics [N,6], v_densities [N][, v_beta [N]])."""
+    mapped = state["mapped"]
+    device = mapped.device
+    N = mapped.shape[0]
+    W, H, D = dims
+    T = int(tile_size)
+    ids, flat = state["isect_ids"], state["flatten_ids"]
+    n_isects = ids.numel()
+    partials = torch.empty((max(1, be.lib().r2s_voxel_partials_bytes(n_isects)),), dtype=torch.uint8, device=device)
+    v_mapped = torch.empty((N, 3), dtype=torch.float32, device=device)
+    v_conics = torch.empty((N, 6), dtype=torch.float32, device=device)
+    v_dens = torch.empty((N,), dtype=torch.float32, device=device)
+    v_beta = torch.empty((N,), dtype=torch.float32, device=device) if beta is not None else None
+    with torch.cuda.device(device):
+        call("r2s_voxelize_fused_vjp", ptr(mapped), ptr(state["conics"]), ptr(densities), ptr(beta),
+             ptr(state["radii"]), ptr(state["tile_cum"]), ptr(ids), ptr(flat), n_isects, N, W, H, D, T,
+             -(-W // T), -(-H // T), -(-D // T), ptr(v_volume), ptr(partials), partials.numel(), ptr(v_mapped),
+             ptr(v_conics), ptr(v_dens), ptr(v_beta), be.stream_of(device))
+    if beta is not None:
+        return v_mapped, v_conics, v_dens, v_beta
+    return v_mapped, v_conics, v_dens

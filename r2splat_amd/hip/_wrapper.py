@@ -284,6 +284,12 @@ def isect_offset_encode(isect_ids, C, n_tiles_dim1, n_tiles_dim0):
 
 def isect_tiles_3d(means3d, radii, tile_size, n_tiles_dim2, n_tiles_dim1, n_tiles_dim0):
     """-> (tiles_per_gauss int32 [N], isect_ids int64 [I], flatten_ids int32 [I]); x/y/z tile counts."""
+    return _isect_tiles_3d(means3d, radii, tile_size, n_tiles_dim2, n_tiles_dim1, n_tiles_dim0)[:3]
+
+
+def _isect_tiles_3d(means3d, radii, tile_size, n_tiles_dim2, n_tiles_dim1, n_tiles_dim0):
+    """isect_tiles_3d plus the exclusive scan of the tile counts, int32 [N + 1]
+    (the first emission index of each primitive)."""
     _check("means3d", means3d, 2)
     _check("radii", radii, 2, dtype=torch.int32)
     _require(radii.device == means3d.device, "radii must be on the same CUDA device as means3d")
@@ -309,7 +315,7 @@ def isect_tiles_3d(means3d, radii, tile_size, n_tiles_dim2, n_tiles_dim1, n_tile
             sort_ws = _workspace(nbytes, dev)
             call("r2s_isect_tiles_3d_emit", dt, ptr(means3d), ptr(radii), N, int(tile_size), nx, ny, nz,
                  ptr(cum), n_isects, ptr(isect_ids), ptr(flatten_ids), ptr(sort_ws), nbytes, L.stream)
-    return tiles, isect_ids, flatten_ids
+    return tiles, isect_ids, flatten_ids, cum
 
 
 def isect_offset_encode_3d(isect_ids, n_tiles_dim2, n_tiles_dim1, n_tiles_dim0):
