@@ -396,6 +396,16 @@ int r2s_fused_param_vjp(const float* means, const float* quats, const float* sca
  * r2s_voxel_partials_bytes(n_isects) bytes) and summed per primitive in a
  * fixed order, so the result is bitwise reproducible. Outputs overwritten:
  * v_means3d [N,3], v_conics [N,6], v_opacities [N], v_betas [N] (iff betas). */
+/* Fused float32 voxel forward (mean only, 8^3 tiles) over the same binning:
+ * render [dim_z, dim_y, dim_x] (overwritten). Replaces voxelize_to_voxels_fwd
+ * (VoxelizeToVoxelsFwd.cu:181-219 and the Beta kernel) for that case; the
+ * summation order within a voxel differs from the operator (rounding only). */
+int r2s_voxelize_fused_fwd(const float* means3d, const float* conics, const float* opacities,
+                           const float* betas, const int32_t* tile_offsets, const int32_t* flatten_ids,
+                           uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+                           uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                           float* render, r2s_stream_t stream);
+
 size_t r2s_voxel_partials_bytes(uint32_t n_isects);
 int r2s_voxelize_fused_vjp(const float* means3d, const float* conics, const float* opacities,
                            const float* betas, const int32_t* radii, const uint32_t* tile_cum,

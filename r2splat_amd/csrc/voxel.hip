@@ -313,6 +313,12 @@ vox_bwd_isect_kernel(VoxArgs<S> A, const int64_t* __restrict__ isect_ids, const 
 // field.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMomT = 8;
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 constexpr uint32_t kVoxPartial = 12;  // floats per intersection partial (11 used, 16-byte aligned)
 constexpr uint32_t kMomThreads = 256;
 constexpr float kL2E = 1.4426950408889634f;
@@ -469,6 +475,108 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
         }
         __syncthreads();  // every thread is done with this tile's adjoint
         pos = lo;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused float32 forward (mean only, 8^3 tiles): kVoxFwdWaves waves per tile,
+// wave k summing the k-th contiguous slice of the tile's list; lane = one
+// (y, z) row of 8 voxels. Records (mean, the quadratic form in log2 units for
+// Gaussians, qlim, opacity, beta) are built from the primitive arrays a batch
+// of 64 ahead by the lanes and read back from LDS by broadcast; per record a
+// lane forms the row terms once and evaluates q = a dx^2 + (bxy dy + bxz dz) dx
+// + c(dy, dz) for its 8 voxels. The slices' partial volumes are summed in LDS
+// in wave order, so the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int kVoxFwdWaves = 4;
+
+struct __align__(16) VoxRec {
+    float mx, my, mz, a, bxy, bxz, cyy, cyz, czz, w, beta;
+    uint32_t qlim;
+};
+static_assert(sizeof(VoxRec) == 48, "voxel record must be 48 bytes");
+
+template <bool BETA>
+__device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, int32_t i, int32_t hi) {
+    VoxRec r = {};
+    if (i >= hi) return r;
+    const int32_t g = A.flatten[i];
+    if (g < 0 || uint32_t(g) >= A.N) return r;
+    const size_t G = size_t(g);
+    const float* c = A.conics + 6 * G;
+    const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
+    r.mx = A.means3d[3 * G];
+    r.my = A.means3d[3 * G + 1];
+    r.mz = A.means3d[3 * G + 2];
+    r.a = s * c[0];
+    r.bxy = t2 * c[1];
+    r.bxz = t2 * c[2];
+    r.cyy = s * c[3];
+    r.cyz = t2 * c[4];
+    r.czz = s * c[5];
+    r.w = A.opac[g];
+    r.beta = BETA ? A.betas[g] : 0.f;
+    r.qlim = vox_qlim(r.w, r.beta, BETA);
+    return r;
+}
+
+template <bool BETA>
+__global__ void __launch_bounds__(kVoxFwdWaves * kWave)
+vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
+    constexpr uint32_t T = kMomT;
+    __shared__ VoxRec stage[kVoxFwdWaves][kWave];
+    __shared__ float part[kVoxFwdWaves][T * T * T];
+    const uint32_t blk = blockIdx.x;
+    const uint32_t nt = A.ntx * A.nty * A.ntz;
+    int32_t start = A.offsets[blk];
+    int32_t end = (blk + 1 == nt) ? int32_t(A.n_isects) : A.offsets[blk + 1];
+    start = max(0, min(start, int32_t(A.n_isects)));
+    end = max(start, min(end, int32_t(A.n_isects)));
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+    const int32_t per = (end - start + kVoxFwdWaves - 1) / kVoxFwdWaves;
+    const int32_t lo = min(end, start + int32_t(wave) * per), hi = min(end, lo + per);
+    const uint32_t tx = blk % A.ntx, rest = blk / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
+    const uint32_t ly = lane % T, lz = lane / T;
+    const float py = float(ty * T + ly) + 0.5f, pz = float(tz * T + lz) + 0.5f, px0 = float(tx * T) + 0.5f;
+    float acc[T];
+#pragma unroll
+    for (uint32_t x = 0; x < T; ++x) acc[x] = 0.f;
+    VoxRec* slot = stage[wave];
+    VoxRec next = load_vox_rec<BETA>(A, lo + int32_t(lane), hi);
+    for (int32_t base = lo; base < hi; base += kWave) {
+        wave_sync_lds();
+        slot[lane] = next;
+        wave_sync_lds();
+        next = load_vox_rec<BETA>(A, base + kWave + int32_t(lane), hi);
+        const int32_t n = min(int32_t(kWave), hi - base);
+        for (int32_t j = 0; j < n; ++j) {
+            const VoxRec r = slot[j];
+            if (r.qlim == 0u) continue;  // wave-uniform (every lane reads the same record)
+            const float dy = py - r.my, dz = pz - r.mz, dx0 = px0 - r.mx;
+            const float c1 = fmaf(r.bxy, dy, r.bxz * dz);
+            const float c0 = fmaf(dy, fmaf(r.cyy, dy, r.cyz * dz), r.czz * dz * dz);
+#pragma unroll
+            for (uint32_t x = 0; x < T; ++x) {
+                const float dx = dx0 + float(x);
+                const float q = fmaf(dx, fmaf(r.a, dx, c1), c0);
+                float e;
+                if constexpr (BETA)
+                    e = __builtin_amdgcn_exp2f(r.beta * __log2f(1.f - q));
+                else
+                    e = __builtin_amdgcn_exp2f(-q);
+                acc[x] = fmaf(__float_as_uint(q) < r.qlim ? e : 0.f, r.w, acc[x]);
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t x = 0; x < T; ++x) part[wave][(lz * T + ly) * T + x] = acc[x];
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < T * T * T; p += blockDim.x) {
+        float v = 0.f;
+#pragma unroll
+        for (int k = 0; k < kVoxFwdWaves; ++k) v += part[k][p];
+        const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
+        if (i < A.dz && j < A.dy && k < A.dx) out[(size_t(i) * A.dy + j) * A.dx + k] = v;
     }
 }
 
@@ -657,4 +765,21 @@ extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* conics,
     vox_partials_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(tile_cum, partials, N, n_isects, v_means3d,
                                                                      v_conics, v_opacities, beta ? v_betas : nullptr);
     return check_launch("voxelize_fused_vjp_reduce");
+}
+
+extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const float* conics, const float* opacities,
+                                      const float* betas, const int32_t* tile_offsets, const int32_t* flatten_ids,
+                                      uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+                                      uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                                      float* render, r2s_stream_t stream) {
+    if (int rc = check_vox_tile(tile_size, n_tiles_x, n_tiles_y, n_tiles_z, dim_x, dim_y, dim_z)) return rc;
+    R2S_REQUIRE(tile_size == kMomT, "the fused voxel forward takes %u^3 tiles (got %u)", kMomT, tile_size);
+    const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
+    R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles");
+    if (n_tiles == 0) return R2S_OK;
+    auto A = make_vox_args<float>(means3d, conics, opacities, betas, nullptr, tile_offsets, flatten_ids, n_isects, N,
+                                  dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
+    auto k = betas ? vox_fwd_fused_kernel<true> : vox_fwd_fused_kernel<false>;
+    k<<<uint32_t(n_tiles), kVoxFwdWaves * kWave, 0, stream>>>(A, render);
+    return check_launch("voxelize_fused_fwd");
 }

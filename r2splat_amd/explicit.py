@@ -44,7 +44,8 @@ def _use_fused(means, beta, uncertainty) -> bool:
 
 
 def _use_fused_voxels(means) -> bool:
-    """float32 mean-adjoint voxel VJPs (8^3 tiles) run without atomics (fused.voxel_vjp)."""
+    """float32 voxel forwards (mean only) and mean-adjoint VJPs (8^3 tiles) run the
+    fused kernels (fused.voxel_render / fused.voxel_vjp)."""
     return BACKEND == "fused" and means.dtype == torch.float32 and VOXEL_TILE == 8
 
 
@@ -322,6 +323,8 @@ def _voxel_render(b, prims, shape):
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
     with stage("voxelize_fwd"):
+        if uncertainty is None and _use_fused_voxels(means):
+            return fused.voxel_render(b, densities, beta, (width, height, depth), T)
         if uncertainty is None:
             return getattr(ops, prefix + "voxelize_to_voxels_fwd")(*args, b["offsets"], b["flatten_ids"], width,
                                                                    height, depth, T)

@@ -218,3 +218,17 @@ def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8):
     if beta is not None:
         return v_mapped, v_conics, v_dens, v_beta
     return v_mapped, v_conics, v_dens
+
+
+def voxel_render(state: dict, densities, beta, dims, tile_size: int = 8):
+    """Volume [D, H, W] of a voxel binning state (float32, 8^3 tiles)."""
+    mapped = state["mapped"]
+    device = mapped.device
+    W, H, D = dims
+    T = int(tile_size)
+    out = torch.empty((D, H, W), dtype=torch.float32, device=device)
+    with torch.cuda.device(device):
+        call("r2s_voxelize_fused_fwd", ptr(mapped), ptr(state["conics"]), ptr(densities), ptr(beta),
+             ptr(state["offsets"]), ptr(state["flatten_ids"]), state["flatten_ids"].numel(), mapped.shape[0], W, H, D,
+             T, -(-W // T), -(-H // T), -(-D // T), ptr(out), be.stream_of(device))
+    return out

@@ -264,9 +264,9 @@ def test_fused_beta_large_exponent_centre_pixel(device):
 # --------------------------------------------------------------------------- voxels
 @pytest.mark.parametrize("use_beta", [False, True])
 def test_fused_voxel_vjp_matches_staged_path_and_is_reproducible(device, use_beta):
-    """float32 voxelize backward without atomics (one partial per intersection,
-    summed per primitive in a fixed order) against the staged per-intersection
-    kernel, on a volume whose edges cut tiles; bitwise reproducible."""
+    """float32 fused voxelize forward and backward (no atomics: one partial per
+    intersection, summed per primitive in a fixed order) against the staged
+    operators, on a volume whose edges cut tiles; bitwise reproducible."""
     sc = workloads.scene(20000, 1, 8, 8, scale_range=(0.005, 0.05), seed=2, device=device,
                          beta_range=(0.5, 4.0) if use_beta else None)
     shape = (90, 77, 64)
@@ -284,13 +284,14 @@ def test_fused_voxel_vjp_matches_staged_path_and_is_reproducible(device, use_bet
 
     vol_a, a = run("fused")
     vol_b, b = run("ops")
-    assert torch.equal(vol_a, vol_b)
+    assert O.relative_error(vol_a, vol_b) <= 1e-5
     names = ("means", "quats", "scales", "densities") + (("beta",) if use_beta else ())
     for name in names:
         x, y = getattr(a, name).double(), getattr(b, name).double()
         l2 = float((x - y).norm() / y.norm())
         assert l2 <= 1e-5, f"{name}: relative L2 {l2:.2e}"
         assert O.relative_error(x, y) <= 1e-4, f"{name}: {O.relative_error(x, y):.2e}"
-    _, a2 = run("fused")
+    vol_a2, a2 = run("fused")
+    assert torch.equal(vol_a, vol_a2)
     for name in names:
         assert torch.equal(getattr(a, name), getattr(a2, name)), name
