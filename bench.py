@@ -34,12 +34,16 @@ FP32_PEAK_TFLOPS = 157.3
 # v_sigma 2, v_xy 12, v_conic 11, v_opacity 3, v_mu 3;
 # RasterizeToPixelsBwd.cu:430-468).
 FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
-# FP32 operations the fused kernels issue per pair (FMA = 2, exp = 1): the
-# forward shares the column terms of a lane's pixels (dy 1, two FMAs 4, exp 1,
-# select+FMA 2, plus the per-record column terms amortised over 4 pixels,
-# ~1); the backward reduces each record to six per-row moments (dx 1, two
-# FMAs 4, exp 1, v*e 1, three moment updates 4, per-row terms ~0.5).
-EXECUTED_FLOP_PER_PAIR = {"rasterize_fwd": 9.0, "rasterize_bwd": 11.5}
+# Vector (VALU) instructions one launch of each fused raster kernel issues on
+# this workload (PMC SQ_INSTS_VALU, summed over the device, from
+# `rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU ... --kernel-include-regex fused_raster`
+# on `bench.py --steps 2 --warmup 1 --plain-steps 0 --no-cpu-baseline --no-voxel
+# --no-graph`; profiles/raster_100k_64x256_occupancy_valu.md). The kernels
+# evaluate only the rows (backward) and row groups (forward) each record can
+# reach, so they issue far fewer operations than pairs x FLOP_PER_PAIR.
+VALU_INSTS = {"rasterize_fwd": 9.527e8, "rasterize_bwd": 8.544e8}
+SIMDS = 256 * 4          # compute units x SIMDs
+CLOCK_HZ = 2.4e9         # peak engine clock
 # HBM bytes per launch of the raster kernels on this workload, from two
 # rocprofv3 --pmc runs of `bench.py --steps 2 --warmup 1 --plain-steps 0
 # --no-cpu-baseline --no-voxel --no-graph --copy-calibration` (FETCH_SIZE,
@@ -47,7 +51,7 @@ EXECUTED_FLOP_PER_PAIR = {"rasterize_fwd": 9.0, "rasterize_bwd": 11.5}
 # plain 256 MiB device copy moves to what the same counter reads for it
 # (FETCH 131,089 KiB -> x2.0; WRITE 262,144 KiB -> x1.0). Summary and
 # commands: profiles/raster_100k_64x256_hbm_traffic.md.
-PMC_KIB = {"rasterize_fwd": (433_808, 16_446), "rasterize_bwd": (445_886, 860_511)}
+PMC_KIB = {"rasterize_fwd": (434_143, 16_462), "rasterize_bwd": (450_231, 861_137)}
 PMC_SCALE = (262_144 / 131_089, 262_144 / 262_144)
 
 
@@ -150,7 +154,7 @@ def main():
     dominant = max(kernel_stages, key=lambda k: kernel_stages[k][0])
     dom_ms = kernel_stages[dominant][0]
     achieved = pairs * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
-    executed = pairs * EXECUTED_FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
+    issue = VALU_INSTS[dominant] / (dom_ms * 1e-3 * CLOCK_HZ * SIMDS)
     fetch, write = PMC_KIB[dominant]
     traffic = int((fetch * PMC_SCALE[0] + write * PMC_SCALE[1]) * 1024)
     roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
@@ -158,9 +162,10 @@ def main():
                 "traffic_gbs": round(traffic / (dom_ms * 1e-3) / 1e9, 1),
                 "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
                 "avg_launch_ms": round(dom_ms, 4),
-                "executed": {"flop_per_pair": EXECUTED_FLOP_PER_PAIR[dominant], "achieved": round(executed, 3),
-                             "frac": round(executed / FP32_PEAK_TFLOPS, 4),
-                             "note": "FP32 operations the kernel itself issues per pair (FMA = 2)"}}
+                "issued": {"valu_insts": VALU_INSTS[dominant], "per_simd_cycle": round(issue, 4),
+                           "note": "vector instructions the kernel issues per SIMD per cycle (measured count / "
+                                   "(launch time x 2.4 GHz x 1,024 SIMDs)); the kernel evaluates only the rows "
+                                   "each record reaches, so pairs x FLOP/pair overstates its work"}}
 
     voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
 
