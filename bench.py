@@ -226,7 +226,7 @@ def batched_main(args):
     t_gen = time.perf_counter() - t0
     tuner = Autotuner(repeats=2)
     t0 = time.perf_counter()
-    batch = BatchedFit(jobs, adam=AdamConfig(lr=1e-2), n_streams=args.streams, tuner=tuner)
+    batch = BatchedFit(jobs, adam=AdamConfig(lr=1e-2, min_scale=1e-6), n_streams=args.streams, tuner=tuner)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t0
     batch.step(1)
@@ -255,9 +255,12 @@ def batched_main(args):
                 opt.zero_grad(set_to_none=True)
                 loss.backward()
                 opt.step()
+                with torch.no_grad():  # the same domain clamp as the batched step
+                    params[2].clamp_(min=1e-6)
+                    params[3].clamp_(min=0.0)
 
-            # raw Adam at lr 1e-2 can drive scales negative, which the explicit API rejects: every timed
-            # step starts from the job's initial parameters (restored outside the timed region)
+            # every timed step starts from the job's initial parameters (restored outside the timed
+            # region), so each job's plain step sees the same inputs as its first batched step
             initial = [p.detach().clone() for p in params]
             for k in range(args.plain_steps + 1):
                 with torch.no_grad():
@@ -302,7 +305,7 @@ def batched_main(args):
         "plain_path": None if plain_ms is None else {
             "ms_per_batched_step": round(plain_ms, 2), "speedup_of_value": round(plain_ms / ms, 3),
             "description": "each job stepped alone through the autograd wrapper on the plain operator path "
-                           "with torch.optim.Adam"},
+                           "with torch.optim.Adam and the same clamp"},
     }
     print(json.dumps(result), flush=True)
 
