@@ -375,8 +375,10 @@ int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const 
                          const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
                          uint32_t tile_size, uint32_t waves, int beta, const float* v_image, float* isect_grads,
                          r2s_stream_t stream);
-/* workspace: r2s_fused_param_workspace_bytes(C, N) bytes (per-camera
- * contributions, summed over cameras in order in a second pass). */
+/* The parameter VJP sums each primitive's cameras inside one kernel in a
+ * fixed order and needs no workspace: r2s_fused_param_workspace_bytes returns
+ * 0 and the workspace arguments may be NULL / 0 (kept for callers that pass
+ * a buffer). */
 size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                         const float* densities, const float* betas, const float* beta_ratio,

@@ -646,60 +646,67 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
 }
 
 // ---------------------------------------------------------------------------
-// Parameter VJP in two deterministic passes.
-// Pass 1, one thread per (camera, primitive): the record's intersection
-// partials [cum[v], cum[v+1]) summed in emission order -> conic/mu VJP
-// (double) -> projection VJP, giving this camera's contribution to v_mean (3),
-// v_covariance (6), v_density (1) and v_beta (1), stored field-major
-// ws[f][c][n].
-// Pass 2, one thread per primitive: the contributions summed over cameras in
-// order -> parameter-transform VJP. A partial is (v_mx, v_my, v_a, v_b, v_c,
-// v_w[, v_b2d]) with w = density * mu [* ratio(beta)]; for Beta primitives
-// v_beta = v_b2d + v_mus mus dlog ratio/dbeta and the conic/mu VJP receives
-// v_mu = v_mus ratio (r2splat/explicit.py:227-231).
+// Parameter VJP, one pass. A workgroup takes 64 primitives (one per lane) and
+// kVjpWaves camera phases (wave k takes cameras k, k + kVjpWaves, ...): per
+// (camera, primitive) the record's intersection partials [cum[v], cum[v+1])
+// are summed in emission order, then the conic/mu VJP (double) and the
+// projection VJP give that camera's contribution to v_mean (3),
+// v_covariance (6), v_density (1) and v_beta (1). Each thread sums its
+// cameras in order, the waves' sums are added in LDS in wave order, and the
+// parameter-transform VJP finishes each primitive: a fixed order throughout.
+// A partial is (v_mx, v_my, v_a, v_b, v_c, v_w[, v_b2d]) with w = density *
+// mu [* ratio(beta)]; for Beta primitives v_beta = v_b2d + v_mus mus dlog
+// ratio/dbeta and the conic/mu VJP receives v_mu = v_mus ratio
+// (r2splat/explicit.py:227-231).
 // ---------------------------------------------------------------------------
 constexpr int kParamFields = 11;
+constexpr int kVjpWaves = 16;
 
 template <bool BETA>
-__global__ void __launch_bounds__(256)
-fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict__ quats,
-                        const float* __restrict__ scales, const float* __restrict__ dens,
-                        const float* __restrict__ ratio, const float* __restrict__ log_grad,
-                        const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W, uint32_t H,
-                        float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
-                        const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
-                        float* __restrict__ ws) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = blockIdx.y;
-    if (n >= N) return;
-    const size_t v = size_t(c) * N + n;
-    const uint32_t n_live = live(d_n, capacity);
-    const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f;
-    for (uint32_t j = lo; j < hi; ++j) {
-        const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
-        const float4 a = gp[0];
-        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
-        if constexpr (BETA) {
-            const float4 b = gp[1];
-            g4 += b.x; g5 += b.y; g6 += b.z;
-        } else {
-            const float2 b = *reinterpret_cast<const float2*>(gp + 1);
-            g4 += b.x; g5 += b.y;
-        }
-    }
-    float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (g0 != 0.f || g1 != 0.f || g2 != 0.f || g3 != 0.f || g4 != 0.f || g5 != 0.f || g6 != 0.f) {
-        float q[4], mag, norm, R[9], cov[6];
+__global__ void __launch_bounds__(kVjpWaves * kWave)
+fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict__ quats,
+                       const float* __restrict__ scales, const float* __restrict__ dens,
+                       const float* __restrict__ ratio, const float* __restrict__ log_grad,
+                       const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W, uint32_t H,
+                       float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
+                       const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
+                       float* __restrict__ v_means, float* __restrict__ v_quats, float* __restrict__ v_scales,
+                       float* __restrict__ v_dens, float* __restrict__ v_betas) {
+    __shared__ float part[kVjpWaves][kParamFields][kWave];
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint32_t n = blockIdx.x * kWave + lane;
+    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float q[4] = {0.f, 0.f, 0.f, 0.f}, mag = 0.f, norm = 0.f, R[9], cov[6], s[3] = {0.f, 0.f, 0.f};
+    if (n < N) {
+        const uint32_t n_live = live(d_n, capacity);
         unit_quat(quats + 4 * size_t(n), q, mag, norm);
         quat_to_rot(q, R);
-        const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
+        s[0] = scales[3 * size_t(n)];
+        s[1] = scales[3 * size_t(n) + 1];
+        s[2] = scales[3 * size_t(n) + 2];
         covariance_from(R, s, cov);
         const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
-        Camera<float> cam;
-        cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
-        ProjState<float> st;
-        if (project_state(cam, p, cov, float(W), float(H), st)) {
+        for (uint32_t c = wave; c < C; c += kVjpWaves) {
+            const size_t v = size_t(c) * N + n;
+            const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f;
+            for (uint32_t j = lo; j < hi; ++j) {
+                const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
+                const float4 a = gp[0];
+                g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
+                if constexpr (BETA) {
+                    const float4 b = gp[1];
+                    g4 += b.x; g5 += b.y; g6 += b.z;
+                } else {
+                    const float2 b = *reinterpret_cast<const float2*>(gp + 1);
+                    g4 += b.x; g5 += b.y;
+                }
+            }
+            if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f && g6 == 0.f) continue;
+            Camera<float> cam;
+            cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
+            ProjState<float> st;
+            if (!project_state(cam, p, cov, float(W), float(H), st)) continue;
             float c2[6];
             projected_cov(st, c2);
             double cd[6], P[3], mu;
@@ -710,6 +717,7 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
             const double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
             const double muf = double(float(mu));
             const double vP[3] = {double(g2), double(g3), double(g4)};
+            float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             double v_mu;
             if constexpr (BETA) {
                 const float mus = float(mu) * ratio[n];
@@ -728,30 +736,20 @@ fused_camera_vjp_kernel(const float* __restrict__ means, const float* __restrict
 #pragma unroll
             for (int k = 0; k < 6; ++k) vc[k] = float(gcov[k]);
             projection_vjp(cam, st, vm, vc, out, out + 3);
+#pragma unroll
+            for (int f = 0; f < kParamFields; ++f) acc[f] += out[f];
         }
     }
-    const size_t CN = size_t(C) * N;
 #pragma unroll
-    for (int f = 0; f < kParamFields; ++f) ws[f * CN + v] = out[f];
-}
-
-__global__ void __launch_bounds__(256)
-fused_param_reduce_kernel(const float* __restrict__ quats, const float* __restrict__ scales, uint32_t C,
-                          uint32_t N, const float* __restrict__ ws, float* __restrict__ v_means,
-                          float* __restrict__ v_quats, float* __restrict__ v_scales, float* __restrict__ v_dens,
-                          float* __restrict__ v_betas) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const size_t CN = size_t(C) * N;
-    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t c = 0; c < C; ++c) {
+    for (int f = 0; f < kParamFields; ++f) part[wave][f][lane] = acc[f];
+    __syncthreads();
+    if (wave != 0 || n >= N) return;
 #pragma unroll
-        for (int f = 0; f < kParamFields; ++f) acc[f] += ws[f * CN + size_t(c) * N + n];
+    for (int f = 0; f < kParamFields; ++f) {
+        float t = part[0][f][lane];
+        for (int w = 1; w < kVjpWaves; ++w) t += part[w][f][lane];
+        acc[f] = t;
     }
-    float q[4], mag, norm, R[9];
-    unit_quat(quats + 4 * size_t(n), q, mag, norm);
-    quat_to_rot(q, R);
-    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     float gq[4], vs[3];
     covariance_vjp(q, R, s, acc + 3, gq, vs);
     normalize_vjp(q, mag, norm, gq, v_quats + 4 * size_t(n));
@@ -980,9 +978,9 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     return check_launch("fused_raster_bwd");
 }
 
-extern "C" size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N) {
-    return size_t(kParamFields) * C * N * sizeof(float);
-}
+// The one-pass parameter VJP needs no workspace; the size query and the
+// workspace arguments stay for callers that pass one.
+extern "C" size_t r2s_fused_param_workspace_bytes(uint32_t, uint32_t) { return 0; }
 
 extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                                    const float* densities, const float* betas, const float* beta_ratio,
@@ -992,27 +990,23 @@ extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const
                                    const float* isect_grads, void* workspace, size_t workspace_bytes,
                                    float* v_means, float* v_quats, float* v_scales, float* v_densities,
                                    float* v_betas, r2s_stream_t stream) {
+    (void)workspace;
+    (void)workspace_bytes;
     if (N == 0) return R2S_OK;
     if (int rc = fused_check(C, N, 1, 1, capacity)) return rc;
-    R2S_REQUIRE(workspace_bytes >= r2s_fused_param_workspace_bytes(C, N), "param workspace too small: %zu < %zu",
-                workspace_bytes, r2s_fused_param_workspace_bytes(C, N));
     const bool beta = betas != nullptr;
     R2S_REQUIRE(!beta || (beta_ratio && beta_log_grad && v_betas), "Beta primitives need ratio, log-gradient and "
                 "v_betas buffers");
-    float* ws = static_cast<float*>(workspace);
-    const dim3 grid(ceil_div(N, 256), C);
+    const uint32_t blocks = ceil_div(N, uint32_t(kWave));
     if (beta)
-        fused_camera_vjp_kernel<true><<<grid, 256, 0, stream>>>(
+        fused_param_vjp_kernel<true><<<blocks, kVjpWaves * kWave, 0, stream>>>(
             means, quats, scales, densities, beta_ratio, beta_log_grad, viewmats, Ks, width, height, eps, C, N,
-            record_offsets, counters, capacity, isect_grads, ws);
+            record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales, v_densities, v_betas);
     else
-        fused_camera_vjp_kernel<false><<<grid, 256, 0, stream>>>(
+        fused_param_vjp_kernel<false><<<blocks, kVjpWaves * kWave, 0, stream>>>(
             means, quats, scales, densities, nullptr, nullptr, viewmats, Ks, width, height, eps, C, N,
-            record_offsets, counters, capacity, isect_grads, ws);
-    if (int rc = check_launch("fused_camera_vjp")) return rc;
-    fused_param_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(quats, scales, C, N, ws, v_means, v_quats,
-                                                                    v_scales, v_densities, beta ? v_betas : nullptr);
-    return check_launch("fused_param_reduce");
+            record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales, v_densities, nullptr);
+    return check_launch("fused_param_vjp");
 }
 
 extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
