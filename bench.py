@@ -67,9 +67,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
     p.add_argument("--no-graph", action="store_true", help="time the eager step instead of its hipGraph replay")
-    p.add_argument("--workload", default="config1", choices=["config1", "batched", "serving"],
+    p.add_argument("--workload", default="config1", choices=["config1", "config2", "batched", "serving"],
                    help="config1: the metric workload; batched: 32 uneven jobs with Adam in one graph; "
-                        "serving: Poisson-arriving fit requests through the continuous-batching server")
+                        "serving: Poisson-arriving fit requests through the continuous-batching server; "
+                        "config2: Beta primitives with uncertainty (mean, variance) and a 256^3 volume with TV")
     p.add_argument("--rate", type=float, default=4.0, help="serving: requests per second")
     p.add_argument("--duration", type=float, default=120.0, help="serving: arrival window in seconds")
     p.add_argument("--serve-steps", type=int, default=200, help="serving: Adam iterations per request")
@@ -96,6 +97,8 @@ def main():
         return batched_main(args)
     if args.workload == "serving":
         return serving_main(args)
+    if args.workload == "config2":
+        return config2_main(args)
     device = torch.device("cuda:0")
     if args.copy_calibration:
         src = torch.ones((64 << 20,), dtype=torch.float32, device=device)
@@ -306,6 +309,80 @@ def batched_main(args):
             "ms_per_batched_step": round(plain_ms, 2), "speedup_of_value": round(plain_ms / ms, 3),
             "description": "each job stepped alone through the autograd wrapper on the plain operator path "
                            "with torch.optim.Adam and the same clamp"},
+    }
+    print(json.dumps(result), flush=True)
+
+
+def config2_main(args):
+    """N = 500,000 Beta primitives with uncertainty, 256 views of 512^2:
+    rasterize_fwd returning (mean, variance) + rasterize_bwd with adjoints
+    N(0, 1e-3) / N(0, 1e-4), for detach_variance False and True; and the same
+    primitives voxelized to 256^3 with the isotropic TV loss (weight 1e-3),
+    forward and backward. Steps are timed eagerly (host-side sizing included)."""
+    import torch
+
+    import r2splat_amd
+    from r2splat_amd import profiling, workloads
+    from r2splat_amd import hip as ops
+
+    device = torch.device("cuda:0")
+    sc = workloads.config2(device=device)
+    g = torch.Generator(device=device).manual_seed(5)
+    v = torch.randn((256, 512, 512), generator=g, device=device) * 1e-3
+    vv = torch.randn((256, 512, 512), generator=g, device=device) * 1e-4
+    one = torch.ones((1,), device=device)
+
+    def raster_step(detach):
+        out, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta,
+                                             uncertainty=sc.uncertainty, detach_variance=detach)
+        return r2splat_amd.rasterize_bwd(ctx, v, vv)
+
+    def voxel_step():
+        vol, ctx = r2splat_amd.voxelize_fwd(*sc.params(), (256, 256, 256), volume_extent=(1.0, 1.0, 1.0),
+                                            beta=sc.beta)
+        volume = vol[None]
+        ops.volume_loss_iso_fwd(volume, 1e-3)
+        gvol = ops.volume_loss_iso_bwd(volume, one, 1e-3)
+        return r2splat_amd.voxelize_bwd(ctx, gvol[0].contiguous())
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        profiling.reset()
+        profiling.enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        stages = {k: round(v_[0], 4) for k, v_ in profiling.collect().items()}
+        profiling.enable(False)
+        return ms, stages
+
+    steps = max(1, min(args.steps, 10))
+    r_ms, r_stages = timed(lambda: raster_step(False), steps, 1)
+    d_ms, _ = timed(lambda: raster_step(True), steps, 1)
+    v_ms, v_stages = timed(voxel_step, steps, 1)
+    result = {
+        "metric": "Beta+uncertainty rasterize fwd+bwd iters/s (500k Beta primitives, 256x512^2 views)",
+        "value": round(1e3 / r_ms, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": round(r_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Beta primitives with uncertainty, orbit cameras, Gaussian adjoints)",
+        "config": {"workload": "config2: N=500000 Beta primitives (beta log-uniform [0.5,4], uncertainty U[0,0.1], "
+                               "scales [0.003,0.03]), C=256 views of 512x512, (mean, variance) fwd+bwd; voxels 256^3 "
+                               "with isotropic TV 1e-3"},
+        "stages_ms": r_stages,
+        "detach_variance_ms_per_step": round(d_ms, 3),
+        "voxel_tv_256": {"ms_per_step": round(v_ms, 3), "stages_ms": v_stages},
     }
     print(json.dumps(result), flush=True)
 
