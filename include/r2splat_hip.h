@@ -329,32 +329,39 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * exponent beta + 1/2), beta_ratio [N] = B(1/2, beta + 1) / sqrt(2 pi)
  * (the line-integral factor, mus = mu * ratio; explicit.py:166-170) and
  * beta_log_grad [N] = d log ratio / d beta (explicit.py:226-230), both as
- * r2splat/_beta.py computes them. The same `beta` flag (0 Gaussian, 1 Beta)
- * selects the record layout in r2s_fused_scratch_bytes / r2s_fused_record_bytes /
- * r2s_fused_render / r2s_fused_raster_vjp and must match the forward. */
+ * r2splat/_beta.py computes them. uncertainties [N] (may be NULL) adds the
+ * variance output var = sum q alpha^2 (explicit.py:176-183 with uncertainty;
+ * RasterizeToPixelsFwd.cu mean_var kernels) and its VJP: the pixel adjoint
+ * becomes v_image + 2 q alpha v_image_var unless `detach` (the reference's
+ * detach_variance), and v_uncertainties = sum v_image_var alpha^2.
+ * `kind` (bit 0 Beta, bit 1 uncertainty) selects the record layout in
+ * r2s_fused_scratch_bytes / r2s_fused_record_bytes / r2s_fused_render /
+ * r2s_fused_raster_vjp and must match the forward. */
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
-                               uint32_t tile_size, uint32_t capacity, int beta);
-size_t r2s_fused_record_bytes(int beta);
+                               uint32_t tile_size, uint32_t capacity, int kind);
+size_t r2s_fused_record_bytes(int kind);
 
 int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* betas, const float* beta_ratio,
-                      const float* viewmats, const float* Ks, uint32_t C,
+                      const float* uncertainties, const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
                       void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
-                      uint32_t* counters, float* image, r2s_stream_t stream);
+                      uint32_t* counters, float* image, float* image_var, r2s_stream_t stream);
 
 /* Outputs: v_means [N,3], v_quats [N,4], v_scales [N,3], v_densities [N],
- * v_betas [N] (Beta only; overwritten; camera sums in a fixed order). */
+ * v_betas [N] (Beta only), v_uncertainties [N] (with uncertainties; all
+ * overwritten; camera sums in a fixed order). v_image_var may be NULL. */
 int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                        const float* densities, const float* betas, const float* beta_ratio,
-                       const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
-                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
-                       uint32_t capacity, const void* packed, const int32_t* tile_offsets,
-                       const uint32_t* tile_order, const uint32_t* record_offsets,
-                       const uint32_t* counters, float* isect_grads, void* workspace,
-                       size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
-                       float* v_scales, float* v_densities, float* v_betas, r2s_stream_t stream);
+                       const float* beta_log_grad, const float* uncertainties, const float* viewmats,
+                       const float* Ks, uint32_t C, uint32_t N, uint32_t width, uint32_t height,
+                       uint32_t tile_size, float eps, uint32_t capacity, const void* packed,
+                       const int32_t* tile_offsets, const uint32_t* tile_order, const uint32_t* record_offsets,
+                       const uint32_t* counters, float* isect_grads, void* workspace, size_t workspace_bytes,
+                       const float* v_image, const float* v_image_var, int detach, float* v_means,
+                       float* v_quats, float* v_scales, float* v_densities, float* v_betas,
+                       float* v_uncertainties, r2s_stream_t stream);
 
 /* The same two calls split into stages: forward = prepare (projection,
  * binning, sort, offsets, record packing) + render (rasterization);
@@ -363,18 +370,20 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
  * VJPs, summed over cameras). */
 int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* betas, const float* beta_ratio,
-                      const float* viewmats, const float* Ks, uint32_t C,
+                      const float* uncertainties, const float* viewmats, const float* Ks, uint32_t C,
                       uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
                       float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
                       void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
                       uint32_t* counters, r2s_stream_t stream);
+/* image_var: [C, H, W] (required iff kind has the uncertainty bit). */
 int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
                      const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                     uint32_t tile_size, uint32_t waves, int beta, float* image, r2s_stream_t stream);
+                     uint32_t tile_size, uint32_t waves, int kind, float* image, float* image_var,
+                     r2s_stream_t stream);
 int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
                          const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
-                         uint32_t tile_size, uint32_t waves, int beta, const float* v_image, float* isect_grads,
-                         r2s_stream_t stream);
+                         uint32_t tile_size, uint32_t waves, int kind, const float* v_image,
+                         const float* v_image_var, int detach, float* isect_grads, r2s_stream_t stream);
 /* The parameter VJP sums each primitive's cameras inside one kernel in a
  * fixed order and needs no workspace: r2s_fused_param_workspace_bytes returns
  * 0 and the workspace arguments may be NULL / 0 (kept for callers that pass
@@ -387,7 +396,7 @@ int r2s_fused_param_vjp(const float* means, const float* quats, const float* sca
                         const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
                         const float* isect_grads, void* workspace, size_t workspace_bytes,
                         float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                        float* v_betas, r2s_stream_t stream);
+                        float* v_betas, float* v_uncertainties, r2s_stream_t stream);
 
 /* Fused float32 voxel VJP (mean only, 8^3 tiles): the whole per-primitive
  * voxelize backward (VoxelizeToVoxelsBwd.cu per-Gaussian kernel, as called by

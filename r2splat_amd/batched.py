@@ -106,7 +106,7 @@ class Autotuner:
                 fwd[w] = self._time(lambda: fused.render(f, C, device, w))
                 bwd[w] = self._time(lambda: call(
                     "r2s_fused_raster_vjp", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters),
-                    f.capacity, C, width, height, tile, w, kind, ptr(v_image), ptr(grads), stream))
+                    f.capacity, C, width, height, tile, w, kind, ptr(v_image), None, 0, ptr(grads), stream))
                 timings[(tile, "render", w)] = fwd[w]
                 timings[(tile, "raster_vjp", w)] = bwd[w]
             timings[(tile, "prepare", 0)] = t_prep

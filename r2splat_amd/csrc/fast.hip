@@ -43,18 +43,21 @@ struct __align__(16) Splat {
 };
 static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
 
-// Beta record: d^2 = A dx^2 + B dx dy + C dy^2 (A = a, B = 2 b, C = c of the
-// conic), alpha = w (1 - d^2)^b2d with w = opacity mu ratio(beta), b2d = beta + 1/2.
+// Wide record, for Beta primitives and for primitives with an uncertainty:
+// Beta: d^2 = A dx^2 + B dx dy + C dy^2 (A = a, B = 2 b, C = c of the conic),
+// alpha = w (1 - d^2)^b2d with w = opacity mu ratio(beta), b2d = beta + 1/2;
+// Gaussian with uncertainty: the Splat fields (b2d unused). unc is the
+// primitive's uncertainty q (variance = sum q alpha^2).
 struct __align__(16) BetaSplat {
     float mx, my, A, B, C, w;
-    uint32_t qlim;  // bits(D) + 1, or 0
+    uint32_t qlim;  // Beta: bits(D) + 1; Gaussian: bits(S) + 1; or 0
     int32_t g;
-    float b2d, pad0, pad1, pad2;
+    float b2d, unc, pad1, pad2;
 };
-static_assert(sizeof(BetaSplat) == 48, "beta record must be 48 bytes");
+static_assert(sizeof(BetaSplat) == 48, "wide record must be 48 bytes");
 
-template <bool BETA>
-using Rec = typename std::conditional<BETA, BetaSplat, Splat>::type;
+template <bool BETA, bool VAR = false>
+using Rec = typename std::conditional<BETA || VAR, BetaSplat, Splat>::type;
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
@@ -113,14 +116,15 @@ __device__ __forceinline__ void row_span(const R& s, float y_base, int T, int& l
 // ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
 // ---------------------------------------------------------------------------
-template <bool BETA>
+template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(256)
 fused_project_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                      const float* __restrict__ scales, const float* __restrict__ dens,
                      const float* __restrict__ betas, const float* __restrict__ ratio,
-                     const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t W,
-                     uint32_t H, float eps, float clip, uint32_t tile_size, uint32_t ntx, uint32_t nty,
-                     uint32_t N, Rec<BETA>* __restrict__ recs, uint2* __restrict__ rects,
+                     const float* __restrict__ unc, const float* __restrict__ viewmats,
+                     const float* __restrict__ Ks, uint32_t W, uint32_t H, float eps, float clip,
+                     uint32_t tile_size, uint32_t ntx, uint32_t nty, uint32_t N,
+                     Rec<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects,
                      uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = blockIdx.y;
@@ -134,8 +138,9 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     covariance_from(R, s, cov);
     const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
-    Rec<BETA> rec = {};
+    Rec<BETA, VAR> rec = {};
     rec.g = int32_t(idx);
+    if constexpr (VAR) rec.unc = unc[n];
     uint2 rect = make_uint2(0, 0);
     uint32_t count = 0;
     ProjState<float> st;
@@ -415,9 +420,9 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
 // Adds one record to a lane's PX pixels. Bit k of the wave-uniform `rows` is
 // set when row group k can hold contributing pixels; the others are skipped
 // by scalar branches (one bit test each; none when every group is set).
-template <int PX, typename R>
+template <int PX, bool BETA, bool VAR, typename R>
 __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, float px, const float (&py)[PX],
-                                                 float (&acc)[PX]) {
+                                                 float (&acc)[PX], float (&acc_var)[PX]) {
     constexpr uint32_t kAll = PX >= 32 ? 0xffffffffu : (1u << PX) - 1u;
     if (rows == 0u) return;
     const bool all = rows == kAll;
@@ -429,11 +434,18 @@ __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, floa
         const float dy = py[k] - s.my;
         const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);  // Gaussian: sigma log2 e; Beta: d^2
         float e;
-        if constexpr (std::is_same<R, BetaSplat>::value)
+        if constexpr (BETA)
             e = __builtin_amdgcn_exp2f(s.b2d * __log2f(1.f - q));
         else
             e = __builtin_amdgcn_exp2f(-q);
-        acc[k] = fmaf(contributes(q, s.qlim) ? e : 0.f, s.w, acc[k]);
+        const float ek = contributes(q, s.qlim) ? e : 0.f;
+        if constexpr (VAR) {
+            const float alpha = ek * s.w;  // variance = sum q alpha^2
+            acc[k] += alpha;
+            acc_var[k] = fmaf(s.unc * alpha, alpha, acc_var[k]);
+        } else {
+            acc[k] = fmaf(ek, s.w, acc[k]);
+        }
     }
 }
 
@@ -457,19 +469,19 @@ __device__ __forceinline__ R load_splat(const R* __restrict__ packed, int32_t i,
 template <typename R>
 __device__ __forceinline__ void settle(const R& s) {
     asm volatile("" ::"v"(s.mx), "v"(s.my), "v"(s.A), "v"(s.B), "v"(s.C), "v"(s.w), "v"(s.qlim), "v"(s.g));
-    if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d));
+    if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d), "v"(s.unc));
 }
 
-template <int T, int WAVES, bool BETA>
+template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
-                        uint32_t H, float* __restrict__ out) {
+                        uint32_t H, float* __restrict__ out, float* __restrict__ out_var) {
     constexpr int PX = T * T / kWave, STEP = kWave / T;
     static_assert(PX * kWave == T * T && STEP * T == kWave, "tile must be 8, 16 or 32");
-    __shared__ float part[WAVES][T * T];
-    using R = Rec<BETA>;
+    __shared__ float part[WAVES][T * T];  // the slices' partial images (mean, then variance)
+    using R = Rec<BETA, VAR>;
     __shared__ R stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
@@ -482,11 +494,12 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
     const uint32_t lane = threadIdx.x % kWave, col = lane % T, row0 = lane / T;
     const float px = float(tx * T + col) + 0.5f;
-    float py[PX], acc[PX];
+    float py[PX], acc[PX], acc_var[PX];
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         py[k] = float(ty * T + row0 + STEP * k) + 0.5f;
         acc[k] = 0.f;
+        acc_var[k] = 0.f;
     }
     R* slot = stage[wave];
     R next = load_splat(packed, lo + int32_t(lane), hi);
@@ -508,10 +521,10 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         R cur = slot[0];
         for (int32_t j = 1; j < n; ++j) {
             const R nxt = slot[j];
-            splat_accumulate<PX>(cur, __builtin_amdgcn_readlane(spans, j - 1), px, py, acc);
+            splat_accumulate<PX, BETA, VAR>(cur, __builtin_amdgcn_readlane(spans, j - 1), px, py, acc, acc_var);
             cur = nxt;
         }
-        splat_accumulate<PX>(cur, __builtin_amdgcn_readlane(spans, n - 1), px, py, acc);
+        splat_accumulate<PX, BETA, VAR>(cur, __builtin_amdgcn_readlane(spans, n - 1), px, py, acc, acc_var);
     }
 #pragma unroll
     for (int k = 0; k < PX; ++k) part[wave][(row0 + STEP * k) * T + col] = acc[k];
@@ -522,6 +535,19 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         for (int k = 0; k < WAVES; ++k) v += part[k][p];
         const uint32_t x = tx * T + p % T, y = ty * T + p / T;
         if (x < W && y < H) out[(size_t(cam) * H + y) * W + x] = v;
+    }
+    if constexpr (VAR) {
+        __syncthreads();  // the mean's partial images are consumed
+#pragma unroll
+        for (int k = 0; k < PX; ++k) part[wave][(row0 + STEP * k) * T + col] = acc_var[k];
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < uint32_t(T * T); p += WAVES * kWave) {
+            float v = 0.f;
+#pragma unroll
+            for (int k = 0; k < WAVES; ++k) v += part[k][p];
+            const uint32_t x = tx * T + p % T, y = ty * T + p / T;
+            if (x < W && y < H) out_var[(size_t(cam) * H + y) * W + x] = v;
+        }
     }
 }
 
@@ -542,14 +568,18 @@ fused_raster_fwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
 // Se = sum v h^b2d and Sl = sum v h^b2d log2 h, and (delta = mean - pixel)
 //   v_mean = w b2d (2a Mx + 2b My, 2b Mx + 2c My),
 //   v_conic = -w b2d (Mxx, 2 Mxy, Myy), v_w = Se, v_b2d = w ln2 Sl.
+// With an uncertainty q (variance = sum q alpha^2) the pixel adjoint becomes
+// v + 2 q alpha v_var (v alone when the variance is detached) and the partial
+// also carries v_q = sum v_var alpha^2 (its eighth slot).
 // ---------------------------------------------------------------------------
-template <int T, int WAVES, bool BETA>
+template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_bwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
-                        uint32_t H, const float* __restrict__ v_img, float* __restrict__ isect_grads) {
-    __shared__ __align__(16) float sv[T * T];
+                        uint32_t H, const float* __restrict__ v_img, const float* __restrict__ v_var, int detach,
+                        float* __restrict__ isect_grads) {
+    __shared__ __align__(16) float sv[VAR ? 2 : 1][T * T];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
@@ -561,12 +591,17 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
     const uint32_t lane = threadIdx.x % kWave;
     for (uint32_t p = threadIdx.x; p < uint32_t(T * T); p += WAVES * kWave) {
         const uint32_t x = tx * T + p % T, y = ty * T + p / T;
-        sv[p] = (x < W && y < H) ? v_img[(size_t(cam) * H + y) * W + x] : 0.f;
+        const bool in = x < W && y < H;
+        const size_t o = (size_t(cam) * H + y) * W + x;
+        sv[0][p] = in ? v_img[o] : 0.f;
+        if constexpr (VAR) sv[VAR ? 1 : 0][p] = (in && v_var) ? v_var[o] : 0.f;
     }
     __syncthreads();
+    // the variance adjoint reaches alpha (2 q alpha v_var) unless the variance is detached
+    const float var_scale = (VAR && !detach) ? 2.f : 0.f;
     const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f;
     constexpr int32_t kStride = WAVES * kWave;
-    using R = Rec<BETA>;
+    using R = Rec<BETA, VAR>;
     int32_t chunk = start + int32_t(wave) * kWave;
     R next = load_splat(packed, chunk + int32_t(lane), end);
     for (; chunk < end; chunk += kStride) {
@@ -576,9 +611,10 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         settle(s);
         next = load_splat(packed, chunk + kStride + int32_t(lane), end);  // next chunk, a chunk ahead
         const float dx0 = x0 - s.mx;
-        float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f;
-        float bm1 = 0.f;
+        float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f, vq = 0.f;
+        float bm1 = 0.f, kvar = 0.f;
         if constexpr (BETA) bm1 = s.b2d - 1.f;
+        if constexpr (VAR) kvar = var_scale * s.unc * s.w;  // d variance / d (alpha / w) factor
         // only the rows this lane's record can reach (the others add exact zeros);
         // the wave runs as many rows as its tallest record needs
         int r_lo = 0, r_hi = 0;
@@ -586,8 +622,8 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         for (int r = r_lo; r < r_hi; ++r) {
             const float dy = y0 + float(r) - s.my;
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
-            const float4* row = reinterpret_cast<const float4*>(sv + r * T);
-            float v[T];
+            const float4* row = reinterpret_cast<const float4*>(sv[0] + r * T);
+            float v[T], vv[VAR ? T : 1];
 #pragma unroll
             for (int k = 0; k < T / 4; ++k) {
                 const float4 q4 = row[k];
@@ -595,6 +631,17 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
                 v[4 * k + 1] = q4.y;
                 v[4 * k + 2] = q4.z;
                 v[4 * k + 3] = q4.w;
+            }
+            if constexpr (VAR) {
+                const float4* rowv = reinterpret_cast<const float4*>(sv[VAR ? 1 : 0] + r * T);
+#pragma unroll
+                for (int k = 0; k < T / 4; ++k) {
+                    const float4 q4 = rowv[k];
+                    vv[4 * k] = q4.x;
+                    vv[4 * k + 1] = q4.y;
+                    vv[4 * k + 2] = q4.z;
+                    vv[4 * k + 3] = q4.w;
+                }
             }
             float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
@@ -607,13 +654,29 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
                     // selected away (not multiplied by a zero weight)
                     const bool in = contributes(q, s.qlim);
                     const float h = 1.f - q, lg = __log2f(h);
-                    g = in ? v[j] * __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;
-                    const float ve = g * h;
+                    const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(b2d - 1)
+                    float ve;
+                    if constexpr (VAR) {
+                        const float he = hb * h;  // h^b2d = alpha / w
+                        const float vj = fmaf(kvar * he, vv[j], v[j]);
+                        g = vj * hb;
+                        ve = vj * he;
+                        vq = fmaf(vv[j] * he, he, vq);
+                    } else {
+                        g = v[j] * hb;
+                        ve = g * h;
+                    }
                     se += ve;
                     sl = fmaf(ve, in ? lg : 0.f, sl);
                 } else {
-                    const float e = __builtin_amdgcn_exp2f(-q);
-                    g = contributes(q, s.qlim) ? e * v[j] : 0.f;
+                    const float e = contributes(q, s.qlim) ? __builtin_amdgcn_exp2f(-q) : 0.f;
+                    if constexpr (VAR) {
+                        const float vj = fmaf(kvar * e, vv[j], v[j]);
+                        g = vj * e;
+                        vq = fmaf(vv[j] * e, e, vq);
+                    } else {
+                        g = e * v[j];
+                    }
                 }
                 r0 += g;
                 const float gx = g * dx;
@@ -629,17 +692,18 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
         }
         if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
             float4* gp = reinterpret_cast<float4*>(isect_grads + kPartialStride * size_t(s.g));
+            const float wq = s.w * s.w * vq;  // sum v_var alpha^2 (the uncertainty's gradient)
             if constexpr (BETA) {
                 const float a = s.A, b = 0.5f * s.B, c = s.C, k = s.w * s.b2d;
                 gp[0] = make_float4(k * (2.f * a * mx + 2.f * b * my), k * (2.f * b * mx + 2.f * c * my), -k * mxx,
                                     -2.f * k * mxy);
-                gp[1] = make_float4(-k * myy, se, s.w * 0.6931471805599453f * sl, 0.f);
+                gp[1] = make_float4(-k * myy, se, s.w * 0.6931471805599453f * sl, wq);
             } else {
                 // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
                 const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
                 gp[0] = make_float4(s.w * (a * mx + b * my), s.w * (b * mx + c * my), -0.5f * s.w * mxx,
                                     -s.w * mxy);
-                gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, 0.f);
+                gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, wq);
             }
         }
     }
@@ -651,7 +715,8 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
 // (camera, primitive) the record's intersection partials [cum[v], cum[v+1])
 // are summed in emission order, then the conic/mu VJP (double) and the
 // projection VJP give that camera's contribution to v_mean (3),
-// v_covariance (6), v_density (1) and v_beta (1). Each thread sums its
+// v_covariance (6), v_density (1), v_beta (1) and v_uncertainty (1, the
+// partials' eighth slot). Each thread sums its
 // cameras in order, the waves' sums are added in LDS in wave order, and the
 // parameter-transform VJP finishes each primitive: a fixed order throughout.
 // A partial is (v_mx, v_my, v_a, v_b, v_c, v_w[, v_b2d]) with w = density *
@@ -659,10 +724,10 @@ fused_raster_bwd_kernel(const Rec<BETA>* __restrict__ packed, const int32_t* __r
 // ratio/dbeta and the conic/mu VJP receives v_mu = v_mus ratio
 // (r2splat/explicit.py:227-231).
 // ---------------------------------------------------------------------------
-constexpr int kParamFields = 11;
+constexpr int kParamFields = 12;  // v_mean 3, v_covariance 6, v_density, v_beta, v_uncertainty
 constexpr int kVjpWaves = 16;
 
-template <bool BETA>
+template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kVjpWaves * kWave)
 fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict__ quats,
                        const float* __restrict__ scales, const float* __restrict__ dens,
@@ -671,11 +736,11 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
                        float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
                        const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
                        float* __restrict__ v_means, float* __restrict__ v_quats, float* __restrict__ v_scales,
-                       float* __restrict__ v_dens, float* __restrict__ v_betas) {
+                       float* __restrict__ v_dens, float* __restrict__ v_betas, float* __restrict__ v_unc) {
     __shared__ float part[kVjpWaves][kParamFields][kWave];
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const uint32_t n = blockIdx.x * kWave + lane;
-    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float q[4] = {0.f, 0.f, 0.f, 0.f}, mag = 0.f, norm = 0.f, R[9], cov[6], s[3] = {0.f, 0.f, 0.f};
     if (n < N) {
         const uint32_t n_live = live(d_n, capacity);
@@ -689,20 +754,23 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
         for (uint32_t c = wave; c < C; c += kVjpWaves) {
             const size_t v = size_t(c) * N + n;
             const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
-            float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f;
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f, g7 = 0.f;
             for (uint32_t j = lo; j < hi; ++j) {
                 const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
                 const float4 a = gp[0];
                 g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
-                if constexpr (BETA) {
+                if constexpr (BETA || VAR) {
                     const float4 b = gp[1];
-                    g4 += b.x; g5 += b.y; g6 += b.z;
+                    g4 += b.x; g5 += b.y; g6 += b.z; g7 += b.w;
                 } else {
                     const float2 b = *reinterpret_cast<const float2*>(gp + 1);
                     g4 += b.x; g5 += b.y;
                 }
             }
-            if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f && g6 == 0.f) continue;
+            if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f && g6 == 0.f &&
+                g7 == 0.f)
+                continue;
+            acc[11] += g7;  // the uncertainty enters the variance directly
             Camera<float> cam;
             cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
             ProjState<float> st;
@@ -717,7 +785,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             const double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
             const double muf = double(float(mu));
             const double vP[3] = {double(g2), double(g3), double(g4)};
-            float out[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float out[kParamFields - 1] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             double v_mu;
             if constexpr (BETA) {
                 const float mus = float(mu) * ratio[n];
@@ -737,7 +805,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             for (int k = 0; k < 6; ++k) vc[k] = float(gcov[k]);
             projection_vjp(cam, st, vm, vc, out, out + 3);
 #pragma unroll
-            for (int f = 0; f < kParamFields; ++f) acc[f] += out[f];
+            for (int f = 0; f < kParamFields - 1; ++f) acc[f] += out[f];
         }
     }
 #pragma unroll
@@ -760,6 +828,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
     }
     v_dens[n] = acc[9];
     if (v_betas) v_betas[n] = acc[10];
+    if (v_unc) v_unc[n] = acc[11];
 }
 
 // ---------------------------------------------------------------------------
@@ -772,13 +841,16 @@ struct FusedScratch {
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
-static size_t record_bytes(int beta) { return beta ? sizeof(BetaSplat) : sizeof(Splat); }
+// kind: bit 0 = Beta primitives, bit 1 = primitives with an uncertainty
+// (variance output); either selects the 48-byte record.
+constexpr int kKindBeta = 1, kKindVar = 2;
+static size_t record_bytes(int kind) { return (kind & (kKindBeta | kKindVar)) ? sizeof(BetaSplat) : sizeof(Splat); }
 
-static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity, int beta) {
+static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity, int kind) {
     FusedScratch L;
     const size_t CN = size_t(C) * N;
     size_t o = 0;
-    L.recs = o; o += al(CN * record_bytes(beta));
+    L.recs = o; o += al(CN * record_bytes(kind));
     L.rects = o; o += al(CN * sizeof(uint2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
     L.sort_bytes = sort_workspace_bytes(capacity);
@@ -804,15 +876,15 @@ static T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<cha
 using namespace r2s;
 
 extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
-                                          uint32_t tile_size, uint32_t capacity, int beta) {
+                                          uint32_t tile_size, uint32_t capacity, int kind) {
     if (tile_size == 0) return 0;
     const uint64_t n_tiles = uint64_t(C) * ((width + tile_size - 1) / tile_size) *
                              ((height + tile_size - 1) / tile_size);
     if (n_tiles >= (1ull << 32)) return 0;
-    return fused_scratch(C, N, uint32_t(n_tiles), capacity, beta).bytes;
+    return fused_scratch(C, N, uint32_t(n_tiles), capacity, kind).bytes;
 }
 
-extern "C" size_t r2s_fused_record_bytes(int beta) { return record_bytes(beta); }
+extern "C" size_t r2s_fused_record_bytes(int kind) { return record_bytes(kind); }
 
 static int fused_check(uint32_t C, uint32_t N, uint32_t ntx, uint32_t nty, uint32_t capacity) {
     R2S_REQUIRE(uint64_t(C) * N < (1ull << 31), "C*N exceeds the int32 index range");
@@ -830,14 +902,17 @@ static int check_tile(uint32_t tile_size, uint32_t waves) {
     return R2S_OK;
 }
 
-// Launches Launch<T, WAVES, BETA>::run for the runtime tile size, wave count and primitive kind.
-template <template <int, int, bool> class Launch, typename... Args>
-static void launch_tiled(uint32_t tile, uint32_t waves, int beta, Args&&... args) {
+// Launches Launch<T, WAVES, BETA, VAR>::run for the runtime tile size, wave count and kind.
+template <template <int, int, bool, bool> class Launch, typename... Args>
+static void launch_tiled(uint32_t tile, uint32_t waves, int kind, Args&&... args) {
     if (waves == 0) waves = kDefaultWaves;
-#define R2S_TW(T_, W_)                                                   \
-    if (tile == T_ && waves == W_) {                                     \
-        if (beta) return Launch<T_, W_, true>::run(args...);             \
-        return Launch<T_, W_, false>::run(args...);                      \
+    const bool beta = kind & kKindBeta, var = kind & kKindVar;
+#define R2S_TW(T_, W_)                                                           \
+    if (tile == T_ && waves == W_) {                                             \
+        if (beta && var) return Launch<T_, W_, true, true>::run(args...);        \
+        if (beta) return Launch<T_, W_, true, false>::run(args...);              \
+        if (var) return Launch<T_, W_, false, true>::run(args...);               \
+        return Launch<T_, W_, false, false>::run(args...);                       \
     }
     R2S_TW(8, 4) R2S_TW(8, 8) R2S_TW(8, 16)
     R2S_TW(16, 4) R2S_TW(16, 8) R2S_TW(16, 16)
@@ -845,49 +920,49 @@ static void launch_tiled(uint32_t tile, uint32_t waves, int beta, Args&&... args
 #undef R2S_TW
 }
 
-template <int T, int WAVES, bool BETA>
+template <int T, int WAVES, bool BETA, bool VAR>
 struct LaunchFwd {
     static void run(uint32_t n_tiles, hipStream_t stream, const void* packed, const int32_t* offsets,
                     const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
-                    uint32_t W, uint32_t H, float* image) {
-        fused_raster_fwd_kernel<T, WAVES, BETA><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            static_cast<const Rec<BETA>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W, H,
-            image);
+                    uint32_t W, uint32_t H, float* image, float* image_var) {
+        fused_raster_fwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            static_cast<const Rec<BETA, VAR>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W,
+            H, image, image_var);
     }
 };
 
-template <int T, int WAVES, bool BETA>
+template <int T, int WAVES, bool BETA, bool VAR>
 struct LaunchBwd {
     static void run(uint32_t n_tiles, hipStream_t stream, const void* packed, const int32_t* offsets,
                     const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
-                    uint32_t W, uint32_t H, const float* v_image, float* isect_grads) {
-        fused_raster_bwd_kernel<T, WAVES, BETA><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            static_cast<const Rec<BETA>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W, H,
-            v_image, isect_grads);
+                    uint32_t W, uint32_t H, const float* v_image, const float* v_image_var, int detach,
+                    float* isect_grads) {
+        fused_raster_bwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
+            static_cast<const Rec<BETA, VAR>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W,
+            H, v_image, v_image_var, detach, isect_grads);
     }
 };
 
-template <bool BETA>
+template <bool BETA, bool VAR>
 static int prepare_impl(const float* means, const float* quats, const float* scales, const float* densities,
-                        const float* betas, const float* beta_ratio, const float* viewmats, const float* Ks,
-                        uint32_t C, uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
-                        float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes, void* packed,
-                        int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters,
-                        hipStream_t stream) {
-    using R = Rec<BETA>;
+                        const float* betas, const float* beta_ratio, const float* uncertainties,
+                        const float* viewmats, const float* Ks, uint32_t C, uint32_t N, uint32_t width,
+                        uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
+                        void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
+                        uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters, hipStream_t stream) {
+    using R = Rec<BETA, VAR>;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
-    const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity, BETA);
+    const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity, (BETA ? kKindBeta : 0) | (VAR ? kKindVar : 0));
     R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
     uint32_t* cum = record_offsets;
     if (CN > 0) {
         dim3 grid(ceil_div(N, 256), C);
-        fused_project_kernel<BETA><<<grid, 256, 0, stream>>>(means, quats, scales, densities, betas, beta_ratio,
-                                                             viewmats, Ks, width, height, eps, radius_clip, tile_size,
-                                                             ntx, nty, N, at<R>(scratch, L.recs),
-                                                             at<uint2>(scratch, L.rects), cum);
+        fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
+            means, quats, scales, densities, betas, beta_ratio, uncertainties, viewmats, Ks, width, height, eps,
+            radius_clip, tile_size, ntx, nty, N, at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
@@ -934,47 +1009,51 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
 
 extern "C" int r2s_fused_prepare(const float* means, const float* quats, const float* scales,
                                  const float* densities, const float* betas, const float* beta_ratio,
-                                 const float* viewmats, const float* Ks, uint32_t C, uint32_t N, uint32_t width,
-                                 uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
-                                 void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
-                                 uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters,
-                                 r2s_stream_t stream) {
+                                 const float* uncertainties, const float* viewmats, const float* Ks, uint32_t C,
+                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                                 uint32_t* counters, r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, 0)) return rc;
     R2S_REQUIRE((betas == nullptr) == (beta_ratio == nullptr), "betas and beta_ratio go together");
-    if (betas)
-        return prepare_impl<true>(means, quats, scales, densities, betas, beta_ratio, viewmats, Ks, C, N, width,
-                                  height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed,
-                                  tile_offsets, tile_order, record_offsets, counters, stream);
-    return prepare_impl<false>(means, quats, scales, densities, nullptr, nullptr, viewmats, Ks, C, N, width, height,
-                               tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed, tile_offsets,
-                               tile_order, record_offsets, counters, stream);
+#define R2S_PREP(B_, V_)                                                                                         \
+    return prepare_impl<B_, V_>(means, quats, scales, densities, betas, beta_ratio, uncertainties, viewmats, Ks, C, \
+                                N, width, height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes,     \
+                                packed, tile_offsets, tile_order, record_offsets, counters, stream)
+    if (betas && uncertainties) R2S_PREP(true, true);
+    if (betas) R2S_PREP(true, false);
+    if (uncertainties) R2S_PREP(false, true);
+    R2S_PREP(false, false);
+#undef R2S_PREP
 }
 
 extern "C" int r2s_fused_render(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
                                 const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width,
-                                uint32_t height, uint32_t tile_size, uint32_t waves, int beta, float* image,
-                                r2s_stream_t stream) {
+                                uint32_t height, uint32_t tile_size, uint32_t waves, int kind, float* image,
+                                float* image_var, r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
+    R2S_REQUIRE(!(kind & kKindVar) || image_var, "a variance kind needs image_var");
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     const uint64_t n_tiles = uint64_t(C) * ntx * nty;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    launch_tiled<LaunchFwd>(tile_size, waves, beta, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
-                            counters, capacity, ntx, nty, width, height, image);
+    launch_tiled<LaunchFwd>(tile_size, waves, kind, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
+                            counters, capacity, ntx, nty, width, height, image, image_var);
     return check_launch("fused_raster_fwd");
 }
 
 extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const uint32_t* tile_order,
                                     const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width,
-                                    uint32_t height, uint32_t tile_size, uint32_t waves, int beta,
-                                    const float* v_image, float* isect_grads, r2s_stream_t stream) {
+                                    uint32_t height, uint32_t tile_size, uint32_t waves, int kind,
+                                    const float* v_image, const float* v_image_var, int detach, float* isect_grads,
+                                    r2s_stream_t stream) {
     if (int rc = check_tile(tile_size, waves)) return rc;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     const uint64_t n_tiles = uint64_t(C) * ntx * nty;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
     if (n_tiles == 0) return R2S_OK;
-    launch_tiled<LaunchBwd>(tile_size, waves, beta, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
-                            counters, capacity, ntx, nty, width, height, v_image, isect_grads);
+    launch_tiled<LaunchBwd>(tile_size, waves, kind, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
+                            counters, capacity, ntx, nty, width, height, v_image, v_image_var, detach, isect_grads);
     return check_launch("fused_raster_bwd");
 }
 
@@ -989,7 +1068,7 @@ extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const
                                    const uint32_t* record_offsets, const uint32_t* counters, uint32_t capacity,
                                    const float* isect_grads, void* workspace, size_t workspace_bytes,
                                    float* v_means, float* v_quats, float* v_scales, float* v_densities,
-                                   float* v_betas, r2s_stream_t stream) {
+                                   float* v_betas, float* v_uncertainties, r2s_stream_t stream) {
     (void)workspace;
     (void)workspace_bytes;
     if (N == 0) return R2S_OK;
@@ -998,45 +1077,52 @@ extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const
     R2S_REQUIRE(!beta || (beta_ratio && beta_log_grad && v_betas), "Beta primitives need ratio, log-gradient and "
                 "v_betas buffers");
     const uint32_t blocks = ceil_div(N, uint32_t(kWave));
-    if (beta)
-        fused_param_vjp_kernel<true><<<blocks, kVjpWaves * kWave, 0, stream>>>(
-            means, quats, scales, densities, beta_ratio, beta_log_grad, viewmats, Ks, width, height, eps, C, N,
-            record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales, v_densities, v_betas);
-    else
-        fused_param_vjp_kernel<false><<<blocks, kVjpWaves * kWave, 0, stream>>>(
-            means, quats, scales, densities, nullptr, nullptr, viewmats, Ks, width, height, eps, C, N,
-            record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales, v_densities, nullptr);
+#define R2S_VJP(B_, V_)                                                                                          \
+    fused_param_vjp_kernel<B_, V_><<<blocks, kVjpWaves * kWave, 0, stream>>>(                                   \
+        means, quats, scales, densities, B_ ? beta_ratio : nullptr, B_ ? beta_log_grad : nullptr, viewmats, Ks,  \
+        width, height, eps, C, N, record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,  \
+        v_densities, B_ ? v_betas : nullptr, v_uncertainties)
+    if (beta && v_uncertainties) R2S_VJP(true, true);
+    else if (beta) R2S_VJP(true, false);
+    else if (v_uncertainties) R2S_VJP(false, true);
+    else R2S_VJP(false, false);
+#undef R2S_VJP
     return check_launch("fused_param_vjp");
 }
 
 extern "C" int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                                  const float* densities, const float* betas, const float* beta_ratio,
-                                 const float* viewmats, const float* Ks, uint32_t C, uint32_t N, uint32_t width,
-                                 uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
-                                 void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
-                                 uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters, float* image,
-                                 r2s_stream_t stream) {
-    if (int rc = r2s_fused_prepare(means, quats, scales, densities, betas, beta_ratio, viewmats, Ks, C, N, width,
-                                   height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes, packed,
-                                   tile_offsets, tile_order, record_offsets, counters, stream))
+                                 const float* uncertainties, const float* viewmats, const float* Ks, uint32_t C,
+                                 uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
+                                 float radius_clip, uint32_t capacity, void* scratch, size_t scratch_bytes,
+                                 void* packed, int32_t* tile_offsets, uint32_t* tile_order, uint32_t* record_offsets,
+                                 uint32_t* counters, float* image, float* image_var, r2s_stream_t stream) {
+    if (int rc = r2s_fused_prepare(means, quats, scales, densities, betas, beta_ratio, uncertainties, viewmats, Ks, C,
+                                   N, width, height, tile_size, eps, radius_clip, capacity, scratch, scratch_bytes,
+                                   packed, tile_offsets, tile_order, record_offsets, counters, stream))
         return rc;
+    const int kind = (betas ? kKindBeta : 0) | (uncertainties ? kKindVar : 0);
     return r2s_fused_render(packed, tile_offsets, tile_order, counters, capacity, C, width, height, tile_size, 0,
-                            betas != nullptr, image, stream);
+                            kind, image, image_var, stream);
 }
 
 extern "C" int r2s_fused_backward(const float* means, const float* quats, const float* scales,
                                   const float* densities, const float* betas, const float* beta_ratio,
-                                  const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,
-                                  uint32_t N, uint32_t width, uint32_t height, uint32_t tile_size, float eps,
-                                  uint32_t capacity, const void* packed, const int32_t* tile_offsets,
-                                  const uint32_t* tile_order, const uint32_t* record_offsets,
-                                  const uint32_t* counters, float* isect_grads, void* workspace,
-                                  size_t workspace_bytes, const float* v_image, float* v_means, float* v_quats,
-                                  float* v_scales, float* v_densities, float* v_betas, r2s_stream_t stream) {
+                                  const float* beta_log_grad, const float* uncertainties, const float* viewmats,
+                                  const float* Ks, uint32_t C, uint32_t N, uint32_t width, uint32_t height,
+                                  uint32_t tile_size, float eps, uint32_t capacity, const void* packed,
+                                  const int32_t* tile_offsets, const uint32_t* tile_order,
+                                  const uint32_t* record_offsets, const uint32_t* counters, float* isect_grads,
+                                  void* workspace, size_t workspace_bytes, const float* v_image,
+                                  const float* v_image_var, int detach, float* v_means, float* v_quats,
+                                  float* v_scales, float* v_densities, float* v_betas, float* v_uncertainties,
+                                  r2s_stream_t stream) {
+    const int kind = (betas ? kKindBeta : 0) | (uncertainties ? kKindVar : 0);
     if (int rc = r2s_fused_raster_vjp(packed, tile_offsets, tile_order, counters, capacity, C, width, height,
-                                      tile_size, 0, betas != nullptr, v_image, isect_grads, stream))
+                                      tile_size, 0, kind, v_image, v_image_var, detach, isect_grads, stream))
         return rc;
     return r2s_fused_param_vjp(means, quats, scales, densities, betas, beta_ratio, beta_log_grad, viewmats, Ks, C, N,
                                width, height, eps, record_offsets, counters, capacity, isect_grads, workspace,
-                               workspace_bytes, v_means, v_quats, v_scales, v_densities, v_betas, stream);
+                               workspace_bytes, v_means, v_quats, v_scales, v_densities, v_betas,
+                               uncertainties ? v_uncertainties : nullptr, stream);
 }

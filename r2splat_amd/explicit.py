@@ -33,14 +33,14 @@ RASTER_TILE = 16
 VOXEL_TILE = 8
 _FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
 
-# "fused": float32 mean-only projections (Gaussian or Beta) run the fused
-# pipeline (r2splat_amd.fused); every other case, and "ops", runs one operator
-# per stage.
+# "fused": float32 projections (Gaussian or Beta, with or without
+# uncertainty) run the fused pipeline (r2splat_amd.fused); every other case,
+# and "ops", runs one operator per stage.
 BACKEND = os.environ.get("R2S_BACKEND", "fused")
 
 
-def _use_fused(means, beta, uncertainty) -> bool:
-    return BACKEND == "fused" and means.dtype == torch.float32 and uncertainty is None
+def _use_fused(means) -> bool:
+    return BACKEND == "fused" and means.dtype == torch.float32
 
 
 def _use_fused_voxels(means) -> bool:
@@ -139,9 +139,9 @@ def _raster_state(prims, viewmats, Ks, width, height, eps, radius_clip) -> dict:
     """Projection and tile binning: everything the rasterizer and its VJP read."""
     means, quats, scales, densities, beta, uncertainty = prims
     C = viewmats.shape[0]
-    if _use_fused(means, beta, uncertainty):
+    if _use_fused(means):
         return {"fused": fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,
-                                       radius_clip, beta=beta)}
+                                       radius_clip, beta=beta, uncertainty=uncertainty)}
     T = RASTER_TILE
     with stage("project"):
         covars, _ = ops.parameter_transform_fwd(quats, scales)
@@ -260,8 +260,13 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
         _, height, width = context.shape
         b.update(_raster_state(context.primitives, b["viewmats"], b["Ks"], width, height, *b.pop("recompute")))
     if "fused" in b:
-        grads = fused.backward(b["fused"], means, quats, scales, densities, b["viewmats"], b["Ks"], v_image)
-        return _consume(context, ParameterGradients(*grads))
+        grads = fused.backward(b["fused"], means, quats, scales, densities, b["viewmats"], b["Ks"], v_image,
+                               v_variance=v_variance, detach_variance=context.detach_variance)
+        v_means, v_quats, v_scales, v_densities = grads[:4]
+        v_beta = grads[4] if beta is not None else None
+        v_uncertainty = grads[-1] if uncertainty is not None else None
+        return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta,
+                                                    v_uncertainty))
     _, height, width = context.shape
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""

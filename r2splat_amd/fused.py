@@ -75,10 +75,20 @@ class FusedForward:
     beta: Optional[torch.Tensor] = None       # [N] (Beta primitives) or None (Gaussian)
     ratio: Optional[torch.Tensor] = None      # [N] B(1/2, beta + 1) / sqrt(2 pi)
     log_grad: Optional[torch.Tensor] = None   # [N] d log ratio / d beta
+    uncertainty: Optional[torch.Tensor] = None  # [N]: render also returns the variance
 
     @property
     def is_beta(self) -> int:
         return int(self.beta is not None)
+
+    @property
+    def kind(self) -> int:
+        """Record kind of the C ABI: bit 0 Beta, bit 1 uncertainty."""
+        return _kind(self.beta, self.uncertainty)
+
+
+def _kind(beta, uncertainty) -> int:
+    return int(beta is not None) | (int(uncertainty is not None) << 1)
 
 
 def _scratch(C, N, width, height, tile_size, capacity, beta, device):
@@ -92,7 +102,7 @@ def beta_factors(beta: torch.Tensor):
 
 
 def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=None, tile_size: int = TILE, beta=None, factors=None) -> FusedForward:
+            check=None, tile_size: int = TILE, beta=None, factors=None, uncertainty=None) -> FusedForward:
     """Projection, binning, sort and record packing (everything but the image). float32, one device.
 
     ``check`` (default: unless the stream is being captured into a graph)
@@ -100,12 +110,12 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     capacity of the last eager call with the same shapes is used and the flag
     is left on the device (``FusedForward.counters[1]``) for the caller.
     ``beta`` [N] selects Beta primitives; ``factors`` = beta_factors(beta)
-    may be passed in to reuse them."""
+    may be passed in to reuse them; ``uncertainty`` [N] adds the variance."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
-    kind = int(beta is not None)
+    kind = _kind(beta, uncertainty)
     ratio = log_grad = None
-    if kind:
+    if beta is not None:
         ratio, log_grad = factors if factors is not None else beta_factors(beta)
     key = (device, C, N, height, width, tile_size, kind)
     capturing = torch.cuda.is_current_stream_capturing()
@@ -129,7 +139,7 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
             packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
             with stage("project_bin"):
                 call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(beta), ptr(ratio),
-                     ptr(viewmats), ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
+                     ptr(uncertainty), ptr(viewmats), ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),
                      scratch.numel(), ptr(packed), ptr(offsets), ptr(tile_order), ptr(record_offsets), ptr(counters),
                      stream)
             del scratch
@@ -143,31 +153,37 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     global LAST_COUNTERS
     LAST_COUNTERS = counters
     return FusedForward(packed, offsets, tile_order, record_offsets, counters, cap, n_isects, width, height,
-                        float(eps), tile_size, beta, ratio, log_grad)
+                        float(eps), tile_size, beta, ratio, log_grad, uncertainty)
 
 
-def render(fwd: FusedForward, C: int, device, waves: int = 0) -> torch.Tensor:
-    """The image [C, H, W] of a prepared forward (`waves` per tile: 4, 8, 16; 0 = 8)."""
+def render(fwd: FusedForward, C: int, device, waves: int = 0):
+    """The image [C, H, W] of a prepared forward, or (image, variance) when it
+    carries uncertainties (`waves` per tile: 4, 8, 16; 0 = 8)."""
     image = torch.empty((C, fwd.height, fwd.width), dtype=torch.float32, device=device)
+    var = torch.empty_like(image) if fwd.uncertainty is not None else None
     with torch.cuda.device(device):
         with stage("rasterize_fwd"):
             call("r2s_fused_render", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
-                 fwd.capacity, C,
-                 fwd.width, fwd.height, fwd.tile_size, waves, fwd.is_beta, ptr(image), be.stream_of(device))
-    return image
+                 fwd.capacity, C, fwd.width, fwd.height, fwd.tile_size, waves, fwd.kind, ptr(image), ptr(var),
+                 be.stream_of(device))
+    return image if var is None else (image, var)
 
 
 def forward(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
-            check=None, tile_size: int = TILE, waves: int = 0, beta=None):
-    """Renders [C, H, W]; returns (image, FusedForward). float32, contiguous, one device."""
+            check=None, tile_size: int = TILE, waves: int = 0, beta=None, uncertainty=None):
+    """Renders [C, H, W] (with uncertainty: (image, variance)); returns
+    (output, FusedForward). float32, contiguous, one device."""
     fwd = prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip,
-                  capacity=capacity, check=check, tile_size=tile_size, beta=beta)
+                  capacity=capacity, check=check, tile_size=tile_size, beta=beta, uncertainty=uncertainty)
     return render(fwd, viewmats.shape[0], means.device, waves), fwd
 
 
-def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image, waves: int = 0):
-    """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities[, v_beta]);
-    v_beta only for Beta primitives."""
+def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v_image, waves: int = 0,
+             v_variance=None, detach_variance: bool = False):
+    """VJP of the fused forward -> (v_means, v_quats, v_scales, v_densities[, v_beta]
+    [, v_uncertainty]); v_beta for Beta primitives, v_uncertainty (and the
+    variance adjoint ``v_variance``, whose path to the mean parameters
+    ``detach_variance`` cuts) for forwards with uncertainties."""
     device = means.device
     C, N = viewmats.shape[0], means.shape[0]
     grad = torch.empty((fwd.capacity * 8,), dtype=torch.float32, device=device)
@@ -177,20 +193,25 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     v_scales = torch.empty_like(scales)
     v_dens = torch.empty_like(densities)
     v_beta = torch.empty_like(fwd.beta) if fwd.is_beta else None
+    v_unc = torch.empty_like(fwd.uncertainty) if fwd.uncertainty is not None else None
     with torch.cuda.device(device):
         stream = be.stream_of(device)
         with stage("rasterize_bwd"):
             call("r2s_fused_raster_vjp", ptr(fwd.packed), ptr(fwd.offsets), ptr(fwd.tile_order), ptr(fwd.counters),
                  fwd.capacity, C,
-                 fwd.width, fwd.height, fwd.tile_size, waves, fwd.is_beta, ptr(v_image), ptr(grad), stream)
+                 fwd.width, fwd.height, fwd.tile_size, waves, fwd.kind, ptr(v_image), ptr(v_variance),
+                 int(bool(detach_variance)), ptr(grad), stream)
         with stage("param_bwd"):
             call("r2s_fused_param_vjp", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(fwd.beta),
                  ptr(fwd.ratio), ptr(fwd.log_grad), ptr(viewmats), ptr(Ks), C, N, fwd.width, fwd.height, fwd.eps,
                  ptr(fwd.record_offsets), ptr(fwd.counters), fwd.capacity, ptr(grad), ptr(ws), ws.numel(),
-                 ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens), ptr(v_beta), stream)
+                 ptr(v_means), ptr(v_quats), ptr(v_scales), ptr(v_dens), ptr(v_beta), ptr(v_unc), stream)
+    out = (v_means, v_quats, v_scales, v_dens)
     if fwd.is_beta:
-        return v_means, v_quats, v_scales, v_dens, v_beta
-    return v_means, v_quats, v_scales, v_dens
+        out = out + (v_beta,)
+    if v_unc is not None:
+        out = out + (v_unc,)
+    return out
 
 
 def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8):

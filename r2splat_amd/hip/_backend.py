@@ -56,14 +56,16 @@ SIGNATURES = {
     "r2s_voxelize_to_voxels_bwd_per_gaussian": [I32, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, P, P, P, P, P, P, P],
     "r2s_fused_scratch_bytes": [U32, U32, U32, U32, U32, U32, I32],
     "r2s_fused_record_bytes": [I32],
-    "r2s_fused_forward": [P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P, P],
-    "r2s_fused_backward": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, U32, P, P, P, P, P, P, P, SZ, P, P,
-                           P, P, P, P, P],
-    "r2s_fused_prepare": [P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P],
-    "r2s_fused_render": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P],
-    "r2s_fused_raster_vjp": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P],
+    "r2s_fused_forward": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P, P,
+                          P],
+    "r2s_fused_backward": [P, P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, U32, P, P, P, P, P, P, P, SZ, P,
+                           P, I32, P, P, P, P, P, P, P],
+    "r2s_fused_prepare": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P],
+    "r2s_fused_render": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P],
+    "r2s_fused_raster_vjp": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, I32, P, P],
     "r2s_fused_param_workspace_bytes": [U32, U32],
-    "r2s_fused_param_vjp": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, F, P, P, U32, P, P, SZ, P, P, P, P, P, P],
+    "r2s_fused_param_vjp": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, F, P, P, U32, P, P, SZ, P, P, P, P, P, P,
+                            P],
     "r2s_voxelize_fused_fwd": [P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P],
     "r2s_voxel_partials_bytes": [U32],
     "r2s_voxelize_fused_vjp": [P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, SZ, P, P,

@@ -189,22 +189,23 @@ class GaussianFit:
             self.beta_ratio.copy_(beta_path_ratio(self.beta))
             self.beta_log_grad.copy_(beta_path_log_grad(self.beta))
         call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
-             ptr(self.beta), ptr(self.beta_ratio), ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size,
+             ptr(self.beta), ptr(self.beta_ratio), None, ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size,
              self.eps, self.radius_clip, self.capacity,
              ptr(ws.scratch), ws.scratch.numel(), ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.record_offsets), ptr(self.counters), s)
         call("r2s_fused_render", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
-             self.capacity, C, W, H, self.tile_size, self.fwd_waves, self.kind, ptr(self.image), s)
+             self.capacity, C, W, H, self.tile_size, self.fwd_waves, self.kind, ptr(self.image), None, s)
         call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
              ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
         call("r2s_fused_raster_vjp", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, self.kind, ptr(self.v_image),
+             None, 0,
              ptr(ws.isect_grads), s)
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.beta), ptr(self.beta_ratio), ptr(self.beta_log_grad), ptr(self.viewmats), ptr(self.Ks), C, N,
              W, H, self.eps, ptr(self.record_offsets), ptr(self.counters), self.capacity, ptr(ws.isect_grads),
              ptr(ws.param_ws), ws.param_ws.numel(), ptr(self.v_means), ptr(self.v_quats), ptr(self.v_scales),
-             ptr(self.v_densities), ptr(self.v_beta), s)
+             ptr(self.v_densities), ptr(self.v_beta), None, s)
         if self.adam is not None:
             a = self.adam
             call("r2s_adam_step", ptr(self.params), ptr(self.grads), ptr(self.exp_avg), ptr(self.exp_avg_sq),

@@ -2,12 +2,12 @@
 beta log-uniform [0.5, 4], uncertainty uniform [0, 0.1], 256 views of 512^2,
 about 274M intersections; voxels 256^3, isotropic TV weight 1e-3), through
 properties that hold at any size: the mean of the (mean, variance) path
-equals the fused mean-only path; the variance is non-negative and exactly
-linear in the uncertainty; with detach_variance the mean-parameter gradients
-are those of the mean alone and the uncertainty gradient is unchanged; the
-backward is linear in the adjoints; the fused and staged backward agree and
-match central differences; voxel mean/variance likewise; TV loss and gradient
-against the FP64 oracle on the full volume."""
+equals the mean-only path; the variance is non-negative and exactly linear in
+the uncertainty; with detach_variance the mean-parameter gradients are those
+of the mean alone and the uncertainty gradient is unchanged; the backward is
+linear in the adjoints; the fused and staged backward agree (mean-only and
+(mean, variance)) and match central differences; voxel mean/variance
+likewise; TV loss and gradient against the FP64 oracle on the full volume."""
 
 import pytest
 import torch
@@ -44,7 +44,7 @@ def _fwd(sc, unc, detach=False):
 def test_config2_mean_and_variance(scene):
     (mean, var), ctx = _fwd(scene, scene.uncertainty)
     fused_mean, _ = _fwd(scene, None)
-    assert explicit._use_fused(scene.means, scene.beta, None)
+    assert explicit._use_fused(scene.means) and "fused" in ctx.buffers
     assert O.relative_error(mean, fused_mean) <= 1e-5
     assert float(var.min()) >= 0.0
     (mean2, var2), _ = _fwd(scene, (2.0 * scene.uncertainty).contiguous())
@@ -63,7 +63,7 @@ def _ops(fn):
 
 
 def test_config2_backward_detach_and_linearity(scene, adjoints):
-    """Staged operator path: detach_variance semantics and linearity."""
+    """Fused (mean, variance) path: detach_variance semantics and linearity."""
     v, vv = adjoints
     _, ctx = _fwd(scene, scene.uncertainty, detach=False)
     joint = r2splat_amd.rasterize_bwd(ctx, v, vv)
@@ -74,7 +74,7 @@ def test_config2_backward_detach_and_linearity(scene, adjoints):
         _, c = _fwd(scene, None)
         return r2splat_amd.rasterize_bwd(c, v)
 
-    ref = _ops(mean_only)
+    ref = mean_only()
     for name in FIELDS:
         err = O.relative_error(getattr(det, name), getattr(ref, name))
         assert err <= 1e-5, f"detached {name}: {err:.2e}"
@@ -157,3 +157,26 @@ def test_config2_voxels_and_isotropic_tv(scene, device):
     assert O.relative_error(g, g_ref) <= 1e-4
     aniso, _, _ = ops.volume_loss_fwd(vol, 1e-3)
     assert abs(aniso.item() - 1e-3 * O.volume_tv(ref.detach()).item()) <= 5e-5 * aniso.item()
+
+
+def test_config2_fused_variance_backward_matches_ops(scene, adjoints):
+    """The fused (mean, variance) backward against the staged operator path
+    over all 274M intersections (relative L2; max-norm on the per-primitive
+    sums that cut-off flips barely move)."""
+    v, vv = adjoints
+    _, ctx = _fwd(scene, scene.uncertainty)
+    assert "fused" in ctx.buffers
+    fz = r2splat_amd.rasterize_bwd(ctx, v, vv)
+
+    def ops_bwd():
+        _, c = _fwd(scene, scene.uncertainty)
+        return r2splat_amd.rasterize_bwd(c, v, vv)
+
+    op = _ops(ops_bwd)
+    for name in FIELDS + ("uncertainty",):
+        a, b = getattr(fz, name).double(), getattr(op, name).double()
+        l2 = float((a - b).norm() / b.norm())
+        assert l2 <= 1e-3, f"{name}: relative L2 {l2:.2e}"
+    for name in ("densities", "uncertainty"):
+        assert O.relative_error(getattr(fz, name), getattr(op, name)) <= 1e-3, name
+    assert O.relative_error(fz.beta, op.beta) <= 1e-4
