@@ -2,9 +2,13 @@
 //
 // Forward:  fused projection (parameter transform + projection + double
 //           precision conic/mu + radii + tile count, one 32-byte splat record
-//           per (camera, primitive)) -> device scan -> record emission ->
-//           capacity-bounded stable radix sort -> tile offsets -> per-
-//           intersection record packing -> rasterization.
+//           per (camera, primitive)) -> device scan -> binning into packed
+//           camera-major tile lists -> rasterization. Binning places each
+//           (record, tile) pair directly (per-range bin counts, a scan, then
+//           placement) when a camera's (tile, height class) bins fit in a
+//           small LDS table, and otherwise emits (tile, record) pairs, sorts
+//           them with a capacity-bounded stable radix sort and packs the
+//           records per list.
 // Backward: rasterization VJP, one partial per intersection stored at the
 //           intersection's emission index (the emission order groups each
 //           (camera, primitive)'s intersections contiguously) -> one lane per
@@ -26,6 +30,7 @@
 // Beta primitives (d^2 = delta^T P delta, alpha = w (1 - d^2)^b2d kept for
 // 0 <= d^2 < 1 and |alpha| >= 1e-5) use the same single compare on d^2 with
 // the bound D = min(largest float below 1, 1 - (1e-5 / |w|)^(1 / b2d)).
+#include <algorithm>
 #include <type_traits>
 
 #include "binning.hpp"
@@ -113,6 +118,45 @@ __device__ __forceinline__ void row_span(const R& s, float y_base, int T, int& l
     hi = max(lo, int(fh));
 }
 
+// Tile rectangle and row reach of one projected record. The projection
+// writes the rectangle (uint2: x0 | x1 << 16, y0 | y1 << 16, tiles [x0, x1) x
+// [y0, y1), 0 when culled) and, for the direct-placement binning, the reach
+// (float2: the mean row and row_span's half-extent, -1 reaching no row and
+// +inf every row of a tile), so the binning never reads a record to classify
+// an intersection by height.
+struct SpanRect {
+    uint32_t xr, yr;
+    float my, ext;
+};
+
+__device__ __forceinline__ SpanRect load_span(const uint2* __restrict__ rects, const float2* __restrict__ reach,
+                                              size_t i) {
+    const uint2 r = rects[i];
+    const float2 e = reach[i];
+    return SpanRect{r.x, r.y, e.x, e.y};
+}
+
+// row_span's extent of a record as a value (-1: no row; +inf: whole tile).
+template <typename R>
+__device__ __forceinline__ float row_reach(const R& s) {
+    if (s.qlim == 0u) return -1.f;
+    const float S = __uint_as_float(s.qlim - 1u);
+    const float den = 4.f * s.A * s.C - s.B * s.B;
+    const float e2 = 4.f * s.A * S * __builtin_amdgcn_rcpf(den);
+    if (!(s.A > 0.f) || !(den > 0.f) || !(e2 < 1e30f)) return __builtin_inff();
+    return __builtin_amdgcn_sqrtf(e2) * 1.001f + 0.05f;
+}
+
+// Number of tile rows [lo, hi) of a record with mean row my and extent ext
+// that a T-row tile starting at y_base holds (row_span's result).
+__device__ __forceinline__ uint32_t rows_reached(float my, float ext, float y_base, uint32_t T) {
+    if (ext < 0.f) return 0u;
+    const float c = my - y_base - 0.5f;
+    const float fl = fminf(fmaxf(ceilf(c - ext), 0.f), float(T));
+    const float fh = fminf(fmaxf(floorf(c + ext) + 1.f, 0.f), float(T));
+    return fh > fl ? uint32_t(fh - fl) : 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
 // ---------------------------------------------------------------------------
@@ -124,7 +168,7 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
                      const float* __restrict__ unc, const float* __restrict__ viewmats,
                      const float* __restrict__ Ks, uint32_t W, uint32_t H, float eps, float clip,
                      uint32_t tile_size, uint32_t ntx, uint32_t nty, uint32_t N,
-                     Rec<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects,
+                     Rec<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects, float2* __restrict__ reach,
                      uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = blockIdx.y;
@@ -141,7 +185,8 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     Rec<BETA, VAR> rec = {};
     rec.g = int32_t(idx);
     if constexpr (VAR) rec.unc = unc[n];
-    uint2 rect = make_uint2(0, 0);
+    uint2 rect = make_uint2(0u, 0u);
+    float2 rr = make_float2(0.f, -1.f);
     uint32_t count = 0;
     ProjState<float> st;
     const float Wf = float(W), Hf = float(H);
@@ -180,10 +225,12 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
             tile_span(my, ry, tile_size, nty, y0, y1)) {
             count = (x1 - x0) * (y1 - y0);
             rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
+            rr = make_float2(my, row_reach(rec));
         }
     }
     recs[idx] = rec;
     rects[idx] = rect;
+    if (reach) reach[idx] = rr;
     counts[idx] = count;
 }
 
@@ -406,16 +453,174 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
 }
 
 // ---------------------------------------------------------------------------
+// Direct placement (the binning used when a camera's bins fit in a small LDS
+// table). Bin = (tile, height class) within a camera, class 0 the tallest
+// records; the packed lists are ordered (camera, tile, class, range, ...):
+// each camera-major tile list holds its records by decreasing height class
+// (the backward's waves take 64 records of similar height). One wave owns a
+// range of R consecutive primitives of one camera, one lane per record, and
+// walks the records' tile rectangles in step (pair k of every lane's
+// rectangle, row-major, at step k):
+//   count: per range, the number of intersections in each bin (LDS counters);
+//   scan:  per (camera, bin) an exclusive scan over the ranges, then one over
+//          (camera, bin): every (range, bin) gets its first packed slot;
+//   place: the same walk, each (record, tile) pair written straight to its
+//          slot; pairs of one step that share a bin are ranked by lane
+//          (ballots on the bin's bits), steps in order.
+// Everything is deterministic: counts do not depend on the order of the LDS
+// increments and the placement order is fixed.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBinClasses = 9;  // heights ceil(8 h / T) = 8 .. 1, then h = 0
+
+__device__ __forceinline__ uint32_t height_class(const SpanRect& r, uint32_t ty, uint32_t T) {
+    const uint32_t h = rows_reached(r.my, r.ext, float(ty * T), T);
+    return h == 0u ? kBinClasses - 1u : kBinClasses - 1u - (8u * h + T - 1u) / T;
+}
+
+// A lane's walk over its record's tile rectangle, row-major.
+struct RectWalk {
+    uint32_t x0, x1, tx, ty, cnt, row, cls;
+    __device__ __forceinline__ void start(const SpanRect& r, uint32_t ntx, uint32_t T) {
+        x0 = r.xr & 0xffffu;
+        x1 = r.xr >> 16;
+        const uint32_t y0 = r.yr & 0xffffu;
+        cnt = (x1 - x0) * ((r.yr >> 16) - y0);
+        tx = x0;
+        ty = y0;
+        row = ty * ntx;
+        cls = cnt ? height_class(r, ty, T) : 0u;
+    }
+    __device__ __forceinline__ uint32_t bin() const { return (row + tx) * kBinClasses + cls; }
+    __device__ __forceinline__ void advance(const SpanRect& r, uint32_t ntx, uint32_t T) {
+        if (++tx == x1) {
+            tx = x0;
+            ++ty;
+            row += ntx;
+            cls = height_class(r, ty, T);
+        }
+    }
+};
+
+// grid (n_ranges, C), one wave; dynamic LDS: n_bins counters.
+__global__ void __launch_bounds__(kWave)
+fused_bin_count_kernel(const uint2* __restrict__ rects, const float2* __restrict__ reach, uint32_t N,
+                       uint32_t range_len, uint32_t n_ranges, uint32_t ntx, uint32_t T, uint32_t n_bins,
+                       uint32_t* __restrict__ range_counts) {
+    extern __shared__ uint32_t hist[];
+    const uint32_t range = blockIdx.x, cam = blockIdx.y, lane = threadIdx.x;
+    for (uint32_t b = lane; b < n_bins; b += kWave) hist[b] = 0u;
+    __syncthreads();
+    const uint32_t n0 = range * range_len, n1 = min(N, n0 + range_len);
+    for (uint32_t n = n0 + lane; n < n1; n += kWave) {
+        const SpanRect sr = load_span(rects, reach, size_t(cam) * N + n);
+        RectWalk w;
+        w.start(sr, ntx, T);
+        for (uint32_t k = 0; k < w.cnt; ++k) {
+            atomicAdd(&hist[w.bin()], 1u);
+            w.advance(sr, ntx, T);
+        }
+    }
+    __syncthreads();
+    uint32_t* out = range_counts + (size_t(cam) * n_ranges + range) * n_bins;
+    for (uint32_t b = lane; b < n_bins; b += kWave) out[b] = hist[b];
+}
+
+// Per (camera, bin): exclusive scan of its counts over the ranges (in place);
+// the column total goes to totals[cam * n_bins + bin].
+__global__ void __launch_bounds__(256)
+fused_bin_scan_ranges_kernel(uint32_t* __restrict__ range_counts, uint32_t C, uint32_t n_ranges, uint32_t n_bins,
+                             uint32_t* __restrict__ totals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C * n_bins) return;
+    const uint32_t cam = i / n_bins, b = i - cam * n_bins;
+    uint32_t* col = range_counts + size_t(cam) * n_ranges * n_bins + b;
+    uint32_t run = 0;
+    for (uint32_t r = 0; r < n_ranges; ++r) {
+        const uint32_t v = col[size_t(r) * n_bins];
+        col[size_t(r) * n_bins] = run;
+        run += v;
+    }
+    totals[i] = run;
+}
+
+// grid (n_ranges, C), one wave; dynamic LDS: the n_bins next free slots.
+template <typename R>
+__global__ void __launch_bounds__(kWave)
+fused_bin_place_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, const float2* __restrict__ reach,
+                       const uint32_t* __restrict__ cum, uint32_t N, uint32_t range_len, uint32_t n_ranges,
+                       uint32_t ntx, uint32_t T, uint32_t n_bins, uint32_t bin_bits,
+                       const uint32_t* __restrict__ range_starts, const uint32_t* __restrict__ bin_starts,
+                       uint32_t capacity, R* __restrict__ packed) {
+    extern __shared__ uint32_t next_slot[];
+    const uint32_t range = blockIdx.x, cam = blockIdx.y, lane = threadIdx.x;
+    const uint32_t* rs = range_starts + (size_t(cam) * n_ranges + range) * n_bins;
+    const uint32_t* bs = bin_starts + size_t(cam) * n_bins;
+    for (uint32_t b = lane; b < n_bins; b += kWave) next_slot[b] = bs[b] + rs[b];
+    __syncthreads();
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    const uint32_t n0 = range * range_len, n1 = min(N, n0 + range_len);
+    for (uint32_t s = n0; s < n1; s += kWave) {
+        const uint32_t n = s + lane;
+        const size_t idx = size_t(cam) * N + n;
+        SpanRect sr = {0u, 0u, 0.f, -1.f};
+        if (n < n1) sr = load_span(rects, reach, idx);
+        RectWalk w;
+        w.start(sr, ntx, T);
+        R rec = {};
+        uint32_t g = 0;
+        if (w.cnt) {
+            rec = recs[idx];
+            g = cum[idx];
+        }
+        for (uint32_t k = 0; __ballot(k < w.cnt); ++k) {
+            const bool act = k < w.cnt;
+            const uint32_t bin = act ? w.bin() : 0u;
+            // lanes of this step with the same bin, ranked by lane
+            uint64_t same = __ballot(act);
+            for (uint32_t bit = 0; bit < bin_bits; ++bit) {
+                const bool one = (bin >> bit) & 1u;
+                const uint64_t m = __ballot(one);
+                same &= one ? m : ~m;
+            }
+            if (act) {
+                // every lane reads its bin's slot before the group's last lane advances it
+                const uint32_t pos = next_slot[bin] + uint32_t(__popcll(same & below));
+                if ((same >> lane) == 1u) next_slot[bin] = pos + 1u;
+                if (pos < capacity) {
+                    R out = rec;
+                    out.g = int32_t(g + k);
+                    packed[pos] = out;
+                }
+                w.advance(sr, ntx, T);
+            }
+        }
+    }
+}
+
+// Camera-major tile list starts: list (cam, t) begins at bin (cam, t, class 0).
+__global__ void __launch_bounds__(256)
+fused_bin_tile_offsets_kernel(const uint32_t* __restrict__ bin_starts, uint32_t n_tiles,
+                              int32_t* __restrict__ tile_offsets) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_tiles) tile_offsets[t] = int32_t(bin_starts[size_t(t) * kBinClasses]);
+}
+
+// counters = {intersections, overflow}: the flag is raised when the count
+// exceeds the capacity.
+__global__ void fused_overflow_kernel(uint32_t* __restrict__ counters, uint32_t capacity) {
+    if (threadIdx.x == 0) counters[1] = counters[0] > capacity ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------------------
 // Forward: one workgroup of WAVES waves per T x T tile (T = 8, 16, 32). Wave
-// k takes the k-th contiguous slice of the tile's list and walks it in
-// batches of 64 records: each lane loads one record of the next batch
-// (coalesced vector loads issued a whole batch ahead, so HBM latency is
-// hidden behind the current batch) and the wave reads the current batch back
-// from its own LDS slot with broadcast loads. Lane = column (lane % T) and
-// row group (lane / T), with PX = T^2 / 64 rows per lane spaced 64 / T apart;
-// sigma reuses the column terms across the lane's pixels. The slices'
-// partial images are summed in LDS in wave order, so the result does not
-// depend on scheduling.
+// k takes batches k, k + WAVES, ... of 64 records of the tile's list: each
+// lane loads one record of its next batch (coalesced vector loads issued a
+// whole batch ahead, so HBM latency is hidden behind the current batch) and
+// the wave reads the current batch back from its own LDS slot with broadcast
+// loads. Lane = column (lane % T) and row group (lane / T), with PX = T^2 / 64
+// rows per lane spaced 64 / T apart; sigma reuses the column terms across the
+// lane's pixels. The waves' partial images are summed in LDS in wave order,
+// so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
 // Adds one record to a lane's PX pixels. Bit k of the wave-uniform `rows` is
 // set when row group k can hold contributing pixels; the others are skipped
@@ -488,8 +693,10 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
-    const int32_t per = (end - start + WAVES - 1) / WAVES;
-    const int32_t lo = min(end, start + int32_t(wave) * per), hi = min(end, lo + per);
+    // wave k takes batches k, k + WAVES, ... of 64 records (the lists are ordered
+    // by height class, so contiguous slices would give the first waves the tallest)
+    constexpr int32_t kStride = WAVES * kWave;
+    const int32_t lo = start + int32_t(wave) * kWave, hi = end;
     const uint32_t nt = ntx * nty, cam = tile / nt, t = tile - cam * nt;
     const uint32_t ty = t / ntx, tx = t - ty * ntx;
     const uint32_t lane = threadIdx.x % kWave, col = lane % T, row0 = lane / T;
@@ -504,7 +711,7 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
     R* slot = stage[wave];
     R next = load_splat(packed, lo + int32_t(lane), hi);
     const float y_base = float(ty * T);
-    for (int32_t base = lo; base < hi; base += kWave) {
+    for (int32_t base = lo; base < hi; base += kStride) {
         // row groups of the record this lane stages (lane rows are row0 + STEP k,
         // row0 < STEP); record j's span is read from lane j's register
         int r_lo, r_hi;
@@ -515,7 +722,7 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
         wave_lds_sync();  // every lane is done reading the previous batch
         slot[lane] = next;
         wave_lds_sync();
-        next = load_splat(packed, base + kWave + int32_t(lane), hi);
+        next = load_splat(packed, base + kStride + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         // LDS reads pipelined one record ahead (the batch holds n >= 1 records)
         R cur = slot[0];
@@ -837,6 +1044,10 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
 struct FusedScratch {
     size_t recs, rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, tm, lengths, cm,
         tile_scan, bytes;
+    // direct placement (place = true) or sort + pack
+    bool place;
+    uint32_t range_len, n_ranges, n_bins, bin_bits;
+    size_t reach, range_counts, bin_totals, bin_scan;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -846,13 +1057,36 @@ static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 constexpr int kKindBeta = 1, kKindVar = 2;
 static size_t record_bytes(int kind) { return (kind & (kKindBeta | kKindVar)) ? sizeof(BetaSplat) : sizeof(Splat); }
 
+// Largest bin table of the direct placement (its waves hold one table each in
+// LDS, so larger tables cost occupancy; larger tile grids sort and pack).
+constexpr size_t kPlaceLdsBytes = 16384;
+constexpr uint32_t kPlaceWaves = 4096;  // target number of ranges (waves) per binning pass
+
 static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint32_t capacity, int kind) {
-    FusedScratch L;
+    FusedScratch L = {};
     const size_t CN = size_t(C) * N;
     size_t o = 0;
     L.recs = o; o += al(CN * record_bytes(kind));
     L.rects = o; o += al(CN * sizeof(uint2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
+    const uint64_t nt = C ? n_tiles / C : 0;
+    L.place = C > 0 && nt * kBinClasses * sizeof(uint32_t) <= kPlaceLdsBytes;
+    if (L.place) {
+        L.reach = o; o += al(CN * sizeof(float2));
+        L.n_bins = uint32_t(nt) * kBinClasses;
+        L.bin_bits = 32u - uint32_t(__builtin_clz(L.n_bins - 1u));
+        const uint64_t per = (CN + kPlaceWaves - 1) / kPlaceWaves;
+        L.range_len = uint32_t(std::max<uint64_t>(256, (per + 63) / 64 * 64));
+        L.n_ranges = (N + L.range_len - 1) / L.range_len;
+        L.range_counts = o; o += al(size_t(C) * L.n_ranges * L.n_bins * 4 + 4);
+        L.bin_totals = o; o += al(size_t(C) * L.n_bins * 4 + 4);
+        L.bin_scan = o; o += al(scan_workspace_bytes(uint64_t(C) * L.n_bins));
+        L.order_sort_bytes = sort_workspace_bytes(n_tiles);
+        L.order_sort = o; o += al(L.order_sort_bytes);
+        L.order_keys = o; o += al(size_t(n_tiles) * 4 + 4);
+        L.bytes = o;
+        return L;
+    }
     L.sort_bytes = sort_workspace_bytes(capacity);
     L.sort = o; o += al(L.sort_bytes);
     L.keys = o; o += al(size_t(capacity) * 4 + 4);
@@ -962,10 +1196,39 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         dim3 grid(ceil_div(N, 256), C);
         fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
             means, quats, scales, densities, betas, beta_ratio, uncertainties, viewmats, Ks, width, height, eps,
-            radius_clip, tile_size, ntx, nty, N, at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), cum);
+            radius_clip, tile_size, ntx, nty, N, at<R>(scratch, L.recs), at<uint2>(scratch, L.rects),
+            L.place ? at<float2>(scratch, L.reach) : nullptr, cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
+    if (L.place) {
+        fused_overflow_kernel<<<1, kWave, 0, stream>>>(counters, capacity);
+        if (int rc = check_launch("fused_overflow")) return rc;
+        uint32_t* range_counts = at<uint32_t>(scratch, L.range_counts);
+        uint32_t* totals = at<uint32_t>(scratch, L.bin_totals);
+        const size_t lds = size_t(L.n_bins) * sizeof(uint32_t);
+        const dim3 grid(L.n_ranges, C);
+        if (CN > 0) {
+            fused_bin_count_kernel<<<grid, kWave, lds, stream>>>(at<uint2>(scratch, L.rects),
+                                                                 at<float2>(scratch, L.reach), N, L.range_len,
+                                                                 L.n_ranges, ntx, tile_size, L.n_bins, range_counts);
+            if (int rc = check_launch("fused_bin_count")) return rc;
+        }
+        const uint32_t n_cols = C * L.n_bins;
+        fused_bin_scan_ranges_kernel<<<ceil_div(n_cols, 256), 256, 0, stream>>>(range_counts, C, L.n_ranges, L.n_bins,
+                                                                               totals);
+        if (int rc = check_launch("fused_bin_scan_ranges")) return rc;
+        if (int rc = scan_exclusive(totals, n_cols, totals, nullptr, at<void>(scratch, L.bin_scan), stream)) return rc;
+        if (CN > 0) {
+            fused_bin_place_kernel<R><<<grid, kWave, lds, stream>>>(
+                at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach), cum, N,
+                L.range_len, L.n_ranges, ntx, tile_size, L.n_bins, L.bin_bits, range_counts, totals, capacity,
+                static_cast<R*>(packed));
+            if (int rc = check_launch("fused_bin_place")) return rc;
+        }
+        fused_bin_tile_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(totals, n_tiles, tile_offsets);
+        if (int rc = check_launch("fused_bin_tile_offsets")) return rc;
+    } else {
     void* sort_ws = at<void>(scratch, L.sort);
     if (CN > 0) {
         fused_emit_kernel<R><<<ceil_div(CN, 256), 256, 0, stream>>>(
@@ -994,6 +1257,7 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C, ntx * nty, ntx, tile_size, vals, tm, cm, counters,
         capacity, static_cast<R*>(packed), tile_offsets);
     if (int rc = check_launch("fused_pack")) return rc;
+    }
     if (tile_order) {
         void* order_ws = at<void>(scratch, L.order_sort);
         fused_tile_order_keys_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(

@@ -92,6 +92,36 @@ def test_fused_tile_offsets_match_operator_binning(device):
     assert torch.equal(f.record_offsets.long(), expect)
 
 
+@pytest.mark.parametrize("use_beta", [False, True])
+def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
+    """The packed tile lists are a permutation of the emission order: each
+    emission index (g field) appears exactly once, in the camera-major list of
+    its own (camera, tile), on a record of its own primitive (same projected
+    mean as the staged projection)."""
+    sc = workloads.scene(3000, 5, 96, 80, scale_range=(0.01, 0.08), seed=9, device=device,
+                         beta_range=(0.5, 4.0) if use_beta else None)
+    clip = 1.0 if use_beta else 3.0
+    _, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 80, 96, 1e-8, clip, beta=sc.beta)
+    n, C, N, ntx, nty = f.n_isects, 5, 3000, 5, 6
+    width = 12 if use_beta else 8
+    words = f.packed.view(torch.int32)[: n * width].reshape(n, width).cpu()
+    g = words[:, 7].numpy()
+    mean = words[:, :2].view(torch.float32).numpy()
+    assert np.array_equal(np.sort(g), np.arange(n))
+    m2, radii = _proj_clip(sc, clip)
+    m2n = m2.cpu().numpy()
+    x0, x1, y0, y1 = O.tile_rects_2d(m2n, radii.cpu().numpy(), 16, ntx, nty)
+    counts = ((x1 - x0) * (y1 - y0)).reshape(-1)
+    cam = np.repeat(np.arange(C), N)
+    keys, owners = O._emit(counts, [y0.reshape(-1), x0.reshape(-1)], [(y1 - y0).reshape(-1), (x1 - x0).reshape(-1)],
+                           [ntx, 1], cam * (ntx * nty))
+    offsets = f.offsets.reshape(-1).cpu().numpy().astype(np.int64)
+    ends = np.append(offsets[1:], n)
+    lists = np.repeat(np.arange(offsets.size), ends - offsets)
+    assert np.array_equal(keys[g], lists)  # every record sits in the list of its own tile
+    np.testing.assert_allclose(mean, m2n.reshape(-1, 2)[owners[g]], rtol=1e-5, atol=1e-3)
+
+
 def test_outstanding_contexts_are_independent(device):
     a = workloads.scene(500, 2, 48, 48, scale_range=(0.02, 0.1), seed=1, device=device)
     b = workloads.scene(700, 2, 48, 48, scale_range=(0.02, 0.1), seed=2, device=device)
@@ -136,9 +166,11 @@ def test_working_size_fused_agrees_with_staged_path(device):
         assert torch.equal(getattr(g2, name), 2.0 * getattr(a, name)), name
 
 
-@pytest.mark.parametrize("tile,waves", [(8, 4), (8, 16), (32, 4), (32, 16), (16, 16)])
-def test_fused_tile_sizes_and_wave_counts_match_oracle(device, tile, waves):
-    W, H = 72, 56
+# 72 x 56: every tile size bins by direct placement; 200 x 160 at 8-pixel tiles
+# (500 tiles per camera, more bins than the placement table holds) sorts and packs
+@pytest.mark.parametrize("tile,waves,W,H", [(8, 4, 72, 56), (8, 16, 72, 56), (32, 4, 72, 56), (32, 16, 72, 56),
+                                           (16, 16, 72, 56), (8, 8, 200, 160)])
+def test_fused_tile_sizes_and_wave_counts_match_oracle(device, tile, waves, W, H):
     sc = workloads.scene(700, 3, H, W, scale_range=(0.02, 0.1), seed=6, device=device)
     img, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 3.0, tile_size=tile, waves=waves)
     _, ids, _ = ops.isect_tiles(*_proj(sc), tile, -(-W // tile), -(-H // tile))
@@ -179,11 +211,12 @@ def _beta_oracle(sc, v, tile, clip=1.0):
     return ref, torch.autograd.grad((ref * v.cpu().double()).sum(), leaves)
 
 
-@pytest.mark.parametrize("tile,waves", [(16, 8), (8, 4), (32, 16)])
-def test_fused_beta_matches_oracle(device, tile, waves):
+@pytest.mark.parametrize("tile,waves,W,H", [(16, 8, 72, 56), (8, 4, 72, 56), (32, 16, 72, 56), (8, 8, 200, 160)])
+def test_fused_beta_matches_oracle(device, tile, waves, W, H):
     """Beta primitives (alpha = w (1 - d^2)^(beta + 1/2), w = density mu ratio(beta)):
-    image to 1e-5 and all five gradients, beta included, to 1e-4 of the fp64 oracle."""
-    sc = _beta_scene(device)
+    image to 1e-5 and all five gradients, beta included, to 1e-4 of the fp64 oracle
+    (both binning paths, as in the Gaussian test)."""
+    sc = _beta_scene(device, H=H, W=W)
     W, H = sc.width, sc.height
     img, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 1.0, tile_size=tile, waves=waves,
                            beta=sc.beta)
