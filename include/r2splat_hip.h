@@ -310,14 +310,20 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * per-pixel summation order does (rounding only).
  *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
- * packed = capacity records of r2s_fused_record_bytes() each, in the
- * reference's list order (camera, tile, primitive); tile_offsets int32
+ * packed = capacity records of r2s_fused_record_bytes() each, in
+ * camera-major tile lists (the reference's list layout; within a list the
+ * records run by decreasing height class, a deterministic order that only
+ * changes the per-pixel summation order); tile_offsets int32
  * [C, ceil(H/tile), ceil(W/tile)] (the reference's layout); tile_order
  * uint32 [same count]
  * (tiles by decreasing intersection count, the order the raster kernels
- * walk them in; may be null, then natural order), record_offsets uint32 [C*N + 1]
- * (first intersection of each (camera, primitive) in emission order) and
- * counters uint32 [2] persist from the forward to the backward. counters[0]
+ * walk them in; may be null, then natural order), record_offsets uint32
+ * [C*N + 1 + N] and counters uint32 [2] persist from the forward to the
+ * backward. The forward processes primitives in the order of a 30-bit
+ * Morton code of their means (ties in input order): record_offsets[C*N + 1
+ * + p] is the input index of position p, and record_offsets[c*N + p] the
+ * first intersection of (camera c, position p) in emission order. The
+ * gradients come back in the input order. counters[0]
  * = intersections, counters[1] = 1 when capacity was exceeded (that
  * forward's outputs are then incomplete and the call must be repeated with
  * a larger capacity). isect_grads: float [capacity * 8], one VJP partial per

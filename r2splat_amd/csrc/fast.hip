@@ -158,6 +158,124 @@ __device__ __forceinline__ uint32_t rows_reached(float my, float ext, float y_ba
 }
 
 // ---------------------------------------------------------------------------
+// Primitive order. The forward processes primitives in the order of a 30-bit
+// Morton code of their means (10 bits per axis over the means' bounding box;
+// ties keep the input order): consecutive primitives then project to nearby
+// tiles in every view, so the binning's gathers and writes stay local. The
+// records, the per-record intersection offsets and the emission order all
+// follow that order; the parameter VJP maps back to the input order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOrderThreads = 1024;
+
+// Floats as unsigned integers of the same order (for min/max atomics).
+__device__ __forceinline__ uint32_t order_bits(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float order_float(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// bounds (ordered bits) = {min x, min y, min z, max x, max y, max z} of the
+// finite means; bounds starts at {~0 x3, 0 x3} (morton_bounds_init_kernel).
+__global__ void morton_bounds_init_kernel(uint32_t* __restrict__ bounds) {
+    if (threadIdx.x < 6) bounds[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+}
+
+__global__ void __launch_bounds__(kOrderThreads)
+morton_bounds_kernel(const float* __restrict__ means, uint32_t N, uint32_t* __restrict__ bounds) {
+    __shared__ float red[6][kOrderThreads / kWave];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t n = blockIdx.x * kOrderThreads + threadIdx.x; n < N; n += gridDim.x * kOrderThreads) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float x = means[3 * size_t(n) + k];
+            if (isfinite(x)) {
+                lo[k] = fminf(lo[k], x);
+                hi[k] = fmaxf(hi[k], x);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off, kWave));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off, kWave));
+        }
+    }
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            red[k][wave] = lo[k];
+            red[3 + k][wave] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const bool is_lo = threadIdx.x < 3;
+        float v = red[threadIdx.x][0];
+        for (uint32_t w = 1; w < kOrderThreads / kWave; ++w)
+            v = is_lo ? fminf(v, red[threadIdx.x][w]) : fmaxf(v, red[threadIdx.x][w]);
+        if (is_lo) atomicMin(bounds + threadIdx.x, order_bits(v));
+        else atomicMax(bounds + threadIdx.x, order_bits(v));
+    }
+}
+
+__device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {  // 10 bits -> every third bit
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void __launch_bounds__(256)
+morton_keys_kernel(const float* __restrict__ means, uint32_t N, const uint32_t* __restrict__ bounds,
+                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // no finite mean: lo = NaN and every key 0
+        const float lo = order_float(bounds[k]), ext = order_float(bounds[3 + k]) - lo;
+        const float t = ext > 0.f ? (means[3 * size_t(n) + k] - lo) / ext : 0.f;
+        const float q = fminf(fmaxf(t * 1024.f, 0.f), 1023.f);  // NaN -> 0
+        key |= spread_bits3(uint32_t(q)) << (2 - k);
+    }
+    keys[n] = key;
+    vals[n] = n;
+}
+
+// Copies of the primitive parameters in the sorted order (perm[p] = input index).
+__global__ void __launch_bounds__(256)
+permute_params_kernel(const uint32_t* __restrict__ perm, uint32_t N, const float* __restrict__ means,
+                      const float* __restrict__ quats, const float* __restrict__ scales,
+                      const float* __restrict__ dens, const float* __restrict__ betas,
+                      const float* __restrict__ ratio, const float* __restrict__ unc, float* __restrict__ o_means,
+                      float* __restrict__ o_quats, float* __restrict__ o_scales, float* __restrict__ o_dens,
+                      float* __restrict__ o_betas, float* __restrict__ o_ratio, float* __restrict__ o_unc) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const size_t n = perm[p];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o_means[3 * size_t(p) + k] = means[3 * n + k];
+        o_scales[3 * size_t(p) + k] = scales[3 * n + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o_quats[4 * size_t(p) + k] = quats[4 * n + k];
+    o_dens[p] = dens[n];
+    if (betas) {
+        o_betas[p] = betas[n];
+        o_ratio[p] = ratio[n];
+    }
+    if (unc) o_unc[p] = unc[n];
+}
+
+// ---------------------------------------------------------------------------
 // Fused projection + tile count. grid (ceil(N/256), C).
 // ---------------------------------------------------------------------------
 template <bool BETA, bool VAR>
@@ -943,13 +1061,16 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
                        float eps, uint32_t C, uint32_t N, const uint32_t* __restrict__ cum,
                        const uint32_t* __restrict__ d_n, uint32_t capacity, const float* __restrict__ isect_grads,
                        float* __restrict__ v_means, float* __restrict__ v_quats, float* __restrict__ v_scales,
-                       float* __restrict__ v_dens, float* __restrict__ v_betas, float* __restrict__ v_unc) {
+                       float* __restrict__ v_dens, float* __restrict__ v_betas, float* __restrict__ v_unc,
+                       const uint32_t* __restrict__ perm) {
     __shared__ float part[kVjpWaves][kParamFields][kWave];
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const uint32_t n = blockIdx.x * kWave + lane;
+    // lane p of the primitive order: records (c, p), parameters and gradients of input primitive n
+    const uint32_t pr = blockIdx.x * kWave + lane;
+    const uint32_t n = pr < N ? perm[pr] : pr;
     float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float q[4] = {0.f, 0.f, 0.f, 0.f}, mag = 0.f, norm = 0.f, R[9], cov[6], s[3] = {0.f, 0.f, 0.f};
-    if (n < N) {
+    if (pr < N) {
         const uint32_t n_live = live(d_n, capacity);
         unit_quat(quats + 4 * size_t(n), q, mag, norm);
         quat_to_rot(q, R);
@@ -959,7 +1080,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
         covariance_from(R, s, cov);
         const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
         for (uint32_t c = wave; c < C; c += kVjpWaves) {
-            const size_t v = size_t(c) * N + n;
+            const size_t v = size_t(c) * N + pr;
             const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
             float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f, g7 = 0.f;
             for (uint32_t j = lo; j < hi; ++j) {
@@ -1018,7 +1139,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
 #pragma unroll
     for (int f = 0; f < kParamFields; ++f) part[wave][f][lane] = acc[f];
     __syncthreads();
-    if (wave != 0 || n >= N) return;
+    if (wave != 0 || pr >= N) return;
 #pragma unroll
     for (int f = 0; f < kParamFields; ++f) {
         float t = part[0][f][lane];
@@ -1048,7 +1169,13 @@ struct FusedScratch {
     bool place;
     uint32_t range_len, n_ranges, n_bins, bin_bits;
     size_t reach, range_counts, bin_totals, bin_scan;
+    // primitive order: bounds, key sort, and the permuted parameter copies
+    size_t order_bounds, order_sort_n, order_sort_n_bytes, order_sorted_keys, params;
 };
+
+// floats per primitive of the permuted copies: mean 3, quaternion 4, scale 3,
+// density, beta, ratio, uncertainty
+constexpr size_t kPermutedFloats = 14;
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -1066,6 +1193,11 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
     FusedScratch L = {};
     const size_t CN = size_t(C) * N;
     size_t o = 0;
+    L.order_bounds = o; o += al(8 * sizeof(float));
+    L.order_sort_n_bytes = sort_workspace_bytes(N);
+    L.order_sort_n = o; o += al(L.order_sort_n_bytes);
+    L.order_sorted_keys = o; o += al(size_t(N) * 4 + 4);
+    L.params = o; o += al(size_t(N) * kPermutedFloats * sizeof(float));
     L.recs = o; o += al(CN * record_bytes(kind));
     L.rects = o; o += al(CN * sizeof(uint2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
@@ -1192,10 +1324,33 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
     uint32_t* cum = record_offsets;
+    uint32_t* perm = record_offsets + size_t(CN) + 1;  // the primitive order (input index per position)
+    float* pp = at<float>(scratch, L.params);
+    float *p_means = pp, *p_quats = pp + 3 * size_t(N), *p_scales = pp + 7 * size_t(N), *p_dens = pp + 10 * size_t(N);
+    float *p_betas = pp + 11 * size_t(N), *p_ratio = pp + 12 * size_t(N), *p_unc = pp + 13 * size_t(N);
+    if (N > 0) {
+        void* ows = at<void>(scratch, L.order_sort_n);
+        uint32_t* bounds = at<uint32_t>(scratch, L.order_bounds);
+        morton_bounds_init_kernel<<<1, kWave, 0, stream>>>(bounds);
+        if (int rc = check_launch("morton_bounds_init")) return rc;
+        morton_bounds_kernel<<<std::min(ceil_div(N, kOrderThreads), 256u), kOrderThreads, 0, stream>>>(means, N,
+                                                                                                       bounds);
+        if (int rc = check_launch("morton_bounds")) return rc;
+        morton_keys_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, N, bounds,
+                                                                 sort_keys_buffer(ows), sort_vals_buffer(ows, N));
+        if (int rc = check_launch("morton_keys")) return rc;
+        if (int rc = sort_pairs_u32(ows, L.order_sort_n_bytes, N, nullptr, 30, at<uint32_t>(scratch, L.order_sorted_keys),
+                                    perm, stream))
+            return rc;
+        permute_params_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
+            perm, N, means, quats, scales, densities, BETA ? betas : nullptr, BETA ? beta_ratio : nullptr,
+            VAR ? uncertainties : nullptr, p_means, p_quats, p_scales, p_dens, p_betas, p_ratio, p_unc);
+        if (int rc = check_launch("permute_params")) return rc;
+    }
     if (CN > 0) {
         dim3 grid(ceil_div(N, 256), C);
         fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
-            means, quats, scales, densities, betas, beta_ratio, uncertainties, viewmats, Ks, width, height, eps,
+            p_means, p_quats, p_scales, p_dens, p_betas, p_ratio, p_unc, viewmats, Ks, width, height, eps,
             radius_clip, tile_size, ntx, nty, N, at<R>(scratch, L.recs), at<uint2>(scratch, L.rects),
             L.place ? at<float2>(scratch, L.reach) : nullptr, cum);
         if (int rc = check_launch("fused_project")) return rc;
@@ -1345,7 +1500,7 @@ extern "C" int r2s_fused_param_vjp(const float* means, const float* quats, const
     fused_param_vjp_kernel<B_, V_><<<blocks, kVjpWaves * kWave, 0, stream>>>(                                   \
         means, quats, scales, densities, B_ ? beta_ratio : nullptr, B_ ? beta_log_grad : nullptr, viewmats, Ks,  \
         width, height, eps, C, N, record_offsets, counters, capacity, isect_grads, v_means, v_quats, v_scales,  \
-        v_densities, B_ ? v_betas : nullptr, v_uncertainties)
+        v_densities, B_ ? v_betas : nullptr, v_uncertainties, record_offsets + size_t(C) * N + 1)
     if (beta && v_uncertainties) R2S_VJP(true, true);
     else if (beta) R2S_VJP(true, false);
     else if (v_uncertainties) R2S_VJP(false, true);

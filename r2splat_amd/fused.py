@@ -129,7 +129,8 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
     rec = be.lib().r2s_fused_record_bytes(kind)
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
     tile_order = torch.empty((C * nty * ntx,), dtype=torch.int32, device=device)
-    record_offsets = torch.empty((C * N + 1,), dtype=torch.int32, device=device)
+    # per-record intersection offsets (C*N + 1) followed by the primitive order (N)
+    record_offsets = torch.empty((C * N + 1 + N,), dtype=torch.int32, device=device)
     counters = torch.zeros((2,), dtype=torch.int32, device=device)
     with torch.cuda.device(device):
         stream = be.stream_of(device)

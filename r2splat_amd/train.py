@@ -137,7 +137,7 @@ class GaussianFit:
         ntx, nty = -(-self.W // self.tile_size), -(-self.H // self.tile_size)
         self.tile_offsets = torch.empty((self.C, nty, ntx), dtype=torch.int32, device=device)
         self.tile_order = torch.empty((self.C * nty * ntx,), dtype=torch.int32, device=device)
-        self.record_offsets = torch.empty((self.C * N + 1,), dtype=torch.int32, device=device)
+        self.record_offsets = torch.empty((self.C * N + 1 + N,), dtype=torch.int32, device=device)
         self.counters = torch.zeros((2,), dtype=torch.int32, device=device)
         self.ws = workspace if workspace is not None else Workspace(device)
         self.graph = None

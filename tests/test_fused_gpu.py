@@ -85,11 +85,22 @@ def test_fused_tile_offsets_match_operator_binning(device):
     off = ops.isect_offset_encode(ids, 5, 5, 6)
     assert f.n_isects == ids.numel()
     assert torch.equal(f.offsets, off)
-    # per-(camera, primitive) intersection offsets: exclusive scan of the tile-rectangle areas
-    flat = counts.reshape(-1).long()
+    # the primitive order (after the offsets): a permutation, sorted by the Morton code of the means
+    CN = counts.numel()
+    perm = f.record_offsets[CN + 1:].long()
+    assert torch.equal(torch.sort(perm).values, torch.arange(3000, device=device))
+    lo, hi = sc.means.min(0).values, sc.means.max(0).values
+    q = ((sc.means - lo) / (hi - lo) * 1024).clamp(0, 1023).long()
+    code = torch.zeros(3000, dtype=torch.long, device=device)
+    for bit in range(10):
+        for k in range(3):
+            code |= ((q[:, k] >> bit) & 1) << (3 * bit + 2 - k)
+    assert bool((code[perm][1:] >= code[perm][:-1]).all())
+    # per-(camera, position) intersection offsets: exclusive scan of the tile-rectangle areas in that order
+    flat = counts[:, perm].reshape(-1).long()
     expect = torch.zeros(flat.numel() + 1, dtype=torch.long, device=device)
     expect[1:] = torch.cumsum(flat, 0)
-    assert torch.equal(f.record_offsets.long(), expect)
+    assert torch.equal(f.record_offsets[:CN + 1].long(), expect)
 
 
 @pytest.mark.parametrize("use_beta", [False, True])
@@ -108,9 +119,11 @@ def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
     g = words[:, 7].numpy()
     mean = words[:, :2].view(torch.float32).numpy()
     assert np.array_equal(np.sort(g), np.arange(n))
+    # records follow the primitive order stored after the offsets
+    perm = f.record_offsets[C * N + 1:].long().cpu().numpy()
     m2, radii = _proj_clip(sc, clip)
-    m2n = m2.cpu().numpy()
-    x0, x1, y0, y1 = O.tile_rects_2d(m2n, radii.cpu().numpy(), 16, ntx, nty)
+    m2n = m2.cpu().numpy()[:, perm]
+    x0, x1, y0, y1 = O.tile_rects_2d(m2n, radii.cpu().numpy()[:, perm], 16, ntx, nty)
     counts = ((x1 - x0) * (y1 - y0)).reshape(-1)
     cam = np.repeat(np.arange(C), N)
     keys, owners = O._emit(counts, [y0.reshape(-1), x0.reshape(-1)], [(y1 - y0).reshape(-1), (x1 - x0).reshape(-1)],
