@@ -811,6 +811,8 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    start = __builtin_amdgcn_readfirstlane(start);  // uniform: keep the loop bounds scalar
+    end = __builtin_amdgcn_readfirstlane(end);
     // wave k takes batches k, k + WAVES, ... of 64 records (the lists are ordered
     // by height class, so contiguous slices would give the first waves the tallest)
     constexpr int32_t kStride = WAVES * kWave;
@@ -842,14 +844,21 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
         wave_lds_sync();
         next = load_splat(packed, base + kStride + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
-        // LDS reads pipelined one record ahead (the batch holds n >= 1 records)
-        R cur = slot[0];
-        for (int32_t j = 1; j < n; ++j) {
-            const R nxt = slot[j];
-            splat_accumulate<PX, BETA, VAR>(cur, __builtin_amdgcn_readlane(spans, j - 1), px, py, acc, acc_var);
-            cur = nxt;
+        // LDS reads pipelined one record ahead in two registers that alternate
+        // roles (no copies between them); the batch holds n >= 1 records
+        // (settle waits for the current record before the next read is issued, so
+        // that read overlaps the current record's work)
+        R a = slot[0];
+        for (int32_t j = 0;;) {
+            settle(a);
+            const R b = slot[min(j + 1, int32_t(kWave) - 1)];
+            splat_accumulate<PX, BETA, VAR>(a, __builtin_amdgcn_readlane(spans, j), px, py, acc, acc_var);
+            if (++j >= n) break;
+            settle(b);
+            a = slot[min(j + 1, int32_t(kWave) - 1)];
+            splat_accumulate<PX, BETA, VAR>(b, __builtin_amdgcn_readlane(spans, j), px, py, acc, acc_var);
+            if (++j >= n) break;
         }
-        splat_accumulate<PX, BETA, VAR>(cur, __builtin_amdgcn_readlane(spans, n - 1), px, py, acc, acc_var);
     }
 #pragma unroll
     for (int k = 0; k < PX; ++k) part[wave][(row0 + STEP * k) * T + col] = acc[k];
