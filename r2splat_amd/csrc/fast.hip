@@ -1060,6 +1060,8 @@ fused_raster_bwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
 // ---------------------------------------------------------------------------
 constexpr int kParamFields = 12;  // v_mean 3, v_covariance 6, v_density, v_beta, v_uncertainty
 constexpr int kVjpWaves = 16;
+// partials one wave stages at a time (its share of the LDS buffer, 8 floats each)
+constexpr uint32_t kVjpStageParts = kParamFields * kWave / kPartialStride;
 
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kVjpWaves * kWave)
@@ -1072,40 +1074,74 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
                        float* __restrict__ v_means, float* __restrict__ v_quats, float* __restrict__ v_scales,
                        float* __restrict__ v_dens, float* __restrict__ v_betas, float* __restrict__ v_unc,
                        const uint32_t* __restrict__ perm) {
-    __shared__ float part[kVjpWaves][kParamFields][kWave];
+    // one LDS buffer: each wave's staging area during the camera loop, then the
+    // same wave's partial sums (part[wave][field][lane]) for the final reduction
+    __shared__ __align__(16) float lds[kVjpWaves * kParamFields * kWave];
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     // lane p of the primitive order: records (c, p), parameters and gradients of input primitive n
     const uint32_t pr = blockIdx.x * kWave + lane;
-    const uint32_t n = pr < N ? perm[pr] : pr;
+    const bool valid = pr < N;
+    const uint32_t n = valid ? perm[pr] : pr;
+    const uint32_t last = min(N - blockIdx.x * kWave, uint32_t(kWave)) - 1u;  // the last valid lane
+    float* stage = lds + wave * (kParamFields * kWave);
     float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float q[4] = {0.f, 0.f, 0.f, 0.f}, mag = 0.f, norm = 0.f, R[9], cov[6], s[3] = {0.f, 0.f, 0.f};
-    if (pr < N) {
-        const uint32_t n_live = live(d_n, capacity);
+    float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, p[3] = {0.f, 0.f, 0.f};
+    const uint32_t n_live = live(d_n, capacity);
+    if (valid) {
+        // the rotation and scales are recomputed for the final transform VJP
+        // rather than kept live across the camera loop
+        float q[4], mag, norm, R[9];
         unit_quat(quats + 4 * size_t(n), q, mag, norm);
         quat_to_rot(q, R);
-        s[0] = scales[3 * size_t(n)];
-        s[1] = scales[3 * size_t(n) + 1];
-        s[2] = scales[3 * size_t(n) + 2];
+        const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
         covariance_from(R, s, cov);
-        const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
+        p[0] = means[3 * size_t(n)];
+        p[1] = means[3 * size_t(n) + 1];
+        p[2] = means[3 * size_t(n) + 2];
+    }
+    {
         for (uint32_t c = wave; c < C; c += kVjpWaves) {
             const size_t v = size_t(c) * N + pr;
-            const uint32_t lo = min(cum[v], n_live), hi = min(cum[v + 1], n_live);
+            uint32_t lo = 0, hi = 0;
+            if (valid) {
+                lo = min(cum[v], n_live);
+                hi = min(cum[v + 1], n_live);
+            }
+            // the wave's 64 positions own one contiguous range of partials in this
+            // camera: it is staged through LDS with coalesced loads, then each lane
+            // sums its own records' partials in emission order
+            const uint32_t wlo = __builtin_amdgcn_readfirstlane(lo);
+            const uint32_t whi = __builtin_amdgcn_readlane(hi, last);
             float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f, g7 = 0.f;
-            for (uint32_t j = lo; j < hi; ++j) {
-                const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(j));
-                const float4 a = gp[0];
-                g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
-                if constexpr (BETA || VAR) {
-                    const float4 b = gp[1];
-                    g4 += b.x; g5 += b.y; g6 += b.z; g7 += b.w;
-                } else {
-                    const float2 b = *reinterpret_cast<const float2*>(gp + 1);
-                    g4 += b.x; g5 += b.y;
+            for (uint32_t base = wlo; base < whi; base += kVjpStageParts) {
+                const uint32_t cnt = min(kVjpStageParts, whi - base);
+                wave_lds_sync();  // every lane is done with the previous chunk
+                for (uint32_t i = lane; i < cnt; i += kWave) {
+                    const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(base + i));
+                    float4* sp = reinterpret_cast<float4*>(stage + kPartialStride * i);
+                    sp[0] = gp[0];
+                    if constexpr (BETA || VAR)
+                        sp[1] = gp[1];
+                    else
+                        *reinterpret_cast<float2*>(sp + 1) = *reinterpret_cast<const float2*>(gp + 1);
+                }
+                wave_lds_sync();
+                const uint32_t j1 = min(hi, base + cnt);
+                for (uint32_t j = max(lo, base); j < j1; ++j) {
+                    const float4* gp = reinterpret_cast<const float4*>(stage + kPartialStride * (j - base));
+                    const float4 a = gp[0];
+                    g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
+                    if constexpr (BETA || VAR) {
+                        const float4 b = gp[1];
+                        g4 += b.x; g5 += b.y; g6 += b.z; g7 += b.w;
+                    } else {
+                        const float2 b = *reinterpret_cast<const float2*>(gp + 1);
+                        g4 += b.x; g5 += b.y;
+                    }
                 }
             }
-            if (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f && g6 == 0.f &&
-                g7 == 0.f)
+            if (!valid || (g0 == 0.f && g1 == 0.f && g2 == 0.f && g3 == 0.f && g4 == 0.f && g5 == 0.f &&
+                           g6 == 0.f && g7 == 0.f))
                 continue;
             acc[11] += g7;  // the uncertainty enters the variance directly
             Camera<float> cam;
@@ -1145,16 +1181,21 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             for (int f = 0; f < kParamFields - 1; ++f) acc[f] += out[f];
         }
     }
+    wave_lds_sync();  // the wave's last chunk is read before its sums overwrite the area
 #pragma unroll
-    for (int f = 0; f < kParamFields; ++f) part[wave][f][lane] = acc[f];
+    for (int f = 0; f < kParamFields; ++f) stage[f * kWave + lane] = acc[f];
     __syncthreads();
     if (wave != 0 || pr >= N) return;
 #pragma unroll
     for (int f = 0; f < kParamFields; ++f) {
-        float t = part[0][f][lane];
-        for (int w = 1; w < kVjpWaves; ++w) t += part[w][f][lane];
+        float t = lds[f * kWave + lane];
+        for (int w = 1; w < kVjpWaves; ++w) t += lds[(w * kParamFields + f) * kWave + lane];
         acc[f] = t;
     }
+    float q[4], mag, norm, R[9];
+    unit_quat(quats + 4 * size_t(n), q, mag, norm);
+    quat_to_rot(q, R);
+    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
     float gq[4], vs[3];
     covariance_vjp(q, R, s, acc + 3, gq, vs);
     normalize_vjp(q, mag, norm, gq, v_quats + 4 * size_t(n));
