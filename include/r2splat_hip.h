@@ -310,10 +310,13 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * per-pixel summation order does (rounding only).
  *
  * Buffers: scratch (r2s_fused_scratch_bytes, dead after the forward);
- * packed = capacity records of r2s_fused_record_bytes() each, in
- * camera-major tile lists (the reference's list layout; within a list the
- * records run by decreasing height class, a deterministic order that only
- * changes the per-pixel summation order); tile_offsets int32
+ * packed = r2s_fused_packed_bytes(C, N, capacity, kind) bytes: `capacity`
+ * list entries {record index, emission index} (uint2) in camera-major tile
+ * lists (the reference's list layout; within a list the entries run by
+ * decreasing height class, a deterministic order that only changes the
+ * per-pixel summation order), then the C*N projected records of
+ * r2s_fused_record_bytes(kind) each (offset: capacity * 8 rounded up to 256
+ * bytes), which the raster kernels gather through the entries; tile_offsets int32
  * [C, ceil(H/tile), ceil(W/tile)] (the reference's layout); tile_order
  * uint32 [same count]
  * (tiles by decreasing intersection count, the order the raster kernels
@@ -346,6 +349,7 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
 size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width, uint32_t height,
                                uint32_t tile_size, uint32_t capacity, int kind);
 size_t r2s_fused_record_bytes(int kind);
+size_t r2s_fused_packed_bytes(uint32_t C, uint32_t N, uint32_t capacity, int kind);
 
 int r2s_fused_forward(const float* means, const float* quats, const float* scales,
                       const float* densities, const float* betas, const float* beta_ratio,

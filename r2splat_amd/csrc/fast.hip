@@ -356,18 +356,15 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
 // at their scanned positions, which are camera-major; entries that would pass
 // the capacity raise the overflow flag. A stable sort on the key alone then
 // orders the list by (tile, camera, primitive): one radix pass for up to 256
-// tiles per camera. The projected record's g field is set to its first
-// emission index.
-template <typename R>
+// tiles per camera.
 __global__ void __launch_bounds__(256)
 fused_emit_kernel(const uint2* __restrict__ rects, const uint32_t* __restrict__ cum, uint32_t total,
                   uint32_t ntx, uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                  uint32_t* __restrict__ overflow, R* __restrict__ recs) {
+                  uint32_t* __restrict__ overflow) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint32_t w0 = cum[i], cnt = cum[i + 1] - w0;
     if (cnt == 0) return;
-    recs[i].g = int32_t(w0);
     if (w0 + cnt > capacity) {
         *overflow = 1u;
         return;
@@ -449,28 +446,15 @@ fused_list_lengths_kernel(const uint32_t* __restrict__ tm, const uint32_t* __res
 }
 
 // One workgroup per camera-major list k = cam * nt + t (the lists of one XCD
-// are a contiguous band of cameras): the records of the list's intersections
-// with g = their emission index w0 + (ty - y0) (x1 - x0) + (tx - x0)
+// are a contiguous band of cameras): the list entries {record index,
+// emission index}, the emission index w0 + (ty - y0) (x1 - x0) + (tx - x0)
 // (row-major within the record's tile rectangle, the order fused_emit_kernel
-// wrote it in; w0 is the record's g). The packed lists are camera-major, the
-// reference's list order. Within each chunk of kPackChunk intersections the
-// records are ordered by the number of tile rows they reach (descending; ties
-// keep the primitive order), a stable counting sort in LDS: the backward's
-// waves then take 64 records of similar height and loop over no more rows
-// than their tallest record needs.
-// Streaming (non-temporal) store of one record: the packed list is written
-// once and read by the next kernel, so it should not evict the projected
-// records the pack kernel is gathering from L2.
-template <typename R>
-__device__ __forceinline__ void store_stream(R* dst, const R& v) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    static_assert(sizeof(R) % sizeof(f4) == 0, "record size must be a multiple of 16 bytes");
-    const f4* src = reinterpret_cast<const f4*>(&v);
-    f4* out = reinterpret_cast<f4*>(dst);
-#pragma unroll
-    for (size_t i = 0; i < sizeof(R) / sizeof(f4); ++i) __builtin_nontemporal_store(src[i], out + i);
-}
-
+// wrote it in; w0 = the record's first emission index). The lists are
+// camera-major, the reference's list layout. Within each chunk of kPackChunk
+// intersections the entries are ordered by the number of tile rows their
+// records reach (descending; ties keep the primitive order), a stable
+// counting sort in LDS: the backward's waves then take 64 records of similar
+// height and loop over no more rows than their tallest record needs.
 constexpr int kPackItems = 4;
 constexpr int kPackThreads = 256;
 constexpr int kPackChunk = kPackItems * kPackThreads;
@@ -478,12 +462,12 @@ constexpr int kPackWaves = kPackThreads / kWave;
 constexpr int kPackClasses = 34;  // rows 0..32 plus "not written"
 constexpr int kPackSlots = kPackItems * kPackWaves;
 
-template <typename R>
 __global__ void __launch_bounds__(kPackThreads)
-fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, uint32_t n_recs, uint32_t C,
-                  uint32_t nt, uint32_t ntx, uint32_t T, const uint32_t* __restrict__ vals,
-                  const uint32_t* __restrict__ tm, const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n,
-                  uint32_t capacity, R* __restrict__ packed, int32_t* __restrict__ tile_offsets) {
+fused_pack_kernel(const uint2* __restrict__ rects, const float2* __restrict__ reach,
+                  const uint32_t* __restrict__ cum, uint32_t n_recs, uint32_t C, uint32_t nt, uint32_t ntx,
+                  uint32_t T, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tm,
+                  const uint32_t* __restrict__ cm, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                  uint2* __restrict__ entries, int32_t* __restrict__ tile_offsets) {
     // cnt[class][item][wave] -> exclusive offsets in that order (class-major:
     // taller records first; within a class the chunk's own order)
     __shared__ uint32_t cnt[kPackClasses * kPackSlots];
@@ -505,17 +489,16 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
             const uint32_t r = r0 + q * kPackThreads + threadIdx.x;
             v[q] = (r < len && src + r < n_live) ? vals[src + r] : n_recs;
         }
-        R rec[kPackItems];
         uint2 rc[kPackItems];
+        uint32_t w0[kPackItems];
 #pragma unroll
         for (int q = 0; q < kPackItems; ++q) {
             cls[q] = skip_cls;
             if (v[q] < n_recs) {
-                rec[q] = recs[v[q]];
                 rc[q] = rects[v[q]];
-                int lo, hi;
-                row_span(rec[q], y_base, int(T), lo, hi);
-                cls[q] = T - uint32_t(hi - lo);
+                w0[q] = cum[v[q]];
+                const float2 e = reach[v[q]];
+                cls[q] = T - rows_reached(e.x, e.y, y_base, T);
             }
         }
         for (uint32_t i = threadIdx.x; i < uint32_t(kPackClasses * kPackSlots); i += kPackThreads) cnt[i] = 0u;
@@ -563,8 +546,7 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
             const uint32_t pos = r0 + cnt[cls[q] * kPackSlots + q * kPackWaves + wave] + rank[q];
             if (dst + pos >= capacity) continue;
             const uint32_t x0 = rc[q].x & 0xffffu, x1 = rc[q].x >> 16, y0 = rc[q].y & 0xffffu;
-            rec[q].g += int32_t((ty - y0) * (x1 - x0) + (tx - x0));
-            store_stream(packed + dst + pos, rec[q]);
+            entries[dst + pos] = make_uint2(v[q], w0[q] + (ty - y0) * (x1 - x0) + (tx - x0));
         }
         __syncthreads();  // cnt is reused by the next chunk
     }
@@ -582,9 +564,10 @@ fused_pack_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, u
 //   count: per range, the number of intersections in each bin (LDS counters);
 //   scan:  per (camera, bin) an exclusive scan over the ranges, then one over
 //          (camera, bin): every (range, bin) gets its first packed slot;
-//   place: the same walk, each (record, tile) pair written straight to its
-//          slot; pairs of one step that share a bin are ranked by lane
-//          (ballots on the bin's bits), steps in order.
+//   place: the same walk, each (record, tile) pair's list entry {record
+//          index, emission index} written straight to its slot; pairs of one
+//          step that share a bin are ranked by lane (ballots on the bin's
+//          bits), steps in order.
 // Everything is deterministic: counts do not depend on the order of the LDS
 // increments and the placement order is fixed.
 // ---------------------------------------------------------------------------
@@ -662,13 +645,12 @@ fused_bin_scan_ranges_kernel(uint32_t* __restrict__ range_counts, uint32_t C, ui
 }
 
 // grid (n_ranges, C), one wave; dynamic LDS: the n_bins next free slots.
-template <typename R>
 __global__ void __launch_bounds__(kWave)
-fused_bin_place_kernel(const R* __restrict__ recs, const uint2* __restrict__ rects, const float2* __restrict__ reach,
+fused_bin_place_kernel(const uint2* __restrict__ rects, const float2* __restrict__ reach,
                        const uint32_t* __restrict__ cum, uint32_t N, uint32_t range_len, uint32_t n_ranges,
                        uint32_t ntx, uint32_t T, uint32_t n_bins, uint32_t bin_bits,
                        const uint32_t* __restrict__ range_starts, const uint32_t* __restrict__ bin_starts,
-                       uint32_t capacity, R* __restrict__ packed) {
+                       uint32_t capacity, uint2* __restrict__ entries) {
     extern __shared__ uint32_t next_slot[];
     const uint32_t range = blockIdx.x, cam = blockIdx.y, lane = threadIdx.x;
     const uint32_t* rs = range_starts + (size_t(cam) * n_ranges + range) * n_bins;
@@ -684,12 +666,7 @@ fused_bin_place_kernel(const R* __restrict__ recs, const uint2* __restrict__ rec
         if (n < n1) sr = load_span(rects, reach, idx);
         RectWalk w;
         w.start(sr, ntx, T);
-        R rec = {};
-        uint32_t g = 0;
-        if (w.cnt) {
-            rec = recs[idx];
-            g = cum[idx];
-        }
+        const uint32_t g = w.cnt ? cum[idx] : 0u;
         for (uint32_t k = 0; __ballot(k < w.cnt); ++k) {
             const bool act = k < w.cnt;
             const uint32_t bin = act ? w.bin() : 0u;
@@ -704,11 +681,7 @@ fused_bin_place_kernel(const R* __restrict__ recs, const uint2* __restrict__ rec
                 // every lane reads its bin's slot before the group's last lane advances it
                 const uint32_t pos = next_slot[bin] + uint32_t(__popcll(same & below));
                 if ((same >> lane) == 1u) next_slot[bin] = pos + 1u;
-                if (pos < capacity) {
-                    R out = rec;
-                    out.g = int32_t(g + k);
-                    packed[pos] = out;
-                }
+                if (pos < capacity) entries[pos] = make_uint2(uint32_t(idx), g + k);
                 w.advance(sr, ntx, T);
             }
         }
@@ -778,11 +751,22 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// List entry i ({record index, emission index}), or an empty entry past the list.
+__device__ __forceinline__ uint2 load_entry(const uint2* __restrict__ entries, int32_t i, int32_t hi) {
+    uint2 e = make_uint2(0xffffffffu, 0u);
+    if (i < hi) e = entries[i];
+    return e;
+}
+
+// The record an entry names, with g = the entry's emission index (g = -1: none).
 template <typename R>
-__device__ __forceinline__ R load_splat(const R* __restrict__ packed, int32_t i, int32_t hi) {
+__device__ __forceinline__ R gather_splat(const R* __restrict__ recs, uint2 e) {
     R s = {};
     s.g = -1;
-    if (i < hi) s = packed[i];
+    if (e.x != 0xffffffffu) {
+        s = recs[e.x];
+        s.g = int32_t(e.y);
+    }
     return s;
 }
 
@@ -797,7 +781,8 @@ __device__ __forceinline__ void settle(const R& s) {
 
 template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+                        const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
                         uint32_t H, float* __restrict__ out, float* __restrict__ out_var) {
@@ -829,7 +814,11 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
         acc_var[k] = 0.f;
     }
     R* slot = stage[wave];
-    R next = load_splat(packed, lo + int32_t(lane), hi);
+    // two-stage prefetch: a batch's records are gathered one batch ahead, their
+    // list entries two batches ahead
+    uint2 ahead = load_entry(entries, lo + int32_t(lane), hi);
+    R next = gather_splat(recs, ahead);
+    ahead = load_entry(entries, lo + kStride + int32_t(lane), hi);
     const float y_base = float(ty * T);
     for (int32_t base = lo; base < hi; base += kStride) {
         // row groups of the record this lane stages (lane rows are row0 + STEP k,
@@ -842,7 +831,8 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
         wave_lds_sync();  // every lane is done reading the previous batch
         slot[lane] = next;
         wave_lds_sync();
-        next = load_splat(packed, base + kStride + int32_t(lane), hi);
+        next = gather_splat(recs, ahead);
+        ahead = load_entry(entries, base + 2 * kStride + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         // LDS reads pipelined one record ahead in two registers that alternate
         // roles (no copies between them); the batch holds n >= 1 records
@@ -908,7 +898,8 @@ fused_raster_fwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
 // ---------------------------------------------------------------------------
 template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_bwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t* __restrict__ offsets,
+fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+                        const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
                         uint32_t H, const float* __restrict__ v_img, const float* __restrict__ v_var, int detach,
@@ -937,13 +928,17 @@ fused_raster_bwd_kernel(const Rec<BETA, VAR>* __restrict__ packed, const int32_t
     constexpr int32_t kStride = WAVES * kWave;
     using R = Rec<BETA, VAR>;
     int32_t chunk = start + int32_t(wave) * kWave;
-    R next = load_splat(packed, chunk + int32_t(lane), end);
+    // records gathered a chunk ahead, list entries two chunks ahead
+    uint2 ahead = load_entry(entries, chunk + int32_t(lane), end);
+    R next = gather_splat(recs, ahead);
+    ahead = load_entry(entries, chunk + kStride + int32_t(lane), end);
     for (; chunk < end; chunk += kStride) {
         const int32_t i = chunk + int32_t(lane);
         const bool active = i < end;
         const R s = next;
         settle(s);
-        next = load_splat(packed, chunk + kStride + int32_t(lane), end);  // next chunk, a chunk ahead
+        next = gather_splat(recs, ahead);
+        ahead = load_entry(entries, chunk + 2 * kStride + int32_t(lane), end);
         const float dx0 = x0 - s.mx;
         float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f, vq = 0.f;
         float bm1 = 0.f, kvar = 0.f;
@@ -1213,7 +1208,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
 // Forward scratch layout (released by the caller after the forward)
 // ---------------------------------------------------------------------------
 struct FusedScratch {
-    size_t recs, rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, tm, lengths, cm,
+    size_t rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, tm, lengths, cm,
         tile_scan, bytes;
     // direct placement (place = true) or sort + pack
     bool place;
@@ -1234,6 +1229,14 @@ static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 constexpr int kKindBeta = 1, kKindVar = 2;
 static size_t record_bytes(int kind) { return (kind & (kKindBeta | kKindVar)) ? sizeof(BetaSplat) : sizeof(Splat); }
 
+// The packed buffer the forward leaves for the raster kernels and the
+// backward: `capacity` list entries {record index, emission index} (uint2),
+// then the C*N projected records.
+static size_t entries_bytes(uint32_t capacity) { return al(size_t(capacity) * sizeof(uint2)); }
+static size_t packed_bytes(uint32_t C, uint32_t N, uint32_t capacity, int kind) {
+    return entries_bytes(capacity) + al(size_t(C) * N * record_bytes(kind));
+}
+
 // Largest bin table of the direct placement (its waves hold one table each in
 // LDS, so larger tables cost occupancy; larger tile grids sort and pack).
 constexpr size_t kPlaceLdsBytes = 16384;
@@ -1248,13 +1251,12 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
     L.order_sort_n = o; o += al(L.order_sort_n_bytes);
     L.order_sorted_keys = o; o += al(size_t(N) * 4 + 4);
     L.params = o; o += al(size_t(N) * kPermutedFloats * sizeof(float));
-    L.recs = o; o += al(CN * record_bytes(kind));
     L.rects = o; o += al(CN * sizeof(uint2));
+    L.reach = o; o += al(CN * sizeof(float2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
     const uint64_t nt = C ? n_tiles / C : 0;
     L.place = C > 0 && nt * kBinClasses * sizeof(uint32_t) <= kPlaceLdsBytes;
     if (L.place) {
-        L.reach = o; o += al(CN * sizeof(float2));
         L.n_bins = uint32_t(nt) * kBinClasses;
         L.bin_bits = 32u - uint32_t(__builtin_clz(L.n_bins - 1u));
         const uint64_t per = (CN + kPlaceWaves - 1) / kPlaceWaves;
@@ -1302,6 +1304,10 @@ extern "C" size_t r2s_fused_scratch_bytes(uint32_t C, uint32_t N, uint32_t width
 
 extern "C" size_t r2s_fused_record_bytes(int kind) { return record_bytes(kind); }
 
+extern "C" size_t r2s_fused_packed_bytes(uint32_t C, uint32_t N, uint32_t capacity, int kind) {
+    return packed_bytes(C, N, capacity, kind);
+}
+
 static int fused_check(uint32_t C, uint32_t N, uint32_t ntx, uint32_t nty, uint32_t capacity) {
     R2S_REQUIRE(uint64_t(C) * N < (1ull << 31), "C*N exceeds the int32 index range");
     R2S_REQUIRE(uint64_t(C) * ntx * nty < (1ull << 32), "C*n_tiles exceeds the key range");
@@ -1342,8 +1348,9 @@ struct LaunchFwd {
                     const uint32_t* order, const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty,
                     uint32_t W, uint32_t H, float* image, float* image_var) {
         fused_raster_fwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            static_cast<const Rec<BETA, VAR>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W,
-            H, image, image_var);
+            static_cast<const uint2*>(packed),
+            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, image, image_var);
     }
 };
 
@@ -1354,8 +1361,9 @@ struct LaunchBwd {
                     uint32_t W, uint32_t H, const float* v_image, const float* v_image_var, int detach,
                     float* isect_grads) {
         fused_raster_bwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
-            static_cast<const Rec<BETA, VAR>*>(packed), offsets, order, counters, capacity, n_tiles, ntx, nty, W,
-            H, v_image, v_image_var, detach, isect_grads);
+            static_cast<const uint2*>(packed),
+            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, v_image, v_image_var, detach, isect_grads);
     }
 };
 
@@ -1373,6 +1381,8 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     R2S_REQUIRE(scratch_bytes >= L.bytes, "fused scratch too small: %zu < %zu", scratch_bytes, L.bytes);
     const uint32_t CN = C * N, n_tiles = C * ntx * nty;
     if (int rc = fill_zero<uint32_t>(counters, 2, stream)) return rc;
+    uint2* entries = static_cast<uint2*>(packed);
+    R* recs = at<R>(packed, entries_bytes(capacity));
     uint32_t* cum = record_offsets;
     uint32_t* perm = record_offsets + size_t(CN) + 1;  // the primitive order (input index per position)
     float* pp = at<float>(scratch, L.params);
@@ -1401,8 +1411,8 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         dim3 grid(ceil_div(N, 256), C);
         fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
             p_means, p_quats, p_scales, p_dens, p_betas, p_ratio, p_unc, viewmats, Ks, width, height, eps,
-            radius_clip, tile_size, ntx, nty, N, at<R>(scratch, L.recs), at<uint2>(scratch, L.rects),
-            L.place ? at<float2>(scratch, L.reach) : nullptr, cum);
+            radius_clip, tile_size, ntx, nty, N, recs, at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach),
+            cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
@@ -1425,10 +1435,10 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         if (int rc = check_launch("fused_bin_scan_ranges")) return rc;
         if (int rc = scan_exclusive(totals, n_cols, totals, nullptr, at<void>(scratch, L.bin_scan), stream)) return rc;
         if (CN > 0) {
-            fused_bin_place_kernel<R><<<grid, kWave, lds, stream>>>(
-                at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach), cum, N,
+            fused_bin_place_kernel<<<grid, kWave, lds, stream>>>(
+                at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach), cum, N,
                 L.range_len, L.n_ranges, ntx, tile_size, L.n_bins, L.bin_bits, range_counts, totals, capacity,
-                static_cast<R*>(packed));
+                entries);
             if (int rc = check_launch("fused_bin_place")) return rc;
         }
         fused_bin_tile_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(totals, n_tiles, tile_offsets);
@@ -1436,9 +1446,9 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     } else {
     void* sort_ws = at<void>(scratch, L.sort);
     if (CN > 0) {
-        fused_emit_kernel<R><<<ceil_div(CN, 256), 256, 0, stream>>>(
+        fused_emit_kernel<<<ceil_div(CN, 256), 256, 0, stream>>>(
             at<uint2>(scratch, L.rects), cum, CN, ntx, capacity, sort_keys_buffer(sort_ws),
-            sort_vals_buffer(sort_ws, capacity), counters + 1, at<R>(scratch, L.recs));
+            sort_vals_buffer(sort_ws, capacity), counters + 1);
         if (int rc = check_launch("fused_emit")) return rc;
     }
     uint32_t* keys = at<uint32_t>(scratch, L.keys);
@@ -1458,9 +1468,9 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
                                 stream))
         return rc;
     const uint32_t pack_blocks = (n_tiles + 7u) & ~7u;  // a multiple of the 8 XCDs
-    fused_pack_kernel<R><<<pack_blocks, kPackThreads, 0, stream>>>(
-        at<R>(scratch, L.recs), at<uint2>(scratch, L.rects), CN, C, ntx * nty, ntx, tile_size, vals, tm, cm, counters,
-        capacity, static_cast<R*>(packed), tile_offsets);
+    fused_pack_kernel<<<pack_blocks, kPackThreads, 0, stream>>>(
+        at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach), cum, CN, C, ntx * nty, ntx, tile_size, vals, tm,
+        cm, counters, capacity, entries, tile_offsets);
     if (int rc = check_launch("fused_pack")) return rc;
     }
     if (tile_order) {

@@ -126,7 +126,6 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
                            "same shapes first")
     cap = int(capacity or CAPACITY.get(key, C, N))
     ntx, nty = -(-width // tile_size), -(-height // tile_size)
-    rec = be.lib().r2s_fused_record_bytes(kind)
     offsets = torch.empty((C, nty, ntx), dtype=torch.int32, device=device)
     tile_order = torch.empty((C * nty * ntx,), dtype=torch.int32, device=device)
     # per-record intersection offsets (C*N + 1) followed by the primitive order (N)
@@ -137,7 +136,8 @@ def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, r
         n_isects = -1
         while True:
             scratch = _scratch(C, N, width, height, tile_size, cap, kind, device)
-            packed = torch.empty((cap * rec,), dtype=torch.uint8, device=device)
+            packed = torch.empty((be.lib().r2s_fused_packed_bytes(C, N, cap, kind),), dtype=torch.uint8,
+                                 device=device)
             with stage("project_bin"):
                 call("r2s_fused_prepare", ptr(means), ptr(quats), ptr(scales), ptr(densities), ptr(beta), ptr(ratio),
                      ptr(uncertainty), ptr(viewmats), ptr(Ks), C, N, width, height, tile_size, float(eps), float(radius_clip), cap, ptr(scratch),

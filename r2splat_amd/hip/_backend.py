@@ -56,6 +56,7 @@ SIGNATURES = {
     "r2s_voxelize_to_voxels_bwd_per_gaussian": [I32, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, P, P, P, P, P, P, P],
     "r2s_fused_scratch_bytes": [U32, U32, U32, U32, U32, U32, I32],
     "r2s_fused_record_bytes": [I32],
+    "r2s_fused_packed_bytes": [U32, U32, U32, I32],
     "r2s_fused_forward": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P, P,
                           P],
     "r2s_fused_backward": [P, P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, U32, P, P, P, P, P, P, P, SZ, P,
@@ -88,6 +89,7 @@ _RESTYPES = {
     "r2s_sort_workspace_bytes": SZ,
     "r2s_fused_scratch_bytes": SZ,
     "r2s_fused_record_bytes": SZ,
+    "r2s_fused_packed_bytes": SZ,
     "r2s_fused_param_workspace_bytes": SZ,
     "r2s_voxel_partials_bytes": SZ,
     "r2s_mse_partials": SZ,

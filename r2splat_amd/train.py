@@ -59,7 +59,7 @@ class Workspace:
         lib = be.lib()
         changed = False
         for name, nbytes in (("scratch", lib.r2s_fused_scratch_bytes(C, N, W, H, tile_size, capacity, beta)),
-                             ("packed", capacity * lib.r2s_fused_record_bytes(beta)),
+                             ("packed", lib.r2s_fused_packed_bytes(C, N, capacity, beta)),
                              ("isect_grads", capacity * 8 * 4),
                              ("param_ws", lib.r2s_fused_param_workspace_bytes(C, N))):
             t, grown = self._grow(getattr(self, name), nbytes, self.device)

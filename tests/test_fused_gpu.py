@@ -115,9 +115,12 @@ def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
     _, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 80, 96, 1e-8, clip, beta=sc.beta)
     n, C, N, ntx, nty = f.n_isects, 5, 3000, 5, 6
     width = 12 if use_beta else 8
-    words = f.packed.view(torch.int32)[: n * width].reshape(n, width).cpu()
-    g = words[:, 7].numpy()
-    mean = words[:, :2].view(torch.float32).numpy()
+    # packed = list entries {record index, emission index}, then the C*N records
+    entries = f.packed[: n * 8].view(torch.int32).reshape(n, 2).cpu().numpy()
+    rec0 = -(-f.capacity * 8 // 256) * 256
+    records = f.packed[rec0: rec0 + C * N * width * 4].view(torch.float32).reshape(C * N, width).cpu().numpy()
+    g = entries[:, 1]
+    mean = records[entries[:, 0], :2]
     assert np.array_equal(np.sort(g), np.arange(n))
     # records follow the primitive order stored after the offsets
     perm = f.record_offsets[C * N + 1:].long().cpu().numpy()
@@ -132,6 +135,7 @@ def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
     ends = np.append(offsets[1:], n)
     lists = np.repeat(np.arange(offsets.size), ends - offsets)
     assert np.array_equal(keys[g], lists)  # every record sits in the list of its own tile
+    assert np.array_equal(entries[:, 0], owners[g])  # and names the record that emitted it
     np.testing.assert_allclose(mean, m2n.reshape(-1, 2)[owners[g]], rtol=1e-5, atol=1e-3)
 
 
