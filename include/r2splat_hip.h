@@ -322,7 +322,7 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * (tiles by decreasing intersection count, the order the raster kernels
  * walk them in; may be null, then natural order), record_offsets uint32
  * [C*N + 1 + N] and counters uint32 [2] persist from the forward to the
- * backward. The forward processes primitives in the order of a 30-bit
+ * backward. The forward processes primitives in the order of a 15-bit
  * Morton code of their means (ties in input order): record_offsets[C*N + 1
  * + p] is the input index of position p, and record_offsets[c*N + p] the
  * first intersection of (camera c, position p) in emission order. The

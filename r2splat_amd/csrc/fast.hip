@@ -158,9 +158,9 @@ __device__ __forceinline__ uint32_t rows_reached(float my, float ext, float y_ba
 }
 
 // ---------------------------------------------------------------------------
-// Primitive order. The forward processes primitives in the order of a 30-bit
-// Morton code of their means (10 bits per axis over the means' bounding box;
-// ties keep the input order): consecutive primitives then project to nearby
+// Primitive order. The forward processes primitives in the order of a 15-bit
+// Morton code of their means (5 bits per axis over the means' bounding box,
+// 32^3 cells; ties keep the input order; two radix passes): consecutive primitives then project to nearby
 // tiles in every view, so the binning's gathers and writes stay local. The
 // records, the per-record intersection offsets and the emission order all
 // follow that order; the parameter VJP maps back to the input order.
@@ -222,7 +222,9 @@ morton_bounds_kernel(const float* __restrict__ means, uint32_t N, uint32_t* __re
     }
 }
 
-__device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {  // 10 bits -> every third bit
+constexpr uint32_t kMortonAxisBits = 5;
+
+__device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {  // up to 10 bits -> every third bit
     v &= 0x3ffu;
     v = (v | (v << 16)) & 0x030000ffu;
     v = (v | (v << 8)) & 0x0300f00fu;
@@ -242,7 +244,8 @@ morton_keys_kernel(const float* __restrict__ means, uint32_t N, const uint32_t* 
         // no finite mean: lo = NaN and every key 0
         const float lo = order_float(bounds[k]), ext = order_float(bounds[3 + k]) - lo;
         const float t = ext > 0.f ? (means[3 * size_t(n) + k] - lo) / ext : 0.f;
-        const float q = fminf(fmaxf(t * 1024.f, 0.f), 1023.f);  // NaN -> 0
+        constexpr float kCells = float(1u << kMortonAxisBits);
+        const float q = fminf(fmaxf(t * kCells, 0.f), kCells - 1.f);  // NaN -> 0
         key |= spread_bits3(uint32_t(q)) << (2 - k);
     }
     keys[n] = key;
@@ -1399,8 +1402,8 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         morton_keys_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, N, bounds,
                                                                  sort_keys_buffer(ows), sort_vals_buffer(ows, N));
         if (int rc = check_launch("morton_keys")) return rc;
-        if (int rc = sort_pairs_u32(ows, L.order_sort_n_bytes, N, nullptr, 30, at<uint32_t>(scratch, L.order_sorted_keys),
-                                    perm, stream))
+        if (int rc = sort_pairs_u32(ows, L.order_sort_n_bytes, N, nullptr, 3 * kMortonAxisBits,
+                                    at<uint32_t>(scratch, L.order_sorted_keys), perm, stream))
             return rc;
         permute_params_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
             perm, N, means, quats, scales, densities, BETA ? betas : nullptr, BETA ? beta_ratio : nullptr,
