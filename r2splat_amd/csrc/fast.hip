@@ -899,8 +899,15 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
 // v + 2 q alpha v_var (v alone when the variance is detached) and the partial
 // also carries v_q = sum v_var alpha^2 (its eighth slot).
 // ---------------------------------------------------------------------------
+// Waves per SIMD the backward asks the register allocator for: the Gaussian
+// 16-pixel-tile kernel fits 72 VGPRs without spills, 7 waves per SIMD instead
+// of 6 (its 75-VGPR default); the other variants keep the default (forcing
+// the next occupancy step spills them).
+template <int T, bool BETA, bool VAR>
+constexpr int bwd_waves_per_eu() { return (T == 16 && !BETA && !VAR) ? 7 : 1; }
+
 template <int T, int WAVES, bool BETA, bool VAR>
-__global__ void __launch_bounds__(WAVES * kWave)
+__global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
 fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
                         const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
