@@ -373,8 +373,8 @@ int r2s_fused_backward(const float* means, const float* quats, const float* scal
                        float* v_quats, float* v_scales, float* v_densities, float* v_betas,
                        float* v_uncertainties, r2s_stream_t stream);
 
-/* The same two calls split into stages: forward = prepare (projection,
- * binning, sort, offsets, record packing) + render (rasterization);
+/* The same two calls split into stages: forward = prepare (primitive order,
+ * projection, binning into tile lists, offsets) + render (rasterization);
  * backward = raster_vjp (rasterization VJP, one partial per intersection) +
  * param_vjp (per-primitive sums, conic/mu, projection and parameter-transform
  * VJPs, summed over cameras). */

@@ -882,8 +882,9 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
 // Backward: one workgroup of WAVES waves per T x T tile; wave k takes chunks
 // k, k + WAVES, ... of 64 records, one lane per record. The tile's adjoint
 // image sits in LDS and is read row by row; each lane walks only the rows its
-// record can reach (row_span; the pack step groups records of similar height
-// into the same chunks) and sums its own primitive's moments over them
+// record can reach (row_span; the lists order records by height class, so a
+// chunk's records reach similar numbers of rows) and sums its own primitive's
+// moments over them
 // without any cross-lane reduction:
 //   g = v e,  M0 = sum g, Mx = sum g dx, My = sum g dy, Mxx, Mxy, Myy
 // with dx = px - mx (the moments are summed directly in dx: expanding them

@@ -1,8 +1,8 @@
 """Fused float32 Gaussian / Beta rasterization (8, 16 or 32 pixel tiles) on the HIP library.
 
-``FusedForward`` holds what the backward needs (per-intersection records,
-tile offsets, per-(camera, primitive) intersection offsets, device counters)
-for one forward call. The backward stores one partial per intersection and
+``FusedForward`` holds what the backward needs (the tile lists' entries and
+the projected records, tile offsets, per-(camera, position) intersection
+offsets and the primitive order, device counters) for one forward call. The backward stores one partial per intersection and
 sums them per primitive in a fixed order, so it is bitwise reproducible. Capacity for the
 intersection list is sized from the previous call with the same shapes
 (``CapacityCache``) and grown on overflow, so steady-state calls perform no
@@ -103,7 +103,7 @@ def beta_factors(beta: torch.Tensor):
 
 def prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, *, capacity=None,
             check=None, tile_size: int = TILE, beta=None, factors=None, uncertainty=None) -> FusedForward:
-    """Projection, binning, sort and record packing (everything but the image). float32, one device.
+    """Primitive order, projection and binning into tile lists (everything but the image). float32, one device.
 
     ``check`` (default: unless the stream is being captured into a graph)
     reads the overflow flag and re-runs with more capacity. Under capture the
