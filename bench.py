@@ -41,7 +41,7 @@ FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
 # --no-graph`; profiles/raster_100k_64x256_occupancy_valu.md). The kernels
 # evaluate only the rows (backward) and row groups (forward) each record can
 # reach, so they issue far fewer operations than pairs x FLOP_PER_PAIR.
-VALU_INSTS = {"rasterize_fwd": 7.370e8, "rasterize_bwd": 8.570e8}
+VALU_INSTS = {"rasterize_fwd": 7.370e8, "rasterize_bwd": 8.573e8}
 SIMDS = 256 * 4          # compute units x SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
 # HBM bytes per launch of the raster kernels on this workload, from two
@@ -51,7 +51,7 @@ CLOCK_HZ = 2.4e9         # peak engine clock
 # plain 256 MiB device copy moves to what the same counter reads for it
 # (FETCH 131,089 KiB -> x2.0; WRITE 262,144 KiB -> x1.0). Summary and
 # commands: profiles/raster_100k_64x256_hbm_traffic.md.
-PMC_KIB = {"rasterize_fwd": (1_343_548, 16_444), "rasterize_bwd": (1_364_330, 862_850)}
+PMC_KIB = {"rasterize_fwd": (1_344_007, 16_445), "rasterize_bwd": (1_367_643, 862_751)}
 PMC_SCALE = (262_144 / 131_089, 262_144 / 262_144)
 
 
