@@ -57,11 +57,20 @@ def test_fused_matches_staged_ops_path(device):
         assert O.relative_error(getattr(a, name), getattr(b, name)) <= 2e-5, name
 
 
-def test_fused_forward_is_deterministic_and_grows_capacity(device):
+def _count_grows(monkeypatch):
+    calls = []
+    grow = fused.CAPACITY.grow
+    monkeypatch.setattr(fused.CAPACITY, "grow", lambda key, needed: calls.append(needed) or grow(key, needed))
+    return calls
+
+
+def test_fused_forward_is_deterministic_and_grows_capacity(device, monkeypatch):
     sc = workloads.config0(device=device)
-    key = (device, 4, 1000, 64, 64, 16)
+    key = (sc.means.device, 4, 1000, 64, 64, 16, 0)  # (device, C, N, H, W, tile, record kind)
     fused.CAPACITY._cap[key] = 1024  # force an overflow on the next call
+    grows = _count_grows(monkeypatch)
     img1, f1 = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 64, 64, 1e-8, 3.0)
+    assert grows == [f1.n_isects]  # the first pass overflowed and the capacity was regrown
     assert f1.capacity >= f1.n_isects > 1024
     assert int(f1.counters[1].item()) == 0
     img2, f2 = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 64, 64, 1e-8, 3.0)
@@ -69,6 +78,29 @@ def test_fused_forward_is_deterministic_and_grows_capacity(device):
     assert f2.n_isects == f1.n_isects
     counts, ids, flat = ops.isect_tiles(*_proj(sc), 16, 4, 4)
     assert ids.numel() == f1.n_isects
+
+
+def test_sort_path_overflow_is_flagged_and_regrown(device, monkeypatch):
+    """The sort + pack binning (500 tiles of 8 pixels per camera, more bins
+    than the direct-placement table holds) under a forced capacity overflow:
+    the forward flags it, regrows, and renders bitwise the same image as a
+    call that never overflowed."""
+    sc = workloads.scene(2000, 2, 160, 200, scale_range=(0.01, 0.06), seed=12, device=device)
+    args = (*sc.params(), sc.viewmats, sc.Ks, 200, 160, 1e-8, 3.0)
+    ref, fr = fused.forward(*args, tile_size=8, capacity=1 << 22)
+    assert int(fr.counters[1].item()) == 0 and fr.n_isects > 256
+    key = (sc.means.device, 2, 2000, 160, 200, 8, 0)  # (device, C, N, H, W, tile, record kind)
+    fused.CAPACITY._cap[key] = 256  # force an overflow on the next call
+    grows = _count_grows(monkeypatch)
+    img, f = fused.forward(*args, tile_size=8)
+    assert grows == [fr.n_isects]  # the first pass overflowed and the capacity was regrown
+    assert f.capacity >= f.n_isects == fr.n_isects > 256
+    assert int(f.counters[1].item()) == 0
+    assert torch.equal(img, ref)
+    # a capacity given explicitly and too small raises the device flag and renders nothing
+    prep = fused.prepare(*args, tile_size=8, capacity=256, check=False)
+    assert int(prep.counters[1].item()) == 1
+    assert not bool(fused.render(prep, 2, sc.means.device).any())
 
 
 def _proj(sc):
