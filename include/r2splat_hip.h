@@ -37,7 +37,10 @@ typedef struct ihipStream_t* r2s_stream_t; /* identical to hipStream_t */
 #define R2S_F32 0
 #define R2S_F64 1
 
-/* Library identification and error reporting. */
+/* Library identification and error reporting. Version 2: the fused entry
+ * points take the record kind (Beta, uncertainty) and the variance buffers,
+ * and `packed` holds tile-list entries followed by the projected records
+ * (r2s_fused_packed_bytes). */
 int r2s_abi_version(void);
 const char* r2s_last_error(void);
 

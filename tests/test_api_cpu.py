@@ -75,10 +75,30 @@ def test_ctypes_table_covers_the_header():
 
 def test_library_version_and_workspace_queries_need_no_gpu():
     lib = _backend.lib(require_gpu=False)
-    assert lib.r2s_abi_version() == 1
+    assert lib.r2s_abi_version() == 2
     assert lib.r2s_scan_workspace_bytes(0) > 0
     big = lib.r2s_sort_workspace_bytes(10**7, 15)
     assert big >= 4 * 4 * 10**7
+
+
+def test_fused_size_queries_need_no_gpu():
+    """The packed buffer: `capacity` 8-byte list entries (rounded up to 256
+    bytes), then C*N records of 32 bytes (Gaussian) or 48 bytes (Beta or
+    uncertainty); the scratch grows with the intersection capacity only on
+    the sort path (tile grids too large for direct placement)."""
+    lib = _backend.lib(require_gpu=False)
+    assert [lib.r2s_fused_record_bytes(k) for k in range(4)] == [32, 48, 48, 48]
+    for C, N, cap, kind in ((4, 1000, 1000, 0), (64, 100_000, 30_000_000, 0), (3, 7, 5, 3)):
+        entries = -(-cap * 8 // 256) * 256
+        records = -(-C * N * lib.r2s_fused_record_bytes(kind) // 256) * 256
+        assert lib.r2s_fused_packed_bytes(C, N, cap, kind) == entries + records
+    # 256 tiles of 16 pixels per camera: direct placement, no capacity-sized sort buffers
+    small = lib.r2s_fused_scratch_bytes(64, 100_000, 256, 256, 16, 10**6, 0)
+    assert lib.r2s_fused_scratch_bytes(64, 100_000, 256, 256, 16, 10**8, 0) == small
+    # 1024 tiles of 16 pixels per camera: sort + pack, keys and values per intersection
+    a = lib.r2s_fused_scratch_bytes(4, 1000, 512, 512, 16, 10**6, 0)
+    b = lib.r2s_fused_scratch_bytes(4, 1000, 512, 512, 16, 2 * 10**6, 0)
+    assert b - a >= 8 * 10**6
 
 
 @pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
