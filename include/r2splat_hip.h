@@ -326,7 +326,8 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * walk them in; may be null, then natural order), record_offsets uint32
  * [C*N + 1 + N] and counters uint32 [2] persist from the forward to the
  * backward. The forward processes primitives in the order of a 15-bit
- * Morton code of their means (ties in input order): record_offsets[C*N + 1
+ * Morton code of their means (its top 8 bits for up to 2^17 primitives;
+ * ties in input order): record_offsets[C*N + 1
  * + p] is the input index of position p, and record_offsets[c*N + p] the
  * first intersection of (camera c, position p) in emission order. The
  * gradients come back in the input order. counters[0]

@@ -160,7 +160,8 @@ __device__ __forceinline__ uint32_t rows_reached(float my, float ext, float y_ba
 // ---------------------------------------------------------------------------
 // Primitive order. The forward processes primitives in the order of a 15-bit
 // Morton code of their means (5 bits per axis over the means' bounding box,
-// 32^3 cells; ties keep the input order; two radix passes): consecutive primitives then project to nearby
+// 32^3 cells; ties keep the input order; two radix passes), or of its top 8
+// bits for sets of up to 2^17 primitives (one pass): consecutive primitives then project to nearby
 // tiles in every view, so the binning's gathers and writes stay local. The
 // records, the per-record intersection offsets and the emission order all
 // follow that order; the parameter VJP maps back to the input order.
@@ -223,6 +224,7 @@ morton_bounds_kernel(const float* __restrict__ means, uint32_t N, uint32_t* __re
 }
 
 constexpr uint32_t kMortonAxisBits = 5;
+constexpr uint32_t kMortonCoarseMax = 1u << 17;  // primitive sets sorted by the top 8 code bits
 
 __device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {  // up to 10 bits -> every third bit
     v &= 0x3ffu;
@@ -235,7 +237,7 @@ __device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {  // up to 10 bits
 
 __global__ void __launch_bounds__(256)
 morton_keys_kernel(const float* __restrict__ means, uint32_t N, const uint32_t* __restrict__ bounds,
-                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                   uint32_t drop, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     uint32_t key = 0;
@@ -248,7 +250,7 @@ morton_keys_kernel(const float* __restrict__ means, uint32_t N, const uint32_t* 
         const float q = fminf(fmaxf(t * kCells, 0.f), kCells - 1.f);  // NaN -> 0
         key |= spread_bits3(uint32_t(q)) << (2 - k);
     }
-    keys[n] = key;
+    keys[n] = key >> drop;
     vals[n] = n;
 }
 
@@ -1407,10 +1409,15 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         morton_bounds_kernel<<<std::min(ceil_div(N, kOrderThreads), 256u), kOrderThreads, 0, stream>>>(means, N,
                                                                                                        bounds);
         if (int rc = check_launch("morton_bounds")) return rc;
-        morton_keys_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, N, bounds,
+        // small sets sort by the code's top 8 bits only (one radix pass; 8 x 8 x 4
+        // cells still keep a range of neighbouring primitives on a few tiles),
+        // larger ones by all 15 (two passes; coarser cells cost the raster
+        // kernels' gather locality there)
+        const uint32_t code_bits = N <= kMortonCoarseMax ? 8u : 3u * kMortonAxisBits;
+        morton_keys_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, N, bounds, 3u * kMortonAxisBits - code_bits,
                                                                  sort_keys_buffer(ows), sort_vals_buffer(ows, N));
         if (int rc = check_launch("morton_keys")) return rc;
-        if (int rc = sort_pairs_u32(ows, L.order_sort_n_bytes, N, nullptr, 3 * kMortonAxisBits,
+        if (int rc = sort_pairs_u32(ows, L.order_sort_n_bytes, N, nullptr, code_bits,
                                     at<uint32_t>(scratch, L.order_sorted_keys), perm, stream))
             return rc;
         permute_params_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(

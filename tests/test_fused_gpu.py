@@ -117,7 +117,7 @@ def test_fused_tile_offsets_match_operator_binning(device):
     off = ops.isect_offset_encode(ids, 5, 5, 6)
     assert f.n_isects == ids.numel()
     assert torch.equal(f.offsets, off)
-    # the primitive order (after the offsets): a permutation, sorted by the 15-bit Morton code of the means
+    # the primitive order (after the offsets): a permutation, sorted by the Morton code of the means
     CN = counts.numel()
     perm = f.record_offsets[CN + 1:].long()
     assert torch.equal(torch.sort(perm).values, torch.arange(3000, device=device))
@@ -127,7 +127,11 @@ def test_fused_tile_offsets_match_operator_binning(device):
     for bit in range(5):
         for k in range(3):
             code |= ((q[:, k] >> bit) & 1) << (3 * bit + 2 - k)
+    code = code >> 7  # sets of up to 2^17 primitives sort by the code's top 8 bits
     assert bool((code[perm][1:] >= code[perm][:-1]).all())
+    # ties keep the input order (the sort is stable)
+    tie = code[perm][1:] == code[perm][:-1]
+    assert bool((perm[1:][tie] > perm[:-1][tie]).all())
     # per-(camera, position) intersection offsets: exclusive scan of the tile-rectangle areas in that order
     flat = counts[:, perm].reshape(-1).long()
     expect = torch.zeros(flat.numel() + 1, dtype=torch.long, device=device)
