@@ -129,6 +129,7 @@ class _Live:
     admitted: float
     issued: int = 0
     nbytes: int = 0
+    stalled: bool = False   # its capacity could not grow yet (device memory); retried at each sync
 
 
 @dataclass
@@ -177,9 +178,13 @@ class FitServer:
         return int(16 * 11 * req.n + 12 * req.views * req.side * req.side + 100 * isects + 48 * req.views * req.n)
 
     def used_bytes(self) -> int:
-        return sum(l.nbytes for l in self.live)
+        """Current footprint of the live jobs (their workspaces grow when their
+        intersection counts do)."""
+        return sum(l.fit.memory_bytes() for l in self.live)
 
     def _admit(self, req: FitRequest, now: float) -> bool:
+        if any(l.stalled for l in self.live):  # a live job is waiting for memory
+            return False
         if self.used_bytes() + self.estimate_bytes(req) > self.budget:
             return False
         job = build_job(req, self.device)
@@ -190,6 +195,18 @@ class FitServer:
         fit.capture()
         self.live.append(_Live(req, fit, now, 0, fit.memory_bytes()))
         return True
+
+    def _sync(self, l: _Live):
+        """Synchronises one job; when growing its capacity runs out of device
+        memory the job stalls (its overflowed steps stay skipped and are
+        replayed by a later sync) and the allocator's cache is released, so
+        that jobs finishing meanwhile make room."""
+        try:
+            l.fit.sync()
+            l.stalled = False
+        except torch.OutOfMemoryError:
+            l.stalled = True
+            torch.cuda.empty_cache()
 
     def run(self, requests: List[FitRequest], clock=None, max_wall: Optional[float] = None,
             keep_results: bool = False, progress: Optional[Callable[[ServeStats, int, int], None]] = None,
@@ -215,6 +232,7 @@ class FitServer:
         pending = sorted(requests, key=lambda r: r.arrival)
         queue: List[FitRequest] = []
         last_report = t0
+        stalled_rounds = 0
         while pending or queue or self.live:
             now = clock()
             while pending and pending[0].arrival <= now:
@@ -239,21 +257,27 @@ class FitServer:
                 continue
             stats.peak_bytes = max(stats.peak_bytes, self.used_bytes())
             stats.peak_live = max(stats.peak_live, len(self.live))
-            for l in self.live:   # one iteration of every live job
+            running = [l for l in self.live if not l.stalled]
+            for l in running:   # one iteration of every live job that can run
                 l.fit.step(1)
                 l.issued += 1
             stats.ticks += 1
-            stats.job_steps += len(self.live)
-            finishing = any(l.issued >= l.req.steps for l in self.live)
-            if not finishing and stats.ticks % self.sync_every:
+            stats.job_steps += len(running)
+            finishing = any(l.issued >= l.req.steps for l in running)
+            if not finishing and stats.ticks % self.sync_every and len(running) == len(self.live):
                 continue
             for l in self.live:
-                l.fit.sync()
+                self._sync(l)
             torch.cuda.synchronize(self.device)
+            # every live job waiting for memory: give the released cache a few
+            # retries before giving up
+            stalled_rounds = stalled_rounds + 1 if self.live and all(l.stalled for l in self.live) else 0
+            if stalled_rounds > 3:
+                raise RuntimeError("every live job is waiting for device memory to grow its intersection capacity")
             now = clock()
             still = []
             for l in self.live:
-                if l.issued >= l.req.steps:
+                if l.issued >= l.req.steps and not l.stalled:
                     loss = float(l.fit.loss.item())
                     stats.finished.append((l.req.rid, l.req.arrival, l.admitted, now, loss, l.req.beta))
                     if keep_results:

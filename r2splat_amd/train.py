@@ -56,16 +56,21 @@ class Workspace:
         return torch.empty((max(1, int(nbytes)),), dtype=torch.uint8, device=device), True
 
     def ensure(self, C, N, W, H, tile_size, capacity, beta: int = 0):
+        """Grows the buffers to the given sizes. All new buffers are allocated
+        before any is replaced, so an allocation failure leaves the workspace
+        (and graphs captured on it) as it was."""
         lib = be.lib()
-        changed = False
+        grown = {}
         for name, nbytes in (("scratch", lib.r2s_fused_scratch_bytes(C, N, W, H, tile_size, capacity, beta)),
                              ("packed", lib.r2s_fused_packed_bytes(C, N, capacity, beta)),
                              ("isect_grads", capacity * 8 * 4),
                              ("param_ws", lib.r2s_fused_param_workspace_bytes(C, N))):
-            t, grown = self._grow(getattr(self, name), nbytes, self.device)
+            t, new = self._grow(getattr(self, name), nbytes, self.device)
+            if new:
+                grown[name] = t
+        for name, t in grown.items():
             setattr(self, name, t)
-            changed |= grown
-        if changed:
+        if grown:
             self.version += 1
 
     def nbytes(self) -> int:
@@ -152,8 +157,11 @@ class GaussianFit:
                 flat[10 * N: 11 * N], flat[11 * N:] if self.kind else None)
 
     def _alloc(self, capacity: int):
-        self.capacity = max(1024, int(capacity))
-        self.ws.ensure(self.C, self.N, self.W, self.H, self.tile_size, self.capacity, self.kind)
+        # the buffers grow first and the capacity is adopted only once they all
+        # exist: an allocation failure leaves the previous (consistent) capacity
+        capacity = max(1024, int(capacity))
+        self.ws.ensure(self.C, self.N, self.W, self.H, self.tile_size, capacity, self.kind)
+        self.capacity = capacity
 
     @property
     def graph_stale(self) -> bool:

@@ -59,3 +59,27 @@ def test_admission_waits_for_memory(device):
     (r0, _, adm0, fin0, _, _), (r1, _, adm1, fin1, _, _) = sorted(stats.finished)
     assert adm1 >= fin0 - 1e-9  # the second job was admitted only after the first finished
     assert torch.equal(stats.results[1], _alone(reqs[1], device))
+
+
+def test_job_stalls_when_its_capacity_cannot_grow_and_resumes(device, monkeypatch):
+    """A job whose capacity growth runs out of device memory stalls (its
+    overflowed steps stay skipped), is retried at the next synchronisation and
+    ends bitwise equal to the job fitted alone."""
+    req = FitRequest(0, 0.0, 800, 3, 48, 113, False, 6)
+    expected = _alone(req, device)
+    monkeypatch.setattr(GaussianFit, "_measure_capacity", lambda self: 1024)  # the first step overflows
+    calls = []
+    regrow = GaussianFit.regrow
+
+    def flaky(self):
+        calls.append(len(calls))
+        if len(calls) == 1:
+            raise torch.OutOfMemoryError("simulated: no device memory to grow the capacity")
+        return regrow(self)
+
+    monkeypatch.setattr(GaussianFit, "regrow", flaky)
+    server = FitServer(device, n_streams=1, adam=ADAM)
+    stats = server.run([req], clock="ticks", keep_results=True)
+    assert len(calls) >= 2  # one failed growth, then a successful retry
+    assert [f[0] for f in stats.finished] == [0]
+    assert torch.equal(stats.results[0], expected)
