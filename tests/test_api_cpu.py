@@ -101,6 +101,39 @@ def test_fused_size_queries_need_no_gpu():
     assert b - a >= 8 * 10**6
 
 
+def test_workspace_growth_is_all_or_nothing(monkeypatch):
+    """A capacity growth that runs out of memory part way leaves every buffer
+    and the version (which graphs captured on the workspace check) as they
+    were; the next growth succeeds and bumps the version once."""
+    from r2splat_amd import train
+
+    load = _backend.lib
+    monkeypatch.setattr(train.be, "lib", lambda: load(require_gpu=False))  # size queries only
+    ws = train.Workspace("cpu")
+    ws.ensure(4, 1000, 64, 64, 16, 4096)
+    names = ("scratch", "packed", "isect_grads", "param_ws")
+    before = {n: getattr(ws, n) for n in names}
+    version = ws.version
+    real_empty, calls = torch.empty, []
+
+    def empty_failing_second(*args, **kwargs):
+        calls.append(args)
+        if len(calls) == 2:
+            raise torch.OutOfMemoryError("simulated")
+        return real_empty(*args, **kwargs)
+
+    monkeypatch.setattr(torch, "empty", empty_failing_second)
+    with pytest.raises(torch.OutOfMemoryError):
+        ws.ensure(4, 1000, 64, 64, 16, 1 << 20)  # grows the list entries and the partials
+    monkeypatch.setattr(torch, "empty", real_empty)
+    assert len(calls) == 2 and ws.version == version
+    assert all(getattr(ws, n) is before[n] for n in names)
+    ws.ensure(4, 1000, 64, 64, 16, 1 << 20)
+    assert ws.version == version + 1
+    assert ws.isect_grads.numel() >= (1 << 20) * 32 and ws.packed.numel() >= (1 << 20) * 8
+    assert ws.scratch is before["scratch"] and ws.param_ws is before["param_ws"]  # sizes independent of capacity
+
+
 @pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
 def test_operators_raise_without_gpu():
     from r2splat_amd import hip
