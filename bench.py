@@ -7,7 +7,7 @@ physical parameter gradients. Inputs are resident in HBM before timing starts.
 Voxelization of the same Gaussians to 128^3 (extent 1, tile 8) is timed
 separately (fwd + bwd of a fixed random adjoint).
 
-Prints one JSON line (see DESIGN.md for the field definitions), including the
+Prints one JSON line (metric, value, unit, timing, workload config), including the
 roofline of the dominant kernel measured with HIP events on its launch stream
 during the timed region, and a CPU baseline (the C/OpenMP restatement of the
 same iteration timed on a sample of the views on the host cores).
@@ -85,8 +85,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md and not measured)"}))
+        print(json.dumps({"error": "this implementation runs on one GPU; --gpus must be 1 (sharding across "
+                                   "GPUs is a design, not a measured path)"}))
         sys.exit(2)
     import torch
 
