@@ -141,3 +141,16 @@ def test_operators_raise_without_gpu():
     assert not hip.is_cuda_extension_available()
     with pytest.raises(RuntimeError):
         hip.parameter_transform_fwd(torch.ones(1, 4), torch.ones(1, 3))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark measures one GPU; asking for more exits with status 2 and a
+    JSON message, before anything touches a device."""
+    import json
+    import subprocess
+    import sys
+
+    proc = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                          timeout=120, cwd=ROOT)
+    assert proc.returncode == 2
+    assert "one GPU" in json.loads(proc.stdout.strip().splitlines()[-1])["error"]
