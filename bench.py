@@ -51,7 +51,7 @@ CLOCK_HZ = 2.4e9         # peak engine clock
 # plain 256 MiB device copy moves to what the same counter reads for it
 # (FETCH 131,089 KiB -> x2.0; WRITE 262,144 KiB -> x1.0). Summary and
 # commands: profiles/raster_100k_64x256_hbm_traffic.md.
-PMC_KIB = {"rasterize_fwd": (1_344_007, 16_445), "rasterize_bwd": (1_367_643, 862_751)}
+PMC_KIB = {"rasterize_fwd": (1_459_759, 16_441), "rasterize_bwd": (1_480_332, 862_928)}
 PMC_SCALE = (262_144 / 131_089, 262_144 / 262_144)
 
 
