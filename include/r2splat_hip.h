@@ -478,10 +478,12 @@ int r2s_clamp_min(float* x, uint64_t n, float lo, r2s_stream_t stream);
 int r2s_volume_tv_fwd(int dtype, const void* volume, uint32_t B, uint32_t D, uint32_t H,
                       uint32_t W, void* tv_map, void* tv_sum, r2s_stream_t stream);
 
-/* grad_volume = lambda * grad_output[0] / edge_count * sign-sum of adjacent
- * differences. Replaces volume_loss_bwd (Loss.cpp:156-188). */
+/* grad_volume = (sign-sum of adjacent differences * (grad_output[0] /
+ * edge_count)) * lambda_tv, lambda_tv a float as in the reference, in the
+ * volume's precision. Replaces volume_loss_bwd (Ops.h:41-44, Loss.cpp:156-188,
+ * VolumeLossBwd.cu:46-115). */
 int r2s_volume_tv_bwd(int dtype, const void* volume, const void* grad_output, uint32_t B,
-                      uint32_t D, uint32_t H, uint32_t W, double lambda_tv,
+                      uint32_t D, uint32_t H, uint32_t W, float lambda_tv,
                       void* grad_volume, r2s_stream_t stream);
 
 /* Isotropic TV over volume [B,D,H,W]: tv_map[v] = sqrt(dz^2 + dy^2 + dx^2 + eps)
@@ -498,12 +500,13 @@ int r2s_volume_tv_iso_bwd(int dtype, const void* volume, const void* grad_output
                           void* grad_volume, r2s_stream_t stream);
 
 /* Mean-L1 image loss map |p - t| and its sum (overwritten); gradient
- * lambda * g[0] * sign(p - t) / n. Replaces image_loss_l1_fwd/bwd
+ * g[0] * lambda * sign(p - t) / n (lambda a float, as in the reference).
+ * Replaces image_loss_l1_fwd/bwd
  * (Ops.h:14-34, r2splat/cuda/csrc/ImageLossFwd.cu:20-31, ImageLossBwd.cu:14-30). */
 int r2s_image_l1_fwd(int dtype, const void* prediction, const void* target, uint64_t n,
                      void* l1_map, void* l1_sum, r2s_stream_t stream);
 int r2s_image_l1_bwd(int dtype, const void* prediction, const void* target,
-                     const void* grad_output, uint64_t n, double lambda_l1,
+                     const void* grad_output, uint64_t n, float lambda_l1,
                      void* grad_prediction, r2s_stream_t stream);
 
 #ifdef __cplusplus

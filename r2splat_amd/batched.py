@@ -218,8 +218,7 @@ class BatchedFit:
             for fit, k in pending:
                 fit.regrow()
                 with torch.cuda.stream(fit.stream):
-                    for _ in range(k):
-                        fit._issue()
+                    fit.replay_skipped(k, graph=False)
             torch.cuda.synchronize(self.device)
         applied = {int(f.adam_state[0].item()) for f in self.fits} if self.fits[0].adam is not None else {
             self.steps_issued}

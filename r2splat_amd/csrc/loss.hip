@@ -44,10 +44,12 @@ tv_fwd_kernel(const S* __restrict__ vol, uint32_t B, uint32_t D, uint32_t H, uin
     block_sum_atomic(tv, tv_sum);
 }
 
+// grad = ((sign sum) * (g / edges)) * lambda: the reference's rounding order
+// (the kernel's g / edges product, then the host-side multiply by lambda).
 template <typename S>
 __global__ void __launch_bounds__(kLossThreads)
-tv_bwd_kernel(const S* __restrict__ vol, const S* __restrict__ grad_out, uint32_t B, uint32_t D,
-              uint32_t H, uint32_t W, S scale, S* __restrict__ grad) {
+tv_bwd_kernel(const S* __restrict__ vol, const S* __restrict__ grad_out, uint32_t B, uint32_t D, uint32_t H,
+              uint32_t W, S edges, S lambda, S* __restrict__ grad) {
     const size_t total = size_t(B) * D * H * W;
     const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
@@ -61,7 +63,8 @@ tv_bwd_kernel(const S* __restrict__ vol, const S* __restrict__ grad_out, uint32_
     if (h > 0) g += sgn(v - vol[idx - sh]);
     if (w + 1 < W) g -= sgn(vol[idx + 1] - v);
     if (w > 0) g += sgn(v - vol[idx - 1]);
-    grad[idx] = g * (grad_out[0] * scale);
+    const S per_edge = grad_out[0] / edges;
+    grad[idx] = (g * per_edge) * lambda;
 }
 
 // Isotropic TV: t(v) = sqrt(dz^2 + dy^2 + dx^2 + eps) with forward differences
@@ -140,7 +143,7 @@ l1_bwd_kernel(const S* __restrict__ p, const S* __restrict__ t, const S* __restr
               S lambda, S* __restrict__ out) {
     const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    out[idx] = g[0] * lambda * sgn(p[idx] - t[idx]) / S(n);
+    out[idx] = g[0] * lambda * sgn(p[idx] - t[idx]) / S(n);  // the reference's operation order
 }
 
 }  // namespace r2s
@@ -161,17 +164,17 @@ extern "C" int r2s_volume_tv_fwd(int dtype, const void* volume, uint32_t B, uint
 }
 
 extern "C" int r2s_volume_tv_bwd(int dtype, const void* volume, const void* grad_output, uint32_t B,
-                                 uint32_t D, uint32_t H, uint32_t W, double lambda_tv,
+                                 uint32_t D, uint32_t H, uint32_t W, float lambda_tv,
                                  void* grad_volume, r2s_stream_t stream) {
     const size_t total = size_t(B) * D * H * W;
     if (total == 0) return R2S_OK;
     const int64_t edges64 = int64_t(B) * ((int64_t(D) - 1) * H * W + int64_t(D) * (H - 1) * W +
                                           int64_t(D) * H * (W - 1));
-    const double edges = double(edges64 < 1 ? 1 : edges64);
+    const int64_t edges = edges64 < 1 ? 1 : edges64;
     return dispatch_dtype(dtype, [&](auto tag) {
         using S = decltype(tag);
         tv_bwd_kernel<S><<<ceil_div(total, kLossThreads), kLossThreads, 0, stream>>>(
-            (const S*)volume, (const S*)grad_output, B, D, H, W, S(lambda_tv / edges), (S*)grad_volume);
+            (const S*)volume, (const S*)grad_output, B, D, H, W, S(edges), S(lambda_tv), (S*)grad_volume);
         return check_launch("volume_tv_bwd");
     });
 }
@@ -218,7 +221,7 @@ extern "C" int r2s_image_l1_fwd(int dtype, const void* prediction, const void* t
 }
 
 extern "C" int r2s_image_l1_bwd(int dtype, const void* prediction, const void* target,
-                                const void* grad_output, uint64_t n, double lambda_l1,
+                                const void* grad_output, uint64_t n, float lambda_l1,
                                 void* grad_prediction, r2s_stream_t stream) {
     if (n == 0) return R2S_OK;
     return dispatch_dtype(dtype, [&](auto tag) {

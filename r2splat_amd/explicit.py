@@ -19,11 +19,11 @@ from typing import Optional, Sequence, Union
 
 import torch
 
+from . import _validation as check
 from . import fused
 from . import hip as ops
 from .hip import _wrapper as _ops_impl
 from ._beta import beta_path_log_grad, beta_path_ratio
-from ._validation import _constant, _options, _primitives, _shape, _tensor, _value_checks, _vector
 from .profiling import stage
 
 Tensor = torch.Tensor
@@ -31,7 +31,7 @@ Output = Union[Tensor, tuple]
 
 RASTER_TILE = 16
 VOXEL_TILE = 8
-_FIELDS = ("means", "quats", "scales", "densities", "beta", "uncertainty")
+_PRIMITIVE_NAMES = ("means", "quats", "scales", "densities", "beta", "uncertainty")
 
 # "fused": float32 projections (Gaussian or Beta, with or without
 # uncertainty) run the fused pipeline (r2splat_amd.fused); every other case,
@@ -61,17 +61,77 @@ class ParameterGradients:
     uncertainty: Optional[Tensor] = None
 
 
-@dataclass
+class _VersionGuard:
+    """The version counter of every distinct tensor a backward will read,
+    taken when the forward returns; ``verify`` names the first one that an
+    in-place operation has changed since."""
+
+    def __init__(self, named):
+        self._watched = []
+        ids = set()
+        for name, value in named:
+            if not isinstance(value, Tensor) or id(value) in ids:
+                continue
+            ids.add(id(value))
+            try:
+                self._watched.append((name, value, value._version))
+            except RuntimeError as error:  # inference tensors carry no version counter
+                raise ValueError("explicit contexts require tensors with version counters") from error
+
+    def verify(self) -> None:
+        for name, value, version in self._watched:
+            if value._version != version:
+                raise RuntimeError(f"saved input {name} was modified in-place after forward")
+
+
 class _Context:
-    kind: str
-    shape: tuple
-    primitives: tuple
-    buffers: dict
-    versions: tuple
-    eps: float
-    backward_mode: str
-    detach_variance: bool
-    used: bool = False
+    """What a forward hands its backward: the validated primitives, the saved
+    buffers and options. Used once; the backward refuses a context of the
+    other operator, a consumed one, or one whose inputs changed in place."""
+
+    def __init__(self, kind: str, shape: tuple, primitives: tuple, buffers: dict, inputs, eps: float,
+                 backward_mode: str, detach_variance: bool):
+        self.kind = kind
+        self.shape = shape
+        self.primitives = primitives
+        self.buffers = buffers
+        self.eps = eps
+        self.backward_mode = backward_mode
+        self.detach_variance = detach_variance
+        self.used = False
+        self._guard = _VersionGuard(list(inputs) + list(zip(_PRIMITIVE_NAMES, primitives)) + list(buffers.items()))
+
+    @staticmethod
+    def open(context, kind: str) -> "_Context":
+        if not isinstance(context, _Context) or context.kind != kind:
+            raise TypeError(f"expected a context returned by {kind}_fwd")
+        if context.used:
+            raise RuntimeError("this context has already been consumed by backward")
+        context._guard.verify()
+        return context
+
+    def adjoints(self, named) -> list:
+        """Validates the output adjoints (None stays None) with one value check."""
+        checks = check.ValueChecks()
+        out = []
+        for name, value in named:
+            if value is not None:
+                value = check.tensor(name, value, self.shape, self.primitives[0])
+                checks.finite(name, value)
+            out.append(value)
+        checks.run()
+        return out
+
+    def zero_gradients(self) -> ParameterGradients:
+        return ParameterGradients(*(None if p is None else torch.zeros_like(p) for p in self.primitives))
+
+    def close(self, gradients: ParameterGradients) -> ParameterGradients:
+        """Marks the context consumed and drops everything it holds."""
+        self.used = True
+        self.primitives = ()
+        self.buffers.clear()
+        self._guard = _VersionGuard(())
+        return gradients
 
 
 def _mode(value) -> str:
@@ -81,46 +141,43 @@ def _mode(value) -> str:
     return value
 
 
-def _make_context(kind, shape, primitives, buffers, originals, eps, mode, detach):
-    """Snapshots the version counter of every tensor the backward depends on
-    (original inputs, their validated copies and the saved buffers)."""
-    watched, seen = [], set()
-    named = list(originals) + list(zip(_FIELDS, primitives)) + list(buffers.items())
-    for name, value in named:
-        if isinstance(value, Tensor) and id(value) not in seen:
-            try:
-                version = value._version
-            except RuntimeError as error:
-                raise ValueError("explicit contexts require tensors with version counters") from error
-            watched.append((name, value, version))
-            seen.add(id(value))
-    return _Context(kind, shape, primitives, buffers, tuple(watched), eps, mode, detach)
+def _cameras(viewmats, Ks, means, height: int, width: int, checks: check.ValueChecks):
+    """viewmats [C,4,4] (world-to-camera, last row 0 0 0 1) and pinhole Ks
+    [C,3,3] (positive focal lengths, zero skew, last row 0 0 1), constant."""
+    for name, value in (("viewmats", viewmats), ("Ks", Ks)):
+        check.constant(name, value)
+    viewmats = check.tensor("viewmats", viewmats, (None, 4, 4), means)
+    C = viewmats.shape[0]
+    if C == 0:
+        raise ValueError("viewmats must contain at least one camera")
+    if C > 65535 or C * max(height * width, means.shape[0]) > check.INT32_MAX:
+        raise ValueError("camera batch exceeds the kernels' indexing or grid limits")
+    if max(height, width) > 16 * 65535:
+        raise ValueError("image_size exceeds the kernels' grid limits")
+    Ks = check.tensor("Ks", Ks, (C, 3, 3), means)
+    checks.finite("viewmats", viewmats)
+    checks.finite("Ks", Ks)
+    checks.add("Ks must have positive fx and fy", lambda: (Ks[:, (0, 1), (0, 1)] <= 0).any())
+    checks.add("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]",
+               lambda: (Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any() | (Ks[:, 2, 2] != 1).any())
+    checks.add("viewmats must have homogeneous last row [0, 0, 0, 1]",
+               lambda: (viewmats[:, 3] != means.new_tensor([0, 0, 0, 1])).any())
+    return viewmats, Ks
 
 
-def _check_context(ctx, kind):
-    if not isinstance(ctx, _Context) or ctx.kind != kind:
-        raise TypeError(f"expected a context returned by {kind}_fwd")
-    if ctx.used:
-        raise RuntimeError("this context has already been consumed by backward")
-    for name, value, version in ctx.versions:
-        if value._version != version:
-            raise RuntimeError(f"saved input {name} was modified in-place after forward")
-
-
-def _adjoint(name, value, ctx):
-    return None if value is None else _tensor(name, value, ctx.shape, ctx.primitives[0])
-
-
-def _zeros(ctx):
-    return ParameterGradients(*(None if v is None else torch.zeros_like(v) for v in ctx.primitives))
-
-
-def _consume(ctx, gradients):
-    ctx.used = True
-    ctx.primitives = ()
-    ctx.buffers.clear()
-    ctx.versions = ()
-    return gradients
+def _volume_frame(extent, center, means, checks: check.ValueChecks):
+    """volume_extent (positive) and volume_center: 3-vectors, constant."""
+    out = []
+    for name, value in (("volume_extent", extent), ("volume_center", center)):
+        check.constant(name, value)
+        if not isinstance(value, Tensor):
+            value = torch.as_tensor(value, device=means.device, dtype=means.dtype)
+        value = check.tensor(name, value, (3,), means)
+        checks.finite(name, value)
+        out.append(value)
+    extent = out[0]
+    checks.add("volume_extent must contain positive physical lengths", lambda: (extent <= 0).any())
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -202,32 +259,17 @@ def rasterize_fwd(
     per-intersection lists between the two calls (for camera batches whose
     intersection lists do not fit next to the rest of the working set).
     """
-    originals = list(zip(("means", "quats", "scales", "densities", "viewmats", "Ks", "beta", "uncertainty"),
-                         (means, quats, scales, densities, viewmats, Ks, beta, uncertainty)))
-    height, width = _shape("image_size", image_size, 2)
-    prims = _primitives(means, quats, scales, densities, beta, uncertainty)
+    inputs = (("means", means), ("quats", quats), ("scales", scales), ("densities", densities),
+              ("viewmats", viewmats), ("Ks", Ks), ("beta", beta), ("uncertainty", uncertainty))
+    height, width = check.positive_ints("image_size", image_size, 2)
+    values = check.ValueChecks()
+    prims = check.primitives((means, quats, scales, densities, beta, uncertainty), values)
     means, quats, scales, densities, beta, uncertainty = prims
-    eps, radius_clip = _options(beta, eps, radius_clip, detach_variance)
+    eps, radius_clip = check.options(beta, eps, radius_clip, detach_variance)
     mode = _mode(backward_mode)
-    for name, value in (("viewmats", viewmats), ("Ks", Ks)):
-        if isinstance(value, Tensor):
-            _constant(name, value)
-    viewmats = _constant("viewmats", _tensor("viewmats", viewmats, (None, 4, 4), means))
+    viewmats, Ks = _cameras(viewmats, Ks, means, height, width, values)
+    values.run()
     C = viewmats.shape[0]
-    if C == 0:
-        raise ValueError("viewmats must contain at least one camera")
-    if C * max(height * width, means.shape[0]) > 2**31 - 1 or C > 65535:
-        raise ValueError("camera batch exceeds the kernels' indexing or grid limits")
-    if max(height, width) > 16 * 65535:
-        raise ValueError("image_size exceeds the kernels' grid limits")
-    Ks = _constant("Ks", _tensor("Ks", Ks, (C, 3, 3), means))
-    if _value_checks():
-        if bool((Ks[:, (0, 1), (0, 1)] <= 0).any()):
-            raise ValueError("Ks must have positive fx and fy")
-        if bool((Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any()) or bool((Ks[:, 2, 2] != 1).any()):
-            raise ValueError("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]")
-        if not torch.equal(viewmats[:, 3], means.new_tensor([0, 0, 0, 1]).expand(C, -1)):
-            raise ValueError("viewmats must have homogeneous last row [0, 0, 0, 1]")
     shape = (C, height, width)
     buffers = {"viewmats": viewmats, "Ks": Ks}
     if means.shape[0] == 0:
@@ -238,21 +280,19 @@ def rasterize_fwd(
         output = _raster_render(buffers, prims, shape)
         if recompute:
             _drop_state(buffers, eps, radius_clip)
-    ctx = _make_context("rasterize", shape, prims, buffers, originals, eps, mode, detach_variance)
-    return output, ctx
+    return output, _Context("rasterize", shape, prims, buffers, inputs, eps, mode, detach_variance)
 
 
 @torch.no_grad()
 def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Optional[Tensor] = None):
     """Consumes a projection context; returns physical-parameter gradients."""
-    _check_context(context, "rasterize")
+    context = _Context.open(context, "rasterize")
     means, quats, scales, densities, beta, uncertainty = context.primitives
     if v_variance is not None and uncertainty is None:
         raise ValueError("v_variance requires uncertainty in the forward call")
-    v_image = _adjoint("v_image", v_image, context)
-    v_variance = _adjoint("v_variance", v_variance, context)
+    v_image, v_variance = context.adjoints((("v_image", v_image), ("v_variance", v_variance)))
     if means.shape[0] == 0 or (v_image is None and v_variance is None):
-        return _consume(context, _zeros(context))
+        return context.close(context.zero_gradients())
     if v_image is None:
         v_image = means.new_zeros(context.shape)
     b = context.buffers
@@ -265,8 +305,7 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
         v_means, v_quats, v_scales, v_densities = grads[:4]
         v_beta = grads[4] if beta is not None else None
         v_uncertainty = grads[-1] if uncertainty is not None else None
-        return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta,
-                                                    v_uncertainty))
+        return context.close(ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
     _, height, width = context.shape
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""
@@ -298,7 +337,7 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
                                                               height, v_mapped.contiguous(), v_cov2d)
         v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
         v_densities = v_opacity.sum(dim=0)
-    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
+    return context.close(ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
 
 
 # ---------------------------------------------------------------------------
@@ -360,23 +399,19 @@ def voxelize_fwd(
     ``recompute=True`` re-runs the volume mapping and 3D binning in the
     backward instead of holding the intersection lists in the context.
     """
-    originals = list(zip(("means", "quats", "scales", "densities", "volume_extent", "volume_center", "beta",
-                          "uncertainty"),
-                         (means, quats, scales, densities, volume_extent, volume_center, beta, uncertainty)))
-    depth, height, width = _shape("volume_shape", volume_shape, 3)
+    inputs = (("means", means), ("quats", quats), ("scales", scales), ("densities", densities),
+              ("volume_extent", volume_extent), ("volume_center", volume_center), ("beta", beta),
+              ("uncertainty", uncertainty))
+    depth, height, width = check.positive_ints("volume_shape", volume_shape, 3)
     if max(height, width) > 8 * 65535:
         raise ValueError("volume_shape exceeds the kernels' grid limits")
-    prims = _primitives(means, quats, scales, densities, beta, uncertainty)
+    values = check.ValueChecks()
+    prims = check.primitives((means, quats, scales, densities, beta, uncertainty), values)
     means, quats, scales, densities, beta, uncertainty = prims
-    eps, radius_clip = _options(beta, eps, radius_clip, detach_variance)
+    eps, radius_clip = check.options(beta, eps, radius_clip, detach_variance)
     mode = _mode(backward_mode)
-    for name, value in (("volume_extent", volume_extent), ("volume_center", volume_center)):
-        if isinstance(value, Tensor):
-            _constant(name, value)
-    extent = _vector("volume_extent", volume_extent, means)
-    center = _vector("volume_center", volume_center, means)
-    if _value_checks() and bool((extent <= 0).any()):
-        raise ValueError("volume_extent must contain positive physical lengths")
+    extent, center = _volume_frame(volume_extent, volume_center, means, values)
+    values.run()
     shape = (depth, height, width)
     buffers = {"extent": extent, "center": center}
     if means.shape[0] == 0:
@@ -387,21 +422,19 @@ def voxelize_fwd(
         output = _voxel_render(buffers, prims, shape)
         if recompute:
             _drop_state(buffers, eps, radius_clip)
-    ctx = _make_context("voxelize", shape, prims, buffers, originals, eps, mode, detach_variance)
-    return output, ctx
+    return output, _Context("voxelize", shape, prims, buffers, inputs, eps, mode, detach_variance)
 
 
 @torch.no_grad()
 def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Optional[Tensor] = None):
     """Consumes a volume context; returns physical-parameter gradients."""
-    _check_context(context, "voxelize")
+    context = _Context.open(context, "voxelize")
     means, quats, scales, densities, beta, uncertainty = context.primitives
     if v_variance is not None and uncertainty is None:
         raise ValueError("v_variance requires uncertainty in the forward call")
-    v_volume = _adjoint("v_volume", v_volume, context)
-    v_variance = _adjoint("v_variance", v_variance, context)
+    v_volume, v_variance = context.adjoints((("v_volume", v_volume), ("v_variance", v_variance)))
     if means.shape[0] == 0 or (v_volume is None and v_variance is None):
-        return _consume(context, _zeros(context))
+        return context.close(context.zero_gradients())
     if v_volume is None:
         v_volume = means.new_zeros(context.shape)
     b = context.buffers
@@ -435,7 +468,7 @@ def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Opti
                                                             height, depth, context.eps, v_mapped.contiguous(),
                                                             v_conics.contiguous())
         v_quats, v_scales = ops.parameter_transform_bwd(quats, scales, v_covars)
-    return _consume(context, ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
+    return context.close(ParameterGradients(v_means, v_quats, v_scales, v_densities, v_beta, v_uncertainty))
 
 
 __all__ = ["ParameterGradients", "rasterize_fwd", "rasterize_bwd", "voxelize_fwd", "voxelize_bwd"]

@@ -76,11 +76,11 @@ SIGNATURES = {
     "r2s_adam_step": [P, P, P, P, U64, F, F, F, F, P, P, P],
     "r2s_clamp_min": [P, U64, F, P],
     "r2s_volume_tv_fwd": [I32, P, U32, U32, U32, U32, P, P, P],
-    "r2s_volume_tv_bwd": [I32, P, P, U32, U32, U32, U32, D, P, P],
+    "r2s_volume_tv_bwd": [I32, P, P, U32, U32, U32, U32, F, P, P],
     "r2s_volume_tv_iso_fwd": [I32, P, U32, U32, U32, U32, D, P, P, P],
     "r2s_volume_tv_iso_bwd": [I32, P, P, U32, U32, U32, U32, D, D, P, P],
     "r2s_image_l1_fwd": [I32, P, P, U64, P, P, P],
-    "r2s_image_l1_bwd": [I32, P, P, P, U64, D, P, P],
+    "r2s_image_l1_bwd": [I32, P, P, P, U64, F, P, P],
 }
 
 _RESTYPES = {

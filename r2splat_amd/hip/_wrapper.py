@@ -11,6 +11,7 @@ raise RuntimeError, as the reference's TORCH_CHECKs do.
 from __future__ import annotations
 
 import math
+import struct
 from typing import Optional
 
 import torch
@@ -688,6 +689,11 @@ def beta_voxelize_to_voxels_mean_var_bwd_per_gaussian(*args):
 # ---------------------------------------------------------------------------
 # losses
 # ---------------------------------------------------------------------------
+def _f32(x) -> float:
+    """x rounded to float32: the reference's loss ops take float scalars (Ops.h:14-44)."""
+    return struct.unpack("f", struct.pack("f", float(x)))[0]
+
+
 def volume_loss_fwd(volume, lambda_tv):
     """Anisotropic TV over (B,D,H,W) -> (lambda * tv, tv, tv_map)."""
     _check("volume", volume, 4)
@@ -698,8 +704,9 @@ def volume_loss_fwd(volume, lambda_tv):
     with _Launch(volume) as L:
         call("r2s_volume_tv_fwd", dt, ptr(volume), B, Dd, H, W, ptr(tv_map), ptr(tv_sum), L.stream)
     edges = max(B * ((Dd - 1) * H * W + Dd * (H - 1) * W + Dd * H * (W - 1)), 1)
-    tv = (tv_sum / edges).squeeze()
-    return (float(lambda_tv) * tv), tv, tv_map
+    # the reference divides by float(edge_count) and scales by a float lambda (Loss.cpp:115-153)
+    tv = (tv_sum / _f32(edges)).squeeze()
+    return _f32(lambda_tv) * tv, tv, tv_map
 
 
 def volume_loss_bwd(volume, grad_output, lambda_tv):
@@ -712,7 +719,7 @@ def volume_loss_bwd(volume, grad_output, lambda_tv):
     B, Dd, H, W = volume.shape
     grad = torch.empty_like(volume)
     with _Launch(volume) as L:
-        call("r2s_volume_tv_bwd", dt, ptr(volume), ptr(grad_output), B, Dd, H, W, float(lambda_tv), ptr(grad),
+        call("r2s_volume_tv_bwd", dt, ptr(volume), ptr(grad_output), B, Dd, H, W, _f32(lambda_tv), ptr(grad),
              L.stream)
     return grad
 
@@ -762,7 +769,7 @@ def image_loss_l1_fwd(prediction, target, lambda_l1):
         call("r2s_image_l1_fwd", dt, ptr(prediction), ptr(target), prediction.numel(), ptr(l1_map), ptr(l1_sum),
              L.stream)
     l1 = (l1_sum / max(prediction.numel(), 1)).squeeze()
-    return float(lambda_l1) * l1, l1, l1_map
+    return _f32(lambda_l1) * l1, l1, l1_map
 
 
 def image_loss_l1_bwd(prediction, target, grad_output, lambda_l1):
@@ -776,7 +783,7 @@ def image_loss_l1_bwd(prediction, target, grad_output, lambda_l1):
     grad = torch.empty_like(prediction)
     with _Launch(prediction) as L:
         call("r2s_image_l1_bwd", dt, ptr(prediction), ptr(target), ptr(grad_output), prediction.numel(),
-             float(lambda_l1), ptr(grad), L.stream)
+             _f32(lambda_l1), ptr(grad), L.stream)
     return grad
 
 
@@ -788,6 +795,6 @@ def is_cuda_extension_available() -> bool:
     return be.is_available()
 
 
-__all__ = [name for name in dir() if not name.startswith("_") and name not in ("annotations", "math", "torch",
+__all__ = [name for name in dir() if not name.startswith("_") and name not in ("annotations", "math", "struct", "torch",
                                                                                    "Tensor", "Optional", "be", "call",
                                                                                    "ptr")]

@@ -147,6 +147,7 @@ class GaussianFit:
         self.ws = workspace if workspace is not None else Workspace(device)
         self.graph = None
         self._graph_version = -1
+        self._regrown = False
         self.steps_issued = 0
         self._alloc(int(capacity) if capacity else self._measure_capacity())
 
@@ -263,26 +264,46 @@ class GaussianFit:
         return skipped
 
     def regrow(self):
-        """Grows the capacity past the last step's intersection count and clears the skip counter."""
+        """Grows the capacity past the last step's intersection count (and past
+        the current capacity, so repeated overflows always make progress); an
+        allocation failure leaves the previous capacity and buffers. The skip
+        counter is left alone: it is cleared only when the skipped steps are
+        re-issued (``replay_skipped``)."""
         count = int(self.counters[0].item())
         self._alloc(int(math.ceil(max(count, self.capacity) * self.headroom)) + 1024)
+
+    def replay_skipped(self, skipped: int, graph: bool):
+        """Clears the skip counter and re-issues `skipped` steps; call it only
+        once every buffer (and, with `graph`, the graph) they need exists."""
         self.adam_state[1] = 0
+        for _ in range(skipped):
+            self.step(1, graph=graph)
 
     def sync(self) -> int:
         """Waits for the issued steps; on an overflow, grows the capacity,
-        re-captures and re-runs the skipped steps. Returns the steps applied."""
+        re-captures and re-runs the skipped steps. Returns the steps applied.
+
+        Each stage is all or nothing: when growing the buffers or re-capturing
+        raises (out of device memory), the skip counter still holds the
+        skipped steps and a later sync retries from there; a graph left stale
+        by a failed capture (or by a shared workspace that grew) is
+        re-captured before anything is replayed."""
         with self._ctx():
             torch.cuda.current_stream(self.device).synchronize()
             while True:
                 skipped = self.pending_skips()
+                had_graph = self.graph is not None
                 if skipped == 0:
+                    if had_graph and self.graph_stale:
+                        self.capture()
                     return int(self.adam_state[0].item()) if self.adam is not None else self.steps_issued
-                recapture = self.graph is not None
-                self.regrow()
-                if recapture:
+                if not self._regrown:  # grow once per run of skipped steps, even if a capture fails after it
+                    self.regrow()
+                    self._regrown = True
+                if had_graph and self.graph_stale:
                     self.capture()
-                for _ in range(skipped):
-                    self.step(1, graph=recapture)
+                self._regrown = False
+                self.replay_skipped(skipped, graph=had_graph)
                 torch.cuda.current_stream(self.device).synchronize()
 
     def loss_value(self) -> float:

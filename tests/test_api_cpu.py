@@ -11,7 +11,7 @@ import torch
 
 import r2splat_amd
 from r2splat_amd import rasterize_fwd, voxelize_fwd
-from r2splat_amd._validation import _options, _shape
+from r2splat_amd._validation import options, positive_ints
 from r2splat_amd.hip import _backend
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -27,19 +27,19 @@ def test_public_api_is_small():
 @pytest.mark.parametrize("shape", [(0, 3), (True, 3), (2.5, 3), (3,), (2**30, 4)])
 def test_invalid_output_shape_rejected(shape):
     with pytest.raises(ValueError, match="image_size"):
-        _shape("image_size", shape, 2)
+        positive_ints("image_size", shape, 2)
 
 
 @pytest.mark.parametrize("eps,radius,detach", [(-1, None, False), (math.nan, None, False), (0, 0, False),
                                                (0, math.inf, False), (0, 1, "yes")])
 def test_invalid_numeric_options_rejected(eps, radius, detach):
     with pytest.raises((ValueError, TypeError)):
-        _options(None, eps, radius, detach)
+        options(None, eps, radius, detach)
 
 
 def test_radius_clip_defaults():
-    assert _options(None, 0.0, None, False) == (0.0, 3.0)
-    assert _options(torch.ones(1), 0.0, None, False) == (0.0, 1.0)
+    assert options(None, 0.0, None, False) == (0.0, 3.0)
+    assert options(torch.ones(1), 0.0, None, False) == (0.0, 1.0)
 
 
 def test_cpu_inputs_fail_before_native_dispatch():

@@ -474,19 +474,29 @@ def test_voxelize_ops_forward_and_vjps(device, dtype, beta, tile):
 
 
 # --------------------------------------------------------------------------- losses
+def _f32(x):
+    return float(np.float32(x))
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_volume_tv_loss(device, dtype):
+    """lambda is a float in the reference's op (Ops.h:36-44): float64 volumes are
+    scaled by float32(0.1) = 0.100000001490116..., not by 0.1."""
     g = torch.Generator().manual_seed(0)
     vol = torch.randn(2, 4, 5, 6, generator=g, dtype=torch.float64)
     vr = vol.clone().requires_grad_()
     tv = O.volume_tv(vr)
-    (grad,) = torch.autograd.grad(0.05 * tv, vr)
-    loss, tv_d, tv_map = ops.volume_loss_fwd(vol.to(device, dtype), 0.05)
-    t = tol(dtype, 1e-6, 1e-12)
+    lam = _f32(0.1)
+    (grad,) = torch.autograd.grad(lam * tv, vr)
+    loss, tv_d, tv_map = ops.volume_loss_fwd(vol.to(device, dtype), 0.1)
+    t = tol(dtype, 1e-6, 1e-13)
     assert abs(tv_d.item() - tv.item()) <= t * abs(tv.item())
-    assert abs(loss.item() - 0.05 * tv.item()) <= t * abs(tv.item())
-    gd = ops.volume_loss_bwd(vol.to(device, dtype), torch.ones(1, device=device, dtype=dtype), 0.05)
+    assert abs(loss.item() - lam * tv.item()) <= t * abs(tv.item())
+    gd = ops.volume_loss_bwd(vol.to(device, dtype), torch.ones(1, device=device, dtype=dtype), 0.1)
     torch.testing.assert_close(gd.cpu().double(), grad, atol=t, rtol=t)
+    if dtype == torch.float64:
+        # the float rounding of lambda is visible at float64 precision
+        assert abs(loss.item() - 0.1 * tv.item()) > 1e-9 * abs(tv.item())
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -513,6 +523,12 @@ def test_image_l1_loss(device):
     assert abs(loss.item() - 0.25 * 2.5) < 1e-6
     g = ops.image_loss_bwd(p, t, torch.ones(1, device=device), 0.25)
     torch.testing.assert_close(g.cpu(), 0.25 * torch.sign(p.cpu()) / 4)
+    # float64: lambda is rounded to float first, as the reference's float argument is
+    pd, td = p.double(), t.double()
+    loss, _, _ = ops.image_loss_fwd(pd, td, 0.1)
+    assert loss.item() == _f32(0.1) * 2.5
+    g = ops.image_loss_bwd(pd, td, torch.ones(1, device=device, dtype=torch.float64), 0.1)
+    assert torch.equal(g.cpu(), torch.sign(pd.cpu()) * _f32(0.1) / 4)
 
 
 def test_ops_reject_bad_inputs_with_runtime_error(device):

@@ -381,3 +381,166 @@ def test_volume_tv_iso_known_answers():
         vm[idx] -= h_
         fd = (O.volume_tv_iso(vp, 1e-6) - O.volume_tv_iso(vm, 1e-6)).item() / (2 * h_)
         assert abs(fd - g[idx].item()) < 1e-7
+
+
+# --------------------------------------------------------------------------- radii (cull rules)
+def test_radii_closed_form_on_axis():
+    """On the optical axis cov2d[0,0] = (fx sx / z)^2 and cov2d[1,1] = (fy sy / z)^2,
+    so rx = ceil(clip sqrt((fx sx / z)^2 + eps)) (WorldToImageProjectionFwd.cu:250-276):
+    checked either side of an integer so a dropped eps or a floor fails."""
+    fx, fy, z, clip, eps = 12.0, 10.0, 2.0, 3.0, 0.25
+    for sx, sy in ((0.3, 0.45), (0.1, 0.05), (0.8, 0.02)):
+        means, quats, scales, view, K = _axis_scene([sx, sy, 0.5], fx=fx, fy=fy, z=z)
+        _, c2, _ = O.project(means, O.covariance(quats, scales), view, K, 9, 7)
+        packed = O.pack_sym(c2).numpy()
+        radii = O.radii_2d(packed, np.array([[[4.5, 3.5]]]), eps, clip, 9, 7)
+        rx = math.ceil(clip * math.sqrt((fx * sx / z) ** 2 + eps))
+        ry = math.ceil(clip * math.sqrt((fy * sy / z) ** 2 + eps))
+        assert radii[0, 0].tolist() == [rx, ry]
+    # variance chosen so that clip * sqrt(var + eps) lands just below / above 5
+    for var, expect in ((25.0 / 9.0 - eps - 1e-9, 5), (25.0 / 9.0 - eps + 1e-9, 6)):
+        packed = np.array([[[var, 0.0, 0.0, 1.0, 0.0, 1.0]]])
+        assert O.radii_2d(packed, np.array([[[4.5, 3.5]]]), eps, clip, 9, 7)[0, 0, 0] == expect
+
+
+def test_radii_cull_rules():
+    """Zero radii for a non-positive-definite 2x2 block, non-finite input and
+    each of the four off-screen tests; the boundary itself (mean +- r exactly on
+    the image edge) is culled (<= 0, >= width/height)."""
+    W, H = 20, 10
+    base = [4.0, 0.0, 0.0, 1.0, 0.0, 1.0]  # sigma_x 2, sigma_y 1: rx = ceil(3 sqrt(4)) = 6, ry = 3
+
+    def radii(cov, mean, eps=0.0):
+        return O.radii_2d(np.array([[cov]]), np.array([[mean]]), eps, 3.0, W, H)[0, 0].tolist()
+
+    assert radii(base, [10.0, 5.0]) == [6, 3]
+    assert radii([4.0, 2.0, 0.0, 1.0, 0.0, 1.0], [10.0, 5.0]) == [0, 0]  # det = 0
+    assert radii([4.0, 2.5, 0.0, 1.0, 0.0, 1.0], [10.0, 5.0]) == [0, 0]  # det < 0
+    assert radii([4.0, 2.0, 0.0, 1.0, 0.0, 1.0], [10.0, 5.0], eps=0.1)[0] > 0  # eps restores det > 0
+    assert radii([math.nan, 0.0, 0.0, 1.0, 0.0, 1.0], [10.0, 5.0]) == [0, 0]
+    assert radii([math.inf, 0.0, 0.0, 1.0, 0.0, 1.0], [10.0, 5.0]) == [0, 0]
+    # left: mean_x + rx <= 0
+    assert radii(base, [-6.0, 5.0]) == [0, 0] and radii(base, [-5.5, 5.0]) == [6, 3]
+    # right: mean_x - rx >= W
+    assert radii(base, [W + 6.0, 5.0]) == [0, 0] and radii(base, [W + 5.5, 5.0]) == [6, 3]
+    # top: mean_y + ry <= 0
+    assert radii(base, [10.0, -3.0]) == [0, 0] and radii(base, [10.0, -2.5]) == [6, 3]
+    # bottom: mean_y - ry >= H
+    assert radii(base, [10.0, H + 3.0]) == [0, 0] and radii(base, [10.0, H + 2.5]) == [6, 3]
+
+
+def test_projection_culls_points_at_or_behind_the_near_plane():
+    """z <= 0.01 gives zero means2d and cov2d and is not visible
+    (WorldToImageProjectionFwd.cu:129-136)."""
+    cov = O.covariance(torch.tensor([[1.0, 0, 0, 0]] * 3), torch.full((3, 3), 0.1))
+    means = torch.tensor([[0.0, 0.0, 0.01], [0.0, 0.0, 0.0100001], [0.3, 0.0, -1.0]])
+    K = torch.tensor([[[12.0, 0, 4.5], [0, 10.0, 3.5], [0, 0, 1]]])
+    m2, c2, vis = O.project(means, cov, torch.eye(4)[None], K, 9, 7)
+    assert vis[0].tolist() == [False, True, False]
+    assert float(m2[0, 0].abs().sum()) == 0.0 and float(c2[0, 0].abs().sum()) == 0.0
+    assert float(m2[0, 2].abs().sum()) == 0.0 and float(c2[0, 2].abs().sum()) == 0.0
+
+
+# --------------------------------------------------------------------------- 3D tile mask
+def test_tile_mask_3d_matches_loops():
+    rng = np.random.default_rng(11)
+    N, T, shape = 6, 4, (7, 9, 13)  # (D, H, W)
+    means = rng.uniform(-3, [16, 12, 10], size=(N, 3))
+    radii = rng.integers(0, 6, size=(N, 3)).astype(np.int32)
+    radii[0] = [2, 0, 3]  # a zero radius on one axis: empty
+    mask = O.tile_mask_3d(means, radii, T, shape)
+    D, H, W = shape
+    for n in range(N):
+        spans = []
+        for k, size in enumerate((W, H, D)):
+            nt = -(-size // T)
+            lo = math.floor(means[n, k] / T - radii[n, k] / T)
+            hi = math.ceil(means[n, k] / T + radii[n, k] / T)
+            spans.append((min(max(lo, 0), nt), min(max(hi, 0), nt)))
+        live = bool((radii[n] > 0).all())
+        for z in range(D):
+            for y in range(H):
+                for x in range(W):
+                    inside = live and all(s[0] <= v // T < s[1] for s, v in zip(spans, (x, y, z)))
+                    assert mask[n, z, y, x] == inside, (n, z, y, x)
+    assert not mask[0].any()
+
+
+# --------------------------------------------------------------------------- voxel Beta / variance
+def _centred_pair():
+    """The reference's two-primitive scene centred on a 1x1x1 volume
+    (its tests/test_beta_api.py:14-19,26-28)."""
+    means = torch.tensor([[0.0, 0, 2], [0.0, 0, 2]])
+    quats = torch.tensor([[2.0, 0, 0, 0], [3.0, 0, 0, 0]])
+    scales = torch.tensor([[0.2, 0.3, 0.4], [0.3, 0.4, 0.5]])
+    dens = torch.tensor([1.2, 0.8])
+    return means, quats, scales, dens
+
+
+@pytest.mark.parametrize("detach", [False, True])
+def test_voxelize_beta_variance_centred_closed_form(detach):
+    """Every primitive sits on the voxel centre: d2 = 0, alpha = density, so
+    mean = sum rho, variance = sum q rho^2; with adjoints (1, 0.4) the VJPs are
+    d/drho = 1 + 0.8 q rho (1 when detached), d/dbeta = 0 (log 1), d/dq = 0.4 rho^2."""
+    means, quats, scales, dens = _centred_pair()
+    rho = dens.clone().requires_grad_()
+    beta = torch.tensor([3.5, 1.25], requires_grad=True)
+    q = torch.tensor([0.2, 0.3], requires_grad=True)
+    mean, var = O.voxelize(means, quats, scales, rho, (1, 1, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 2.0), beta=beta,
+                           uncertainty=q, eps=0.0, detach=detach)
+    assert abs(mean.item() - 2.0) < 1e-15
+    assert abs(var.item() - (0.2 * 1.44 + 0.3 * 0.64)) < 1e-15
+    g_rho, g_beta, g_q = torch.autograd.grad(mean.sum() + 0.4 * var.sum(), (rho, beta, q))
+    factor = 1.0 + (0.0 if detach else 0.8) * q.detach() * dens
+    torch.testing.assert_close(g_rho, factor, atol=1e-15, rtol=0)
+    assert float(g_beta.abs().max()) == 0.0
+    torch.testing.assert_close(g_q, 0.4 * dens.square(), atol=1e-15, rtol=0)
+
+
+def test_voxelize_beta_off_centre_axis_aligned_by_hand():
+    """Axis-aligned ellipsoid (distinct semi-axes) on a non-cubic grid with
+    distinct extents: d2 = sum_k (x_k - m_k)^2 / s_k^2 at voxel centres
+    center + ((index + 1/2) / n - 1/2) extent, alpha = rho (1 - d2)^beta kept
+    for 0 <= d2 < 1 and alpha >= 1e-5; variance q alpha^2. Catches swapped
+    axes, a missing half-voxel offset and a Gaussian/Beta branch mix-up."""
+    mean = (0.13, -0.21, 1.92)
+    s = (0.7, 0.45, 0.3)
+    rho, beta, q = 0.9, 2.5, 0.07
+    shape = (3, 5, 4)  # (D, H, W)
+    extent, center = (1.6, 1.1, 0.75), (0.05, -0.1, 2.0)
+    vol, var = O.voxelize(torch.tensor([mean]), torch.tensor([[1.0, 0, 0, 0]]), torch.tensor([s]),
+                          torch.tensor([rho]), shape, extent, center, beta=torch.tensor([beta]),
+                          uncertainty=torch.tensor([q]), eps=0.0)
+    D, H, W = shape
+    kept = 0
+    for z in range(D):
+        for y in range(H):
+            for x in range(W):
+                p = [center[k] + ((i + 0.5) / n - 0.5) * extent[k] for k, (i, n) in enumerate(((x, W), (y, H), (z, D)))]
+                d2 = sum((p[k] - mean[k]) ** 2 / s[k] ** 2 for k in range(3))
+                a = rho * (1 - d2) ** beta if 0 <= d2 < 1 else 0.0
+                a = a if a >= 1e-5 else 0.0
+                kept += a > 0
+                assert abs(vol[z, y, x].item() - a) < 1e-14, (z, y, x)
+                assert abs(var[z, y, x].item() - q * a * a) < 1e-15
+    assert 0 < kept < D * H * W  # both branches of the support test are exercised
+
+
+def test_voxelize_cutoffs_1e6_gaussian_1e5_beta():
+    """The thresholds themselves: a single voxel at distance d2 from the
+    primitive is kept iff alpha >= 1e-6 (Gaussian, VoxelizeToVoxelsFwd.cu:213)
+    or alpha >= 1e-5 (Beta, :575)."""
+    s = 0.5
+
+    def one(d2, beta=None):
+        dx = s * math.sqrt(d2)  # offset along x giving that d2
+        b = None if beta is None else torch.tensor([beta])
+        return O.voxelize(torch.tensor([[dx, 0.0, 2.0]]), torch.tensor([[1.0, 0, 0, 0]]), torch.tensor([[s, s, s]]),
+                          torch.tensor([1.0]), (1, 1, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 2.0), beta=b, eps=0.0).item()
+
+    g_edge = -2 * math.log(1e-6)  # e^(-d2/2) = 1e-6
+    assert one(g_edge * (1 - 1e-9)) > 0 and one(g_edge * (1 + 1e-9)) == 0.0
+    beta = 2.0
+    b_edge = 1 - 1e-5 ** (1 / beta)  # (1 - d2)^2 = 1e-5
+    assert one(b_edge * (1 - 1e-9), beta) > 0 and one(b_edge * (1 + 1e-9), beta) == 0.0
+    assert one(1.0, 0.5) == 0.0 and one(0.999999, 0.5) > 0  # support ends at d2 = 1

@@ -83,3 +83,35 @@ def test_job_stalls_when_its_capacity_cannot_grow_and_resumes(device, monkeypatc
     assert len(calls) >= 2  # one failed growth, then a successful retry
     assert [f[0] for f in stats.finished] == [0]
     assert torch.equal(stats.results[0], expected)
+
+
+def test_job_survives_a_failed_recapture_after_growth(device, monkeypatch):
+    """The capacity grows, then re-capturing the step runs out of device memory
+    once: the skipped steps stay counted (not lost), the next synchronisation
+    re-captures without growing again, replays them, and the job ends bitwise
+    equal to the job fitted alone."""
+    req = FitRequest(0, 0.0, 800, 3, 48, 117, True, 6)
+    expected = _alone(req, device)
+    monkeypatch.setattr(GaussianFit, "_measure_capacity", lambda self: 1024)  # the first step overflows
+    events = []
+    capture, regrow = GaussianFit.capture, GaussianFit.regrow
+
+    def flaky_capture(self):
+        if events == ["capture", "grow"]:  # the re-capture right after the growth
+            events.append("capture failed")
+            raise torch.OutOfMemoryError("simulated: no device memory to re-capture the step")
+        events.append("capture")
+        return capture(self)
+
+    def counted_regrow(self):
+        events.append("grow")
+        return regrow(self)
+
+    monkeypatch.setattr(GaussianFit, "capture", flaky_capture)
+    monkeypatch.setattr(GaussianFit, "regrow", counted_regrow)
+    server = FitServer(device, n_streams=1, adam=ADAM)
+    stats = server.run([req], clock="ticks", keep_results=True)
+    # one growth, the failed capture, then a re-capture without a second growth
+    assert events[:4] == ["capture", "grow", "capture failed", "capture"], events
+    assert [f[0] for f in stats.finished] == [0]
+    assert torch.equal(stats.results[0], expected)
