@@ -26,14 +26,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fwd+bwd iters/s (100k Gauss, 64×256² views) vs plain path; p99 fit latency"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
-# Reference-equivalent FP32 operations per pixel-primitive pair, counted from
-# the reference's per-pair arithmetic with an exp as one operation: forward 16
-# (delta 2, sigma 9, negate+exp 2, opacity*mu*e 2, accumulate 1;
-# RasterizeToPixelsFwd.cu:160-179), backward 46 (sigma recompute 13, alpha 2,
-# v_sigma 2, v_xy 12, v_conic 11, v_opacity 3, v_mu 3;
-# RasterizeToPixelsBwd.cu:430-468).
+# FP32 vector peak without packed FP32 (which every kernel here is built
+# without): 256 CUs x 4 SIMDs x 16 lanes x 2 FLOP per FMA x 2.4 GHz.
+FP32_PEAK_TFLOPS = 78.6
+# Reference-equivalent FP32 operations per pixel-primitive pair that passes the
+# cut-off, counted from the reference's per-pair arithmetic with an exp as one
+# operation: forward 16 (delta 2, sigma 9, negate+exp 2, opacity*mu*e 2,
+# accumulate 1; RasterizeToPixelsFwd.cu:160-179), backward 46 (sigma
+# recompute 13, alpha 2, v_sigma 2, v_xy 12, v_conic 11, v_opacity 3, v_mu 3;
+# RasterizeToPixelsBwd.cu:430-468). Pairs the cut-off drops contribute nothing
+# and are not counted as work.
 FLOP_PER_PAIR = {"rasterize_fwd": 16.0, "rasterize_bwd": 46.0}
+# serving: the plain per-job path is compared with the server's step over this many iterations
+PLAIN_CHECK_STEPS = 5
 # Vector (VALU) instructions one launch of each fused raster kernel issues on
 # this workload (PMC SQ_INSTS_VALU, summed over the device, from
 # `rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU ... --kernel-include-regex fused_raster`
@@ -67,6 +72,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
     p.add_argument("--no-graph", action="store_true", help="time the eager step instead of its hipGraph replay")
+    p.add_argument("--serve-window", type=float, default=10.0,
+                   help="config1: seconds of Poisson fit requests (4 req/s) served for the p50/p99 fit latency "
+                        "(0 skips it)")
     p.add_argument("--workload", default="config1", choices=["config1", "config2", "batched", "serving"],
                    help="config1: the metric workload; batched: 32 uneven jobs with Adam in one graph; "
                         "serving: Poisson-arriving fit requests through the continuous-batching server; "
@@ -146,31 +154,31 @@ def main():
     # intersections of this workload (fixed across steps: the parameters do not change)
     from r2splat_amd import hip as ops
 
-    with torch.no_grad():
-        covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-        m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width,
-                                                         sc.height, 1e-8, 3.0)
-        counts, _, _ = ops.isect_tiles(m2, radii, 16, 16, 16)
-        n_isects = int(counts.sum().item())
-    pairs = n_isects * 16 * 16
+    pc = pair_counts(sc)
+    n_isects = pc["intersections"]
     kernel_stages = {k: v for k, v in stages.items() if k in FLOP_PER_PAIR}
     dominant = max(kernel_stages, key=lambda k: kernel_stages[k][0])
     dom_ms = kernel_stages[dominant][0]
-    achieved = pairs * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
+    kept = pc["kept_bwd"] if dominant == "rasterize_bwd" else pc["kept_fwd"]
+    achieved = kept * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
     issue = VALU_INSTS[dominant] / (dom_ms * 1e-3 * CLOCK_HZ * SIMDS)
     fetch, write = PMC_KIB[dominant]
     traffic = int((fetch * PMC_SCALE[0] + write * PMC_SCALE[1]) * 1024)
     roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "traffic_gbs": round(traffic / (dom_ms * 1e-3) / 1e9, 1),
-                "algorithmic_work": f"{pairs:.4e} pixel-primitive pairs x {FLOP_PER_PAIR[dominant]:.0f} FLOP",
+                "algorithmic_work": f"{kept:.4e} pixel-primitive pairs that pass the cut-off x "
+                                    f"{FLOP_PER_PAIR[dominant]:.0f} FLOP (of {pc['list_pairs']:.4e} pairs in the "
+                                    f"intersected tiles)",
                 "avg_launch_ms": round(dom_ms, 4),
                 "issued": {"valu_insts": VALU_INSTS[dominant], "per_simd_cycle": round(issue, 4),
                            "note": "vector instructions the kernel issues per SIMD per cycle (measured count / "
-                                   "(launch time x 2.4 GHz x 1,024 SIMDs)); the kernel evaluates only the rows "
-                                   "each record reaches, so pairs x FLOP/pair overstates its work"}}
+                                   "(launch time x 2.4 GHz x 1,024 SIMDs)); a 16-lane SIMD completes one "
+                                   "full-rate wave64 instruction per 4 cycles"}}
 
     voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
+    latency = fit_latency(args.rate, args.serve_window, args.serve_steps, args.streams) if args.serve_window > 0 \
+        else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -192,6 +200,7 @@ def main():
         "config": {"workload": "config1: N=100000 Gaussians, C=64 views of 256x256, tile 16, MSE fwd+bwd "
                                "through rasterize_fwd/rasterize_bwd",
                    "path": args.path, "n_isects": n_isects},
+        "pairs": pc,
         "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
         "eager_path": {"value": round(1e3 / eager_ms, 3), "unit": "iters/s", "ms_per_step": round(eager_ms, 3),
                        "description": "the same fused step issued eagerly (one host overflow check per step)"},
@@ -202,6 +211,7 @@ def main():
                            "host-side intersection sizing)",
             "stages_ms": {k: round(v[0], 4) for k, v in plain[1].items()}},
         "voxelize_128": voxel,
+        "fit_latency": latency,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -392,14 +402,15 @@ def serving_main(args):
     in [10k, 400k], C in {16, 32, 64, 128}, side in {128, 256, 512}, Gaussian
     or Beta with p = 0.5, `serve-steps` Adam iterations each) served by the
     continuous-batching server against the device's HBM; time-to-finished-fit
-    p50/p99 from arrival, served iterations/s, and the final MSE of a few
-    requests against the same requests fitted on the plain per-job path
-    (explicit API over the staged operators, torch.optim.Adam)."""
+    p50/p99 from arrival and served iterations/s. A few requests are also
+    fitted on the plain per-job path (explicit API over the staged operators,
+    torch.optim.Adam) and compared, parameter for parameter, with the server's
+    captured step fitted alone over their first iterations."""
     import torch
 
     from r2splat_amd import explicit
     from r2splat_amd.serving import ExplicitFit, FitServer, build_job, poisson_requests
-    from r2splat_amd.train import AdamConfig
+    from r2splat_amd.train import AdamConfig, GaussianFit
 
     device = torch.device("cuda:0")
     adam = AdamConfig(lr=1e-2, min_scale=1e-6)
@@ -415,18 +426,34 @@ def serving_main(args):
     lat = stats.latencies()
     final = {f[0]: f[4] for f in stats.finished}
 
+    # the plain per-job path (explicit API over the staged operators, torch Adam) against the same
+    # request fitted alone by the server's captured step, over the first few iterations: 200 steps of
+    # Adam at lr 1e-2 amplify rounding into different (equally valid) fits, the first steps do not
     plain = []
     explicit.BACKEND = "ops"
     for req in reqs[:max(0, args.plain_jobs)]:
-        fit = ExplicitFit(build_job(req, device), adam)
+        job = build_job(req, device)
+        fit = ExplicitFit(job, adam)
         torch.cuda.synchronize()
         s0 = time.perf_counter()
-        for _ in range(req.steps):
+        for _ in range(PLAIN_CHECK_STEPS):
             fit.step()
         torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - s0) * 1e3 / PLAIN_CHECK_STEPS
+        explicit.BACKEND = "fused"
+        alone = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
+                            job["target"], adam=adam, beta=job["beta"])
+        alone.capture()
+        alone.step(PLAIN_CHECK_STEPS)
+        alone.sync()
+        explicit.BACKEND = "ops"
+        ref = torch.cat([q.detach().reshape(-1) for q in fit.params])
+        err = float((alone.params.double() - ref.double()).abs().max() / ref.double().abs().max())
         plain.append({"rid": req.rid, "n": req.n, "views": req.views, "side": req.side, "beta": req.beta,
-                      "final_mse_served": round(final[req.rid], 8), "final_mse_plain": round(float(fit.loss), 8),
-                      "plain_ms_per_iter": round((time.perf_counter() - s0) * 1e3 / req.steps, 3)})
+                      "steps_compared": PLAIN_CHECK_STEPS, "param_rel_err": float(f"{err:.3e}"),
+                      "final_mse_served": round(final[req.rid], 8),
+                      "plain_ms_per_iter": round(plain_ms, 3)})
+        del alone, fit, job
         print(f"[serving] plain path request {req.rid} done", file=sys.stderr, flush=True)
     explicit.BACKEND = "fused"
 
@@ -459,6 +486,28 @@ def serving_main(args):
         "plain_path": plain,
     }
     print(json.dumps(result), flush=True)
+
+
+def fit_latency(rate, duration, steps, streams):
+    """p50 / p99 time-to-finished-fit of Poisson fit requests (the serving
+    workload's request mix) over a short arrival window, served by the
+    continuous-batching server on this GPU."""
+    import torch
+
+    from r2splat_amd.serving import FitServer, poisson_requests
+    from r2splat_amd.train import AdamConfig
+
+    reqs = poisson_requests(rate, duration, seed=0, steps=steps)
+    server = FitServer(torch.device("cuda:0"), n_streams=streams, adam=AdamConfig(lr=1e-2, min_scale=1e-6))
+    stats = server.run(reqs)
+    del server
+    torch.cuda.empty_cache()
+    return {"p50_s": round(stats.percentile(50), 3), "p99_s": round(stats.percentile(99), 3),
+            "served_iters_per_s": round(stats.job_steps / stats.wall, 2), "requests": len(reqs),
+            "finished": len(stats.finished), "wall_s": round(stats.wall, 2),
+            "window": f"Poisson {rate:g} req/s over {duration:g} s (seed 0): N log-uniform [10k, 400k], C in "
+                      f"{{16,32,64,128}}, side in {{128,256,512}}, Gaussian or Beta (p=0.5), {steps} Adam "
+                      f"iterations (lr 1e-2) each; latency from arrival to finished fit"}
 
 
 def graph_timing(step, steps, warmup):
@@ -522,6 +571,40 @@ def voxel_timing(sc, steps):
             "stages_ms": {k: round(v[0], 4) for k, v in profiling.collect().items()}}
 
 
+def _popcount(words):
+    """Set bits of an int32 tensor (SWAR in int64)."""
+    x = words.long() & 0xFFFFFFFF
+    x = x - ((x >> 1) & 0x55555555)
+    x = (x & 0x33333333) + ((x >> 2) & 0x33333333)
+    x = (x + (x >> 4)) & 0x0F0F0F0F
+    return int((((x * 0x01010101) & 0xFFFFFFFF) >> 24).sum().item())
+
+
+def pair_counts(sc):
+    """Pixel-primitive pairs of the workload: in the intersected tiles
+    (intersections x 256) and those the fused forward and backward keep at the
+    cut-off (their keep masks, r2s_fused_keep_masks; 256^2 images have no
+    pixels past the edge of a 16-pixel tile)."""
+    import torch
+
+    from r2splat_amd import fused
+    from r2splat_amd.hip import _backend as be
+
+    with torch.no_grad():
+        fwd = fused.prepare(*sc.params(), sc.viewmats, sc.Ks, sc.width, sc.height, 1e-8, 3.0)
+        n, T = fwd.n_isects, fwd.tile_size
+        words = T * T // 32
+        mf = torch.empty((max(n, 1), words), dtype=torch.int32, device=sc.means.device)
+        mb = torch.empty_like(mf)
+        be.call("r2s_fused_keep_masks", be.ptr(fwd.packed), be.ptr(fwd.offsets), be.ptr(fwd.counters), fwd.capacity,
+                sc.viewmats.shape[0], sc.width, sc.height, T, fwd.kind, be.ptr(mf), be.ptr(mb),
+                be.stream_of(sc.means.device))
+        out = {"intersections": n, "list_pairs": n * T * T, "kept_fwd": _popcount(mf[:n]),
+               "kept_bwd": _popcount(mb[:n])}
+        del fwd, mf, mb
+    return out
+
+
 def cpu_baseline(sc, target, views, min_seconds):
     import numpy as np
 
@@ -529,7 +612,10 @@ def cpu_baseline(sc, target, views, min_seconds):
 
     C = sc.viewmats.shape[0]
     views = max(1, min(views, C))
-    threads = min(cb.host_cores(), 16)
+    cores = cb.host_cores()
+    # the box allots a share of its cores to each GPU and says so in OMP_NUM_THREADS
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(cores, share) if share > 0 else cores
     params = [t.detach().cpu().numpy() for t in sc.params()]
     vm = sc.viewmats[:views].cpu().numpy()
     K = sc.Ks[:views].cpu().numpy()
@@ -543,8 +629,22 @@ def cpu_baseline(sc, target, views, min_seconds):
             break
     per_iter = dt / reps * C / views
     return {"value": round(1.0 / per_iter, 5), "unit": "iters/s", "cores": threads, "kind": "port",
+            "host": {"cpu_model": _cpu_model(), "cores_visible": cores,
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
             "sample": f"{reps} fwd+MSE+bwd iterations over {views} of {C} views x all {sc.means.shape[0]} "
-                      f"Gaussians ({dt:.1f} s), scaled by {C}/{views}"}
+                      f"Gaussians ({dt:.1f} s), scaled by {C}/{views}; {threads} OpenMP threads (all the "
+                      f"cores this process may use: the box's per-GPU CPU share)"}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

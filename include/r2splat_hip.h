@@ -37,10 +37,11 @@ typedef struct ihipStream_t* r2s_stream_t; /* identical to hipStream_t */
 #define R2S_F32 0
 #define R2S_F64 1
 
-/* Library identification and error reporting. Version 2: the fused entry
- * points take the record kind (Beta, uncertainty) and the variance buffers,
- * and `packed` holds tile-list entries followed by the projected records
- * (r2s_fused_packed_bytes). */
+/* Library identification and error reporting. Version 3: the fused voxel
+ * entry points take uncertainties and the variance buffers; the TV and L1
+ * loss backward take lambda as a float (version 2: the fused raster entry
+ * points take the record kind and the variance buffers, and `packed` holds
+ * tile-list entries followed by the projected records). */
 int r2s_abi_version(void);
 const char* r2s_last_error(void);
 
@@ -334,8 +335,9 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * = intersections, counters[1] = 1 when capacity was exceeded (that
  * forward's outputs are then incomplete and the call must be repeated with
  * a larger capacity). isect_grads: float [capacity * 8], one VJP partial per
- * intersection, summed per primitive in a fixed order (the backward is
- * deterministic and uses no atomics).
+ * intersection (6 floats for Gaussians without uncertainty, 8 otherwise),
+ * summed per primitive in a fixed order (the backward is deterministic and
+ * uses no atomics).
  *
  * Primitive kind: betas == NULL selects the Gaussian profile (explicit.py
  * with beta=None). Otherwise betas [N] is the per-primitive beta (projected
@@ -398,6 +400,17 @@ int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offsets, const 
                          const uint32_t* counters, uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
                          uint32_t tile_size, uint32_t waves, int kind, const float* v_image,
                          const float* v_image_var, int detach, float* isect_grads, r2s_stream_t stream);
+/* Diagnostic, no counterpart in the reference: the (list entry, pixel) pairs
+ * that r2s_fused_render and r2s_fused_raster_vjp keep at the cut-off, decided
+ * by the same operations as those kernels. keep_fwd and keep_bwd are uint32
+ * [counters[0]][tile_size^2 / 32]: bit b of word w of entry i is pixel
+ * p = 32 w + b = row * tile_size + column of entry i's tile (pixels past the
+ * image edge included). For tests that evaluate exactly the kept pairs in
+ * float64, separating arithmetic error from pairs at the threshold. */
+int r2s_fused_keep_masks(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                         uint32_t capacity, uint32_t C, uint32_t width, uint32_t height, uint32_t tile_size,
+                         int kind, uint32_t* keep_fwd, uint32_t* keep_bwd, r2s_stream_t stream);
+
 /* The parameter VJP sums each primitive's cameras inside one kernel in a
  * fixed order and needs no workspace: r2s_fused_param_workspace_bytes returns
  * 0 and the workspace arguments may be NULL / 0 (kept for callers that pass
@@ -412,34 +425,51 @@ int r2s_fused_param_vjp(const float* means, const float* quats, const float* sca
                         float* v_means, float* v_quats, float* v_scales, float* v_densities,
                         float* v_betas, float* v_uncertainties, r2s_stream_t stream);
 
-/* Fused float32 voxel VJP (mean only, 8^3 tiles): the whole per-primitive
- * voxelize backward (VoxelizeToVoxelsBwd.cu per-Gaussian kernel, as called by
- * explicit.py:327-350) without atomics. radii int32 [N,3] and tile_cum uint32
- * [N+1] (exclusive scan of the per-primitive tile counts) are the binning's
- * own (r2s_isect_tiles_3d_count); isect_ids / flatten_ids its sorted output.
- * One partial per intersection is written at its emission index (partials:
+/* Fused float32 voxel forward (8^3 tiles) over the 3D binning: render
+ * [dim_z, dim_y, dim_x] and, with uncertainties [N] and render_var, the
+ * variance sum q alpha^2 (both overwritten). Replaces voxelize_to_voxels_fwd
+ * and voxelize_to_voxels_mean_var_fwd (VoxelizeToVoxelsFwd.cu:181-219, :297
+ * and the Beta kernels) for float32; the summation order within a voxel
+ * differs from the operator (rounding only). means3d [N,3] are the voxel-frame
+ * means the binning used (r2s_world_to_volume_mapping_fwd); when the world
+ * means [N,3], ranges [3] and centers [3] of that mapping are given too (all
+ * three or none), each voxel-frame mean is recomputed in double precision and
+ * the part its float32 value rounds off enters every voxel offset, so the
+ * profile sees the mean to float32 precision of the offset rather than of the
+ * voxel coordinate (the Beta profile (1 - d^2)^beta with beta < 1 is steep at
+ * its support edge). */
+int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float* ranges,
+                           const float* centers, const float* conics, const float* opacities,
+                           const float* betas, const float* uncertainties, const int32_t* tile_offsets,
+                           const int32_t* flatten_ids, uint32_t n_isects, uint32_t N, uint32_t dim_x,
+                           uint32_t dim_y, uint32_t dim_z, uint32_t tile_size, uint32_t n_tiles_x,
+                           uint32_t n_tiles_y, uint32_t n_tiles_z, float* render, float* render_var,
+                           r2s_stream_t stream);
+
+/* Fused float32 voxel VJP (8^3 tiles): the whole per-primitive voxelize
+ * backward (VoxelizeToVoxelsBwd.cu per-Gaussian kernels, mean and mean_var, as
+ * called by explicit.py:327-350) without atomics. radii int32 [N,3] and
+ * tile_cum uint32 [N+1] (exclusive scan of the per-primitive tile counts) are
+ * the binning's own (r2s_isect_tiles_3d_count); isect_ids / flatten_ids its
+ * sorted output; means, ranges and centers as for r2s_voxelize_fused_fwd.
+ * With v_render_var the voxel adjoint becomes v_render + 2 q
+ * alpha v_render_var (v_render alone when `detach`, the reference's
+ * detach_variance) and v_uncertainties [N] = sum v_render_var alpha^2. One
+ * partial per intersection is written at its emission index (partials:
  * r2s_voxel_partials_bytes(n_isects) bytes) and summed per primitive in a
  * fixed order, so the result is bitwise reproducible. Outputs overwritten:
- * v_means3d [N,3], v_conics [N,6], v_opacities [N], v_betas [N] (iff betas). */
-/* Fused float32 voxel forward (mean only, 8^3 tiles) over the same binning:
- * render [dim_z, dim_y, dim_x] (overwritten). Replaces voxelize_to_voxels_fwd
- * (VoxelizeToVoxelsFwd.cu:181-219 and the Beta kernel) for that case; the
- * summation order within a voxel differs from the operator (rounding only). */
-int r2s_voxelize_fused_fwd(const float* means3d, const float* conics, const float* opacities,
-                           const float* betas, const int32_t* tile_offsets, const int32_t* flatten_ids,
+ * v_means3d [N,3], v_conics [N,6], v_opacities [N], v_betas [N] (iff betas),
+ * v_uncertainties [N] (iff v_render_var). */
+size_t r2s_voxel_partials_bytes(uint32_t n_isects);
+int r2s_voxelize_fused_vjp(const float* means3d, const float* means, const float* ranges,
+                           const float* centers, const float* conics, const float* opacities,
+                           const float* betas, const float* uncertainties, const int32_t* radii,
+                           const uint32_t* tile_cum, const int64_t* isect_ids, const int32_t* flatten_ids,
                            uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
                            uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
-                           float* render, r2s_stream_t stream);
-
-size_t r2s_voxel_partials_bytes(uint32_t n_isects);
-int r2s_voxelize_fused_vjp(const float* means3d, const float* conics, const float* opacities,
-                           const float* betas, const int32_t* radii, const uint32_t* tile_cum,
-                           const int64_t* isect_ids, const int32_t* flatten_ids, uint32_t n_isects,
-                           uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, uint32_t tile_size,
-                           uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
-                           const float* v_render, float* partials, size_t partials_bytes,
-                           float* v_means3d, float* v_conics, float* v_opacities, float* v_betas,
-                           r2s_stream_t stream);
+                           const float* v_render, const float* v_render_var, int detach, float* partials,
+                           size_t partials_bytes, float* v_means3d, float* v_conics, float* v_opacities,
+                           float* v_betas, float* v_uncertainties, r2s_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Training-step kernels (no counterpart in the reference, which leaves image */

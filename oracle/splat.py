@@ -259,6 +259,34 @@ def tile_mask_3d(means3d: np.ndarray, radii: np.ndarray, tile: int, shape_zyx) -
 # ---------------------------------------------------------------------------
 # rasterization (RasterizeToPixelsFwd.cu:155-180, :664-680, mean/var :318-352)
 # ---------------------------------------------------------------------------
+def _camera_alpha(mx, my, conics, opac, mus, width, height, beta2d=None, mask=None, visible=None):
+    """Kept alpha [N,H,W] of one camera's primitives at every pixel centre
+    (RasterizeToPixelsFwd.cu:155-180 Gaussian, :664-680 Beta); mx, my
+    broadcast against [N,H,W] (per-primitive [N,1,1] or per pixel [N,H,W])."""
+    dt = conics.dtype
+    yy, xx = torch.meshgrid(torch.arange(height, dtype=dt) + 0.5, torch.arange(width, dtype=dt) + 0.5,
+                            indexing="ij")
+    a, b, cc = (conics[:, k, None, None] for k in range(3))
+    amp = (opac * mus)[:, None, None]
+    if beta2d is None:
+        dx, dy = xx - mx, yy - my
+        sigma = 0.5 * (a * dx * dx + cc * dy * dy) + b * dx * dy
+        alpha = amp * torch.exp(-sigma)
+        keep = (sigma >= 0) & (alpha.abs() >= 1e-5)
+    else:
+        dx, dy = mx - xx, my - yy
+        d2 = a * dx * dx + 2 * b * dx * dy + cc * dy * dy
+        inside = (d2 >= 0) & (d2 < 1)
+        h = torch.where(inside, 1 - d2, torch.ones_like(d2))
+        alpha = torch.where(inside, amp * h.pow(beta2d[:, None, None]), torch.zeros_like(d2))
+        keep = inside & (alpha.abs() >= 1e-5)
+    if mask is not None:
+        keep = keep & torch.as_tensor(mask)
+    if visible is not None:
+        keep = keep & visible[:, None, None]
+    return torch.where(keep, alpha, torch.zeros_like(alpha))
+
+
 def rasterize(means2d, conics, opac, mus, width, height, *, beta2d=None, unc=None, detach=False, mask=None,
               visible=None):
     """Dense per-pixel sums (differentiable). Gaussian: delta = pixel - mean,
@@ -268,38 +296,40 @@ def rasterize(means2d, conics, opac, mus, width, height, *, beta2d=None, unc=Non
     sum q alpha^2 (alpha detached when `detach`). `mask` [C,N,H,W] restricts
     (pixel, primitive) pairs to intersected tiles."""
     C, N = means2d.shape[:2]
-    dt = means2d.dtype
-    yy, xx = torch.meshgrid(torch.arange(height, dtype=dt) + 0.5, torch.arange(width, dtype=dt) + 0.5,
-                            indexing="ij")
     images, variances = [], []
     for c in range(C):
-        mx = means2d[c, :, 0, None, None]
-        my = means2d[c, :, 1, None, None]
-        a, b, cc = (conics[c, :, k, None, None] for k in range(3))
-        amp = (opac[c] * mus[c])[:, None, None]
-        if beta2d is None:
-            dx, dy = xx - mx, yy - my
-            sigma = 0.5 * (a * dx * dx + cc * dy * dy) + b * dx * dy
-            alpha = amp * torch.exp(-sigma)
-            keep = (sigma >= 0) & (alpha.abs() >= 1e-5)
-        else:
-            dx, dy = mx - xx, my - yy
-            d2 = a * dx * dx + 2 * b * dx * dy + cc * dy * dy
-            inside = (d2 >= 0) & (d2 < 1)
-            h = torch.where(inside, 1 - d2, torch.ones_like(d2))
-            alpha = torch.where(inside, amp * h.pow(beta2d[c][:, None, None]), torch.zeros_like(d2))
-            keep = inside & (alpha.abs() >= 1e-5)
-        if mask is not None:
-            keep = keep & torch.as_tensor(mask[c])
-        if visible is not None:
-            keep = keep & visible[c][:, None, None]
-        alpha = torch.where(keep, alpha, torch.zeros_like(alpha))
+        alpha = _camera_alpha(means2d[c, :, 0, None, None], means2d[c, :, 1, None, None], conics[c], opac[c], mus[c],
+                              width, height, None if beta2d is None else beta2d[c],
+                              None if mask is None else mask[c], None if visible is None else visible[c])
         images.append(alpha.sum(0))
         if unc is not None:
             branch = alpha.detach() if detach else alpha
             variances.append((unc[:, None, None] * branch.square()).sum(0))
     image = torch.stack(images)
     return image if unc is None else (image, torch.stack(variances))
+
+
+def abs_mean2d_grad(means2d, conics, opac, mus, width, height, v, *, beta2d=None, unc=None, v_var=None,
+                    detach=False, mask=None):
+    """The absgrad output [C,N,2]: sum over pixels of |d(G alpha)/d mean2d| per
+    component, G = v (plus 2 q alpha v_var unless detached when a variance
+    adjoint is given) held constant per pixel (RasterizeToPixelsBwd.cu:219-221,
+    250-255, 455-466). Per-pixel partials come from autograd through
+    per-pixel copies of the means."""
+    C, N = means2d.shape[:2]
+    out = torch.zeros(C, N, 2, dtype=means2d.dtype)
+    for c in range(C):
+        mx = means2d[c, :, 0, None, None].detach().expand(N, height, width).clone().requires_grad_()
+        my = means2d[c, :, 1, None, None].detach().expand(N, height, width).clone().requires_grad_()
+        alpha = _camera_alpha(mx, my, conics[c].detach(), opac[c].detach(), mus[c].detach(), width, height,
+                              None if beta2d is None else beta2d[c].detach(), None if mask is None else mask[c])
+        G = v[c].expand_as(alpha)
+        if v_var is not None and unc is not None and not detach:
+            G = G + 2 * unc.detach()[:, None, None] * alpha.detach() * v_var[c]
+        gx, gy = torch.autograd.grad((G.detach() * alpha).sum(), (mx, my))
+        out[c, :, 0] = gx.abs().sum((1, 2))
+        out[c, :, 1] = gy.abs().sum((1, 2))
+    return out
 
 
 def rasterize_scene(means, quats, scales, densities, viewmats, Ks, image_size, *, beta=None, uncertainty=None,

@@ -67,7 +67,12 @@ using Rec = typename std::conditional<BETA || VAR, BetaSplat, Splat>::type;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
 constexpr int kDefaultWaves = 8;                     // waves sharing one tile unless chosen otherwise
-constexpr int kPartialStride = 8;                    // floats per intersection partial (6 used; 32-byte aligned)
+// Floats per intersection partial: (v_mx, v_my, v_a, v_b, v_c, v_w) for plain
+// Gaussians, packed back to back (8-byte aligned); Beta and uncertainty kinds
+// add v_b2d and v_q in a 32-byte slot.
+template <bool BETA, bool VAR>
+constexpr int partial_stride() { return (BETA || VAR) ? 8 : 6; }
+constexpr int kPartialStride = 8;  // the widest partial (the buffer is sized for it)
 
 __device__ __forceinline__ uint32_t qlim_of(float w) {
     const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
@@ -82,6 +87,20 @@ __device__ __forceinline__ uint32_t beta_qlim_of(float w, float b2d) {
 }
 
 __device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
+
+// The record's quadratic form q (Gaussian: sigma log2 e; Beta: d^2) at one
+// pixel, in the two evaluation orders of the raster kernels. The forward
+// shares the column terms of a lane's pixels (adx2 = A dx dx, bdx = B dx with
+// dx = px - mx, dy = py - my); the backward shares the row terms of a row's
+// pixels (bdy = B dy, cdy2 = C dy dy with dy = (y0 + r) - my and dx =
+// (x0 - mx) + j). Which pixels pass the cut-off is decided by these exact
+// operations (fused_keep_mask_kernel reproduces both).
+__device__ __forceinline__ float form_by_columns(float adx2, float bdx, float C, float dy) {
+    return fmaf(dy, fmaf(C, dy, bdx), adx2);
+}
+__device__ __forceinline__ float form_by_rows(float A, float dx, float bdy, float cdy2) {
+    return fmaf(dx, fmaf(A, dx, bdy), cdy2);
+}
 
 // Live intersection count; an overflowed forward (count > capacity) has an
 // incomplete record list, so every consumer treats it as empty.
@@ -733,7 +752,7 @@ __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, floa
     for (int k = 0; k < PX; ++k) {
         if (!all && !((rows >> k) & 1u)) continue;
         const float dy = py[k] - s.my;
-        const float q = fmaf(dy, fmaf(s.C, dy, bdx), adx2);  // Gaussian: sigma log2 e; Beta: d^2
+        const float q = form_by_columns(adx2, bdx, s.C, dy);  // Gaussian: sigma log2 e; Beta: d^2
         float e;
         if constexpr (BETA)
             e = __builtin_amdgcn_exp2f(s.b2d * __log2f(1.f - q));
@@ -989,7 +1008,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
 #pragma unroll
             for (int j = 0; j < T; ++j) {
                 const float dx = dx0 + float(j);
-                const float q = fmaf(dx, fmaf(s.A, dx, bdy), cdy2);
+                const float q = form_by_rows(s.A, dx, bdy, cdy2);
                 float g;
                 if constexpr (BETA) {
                     // h <= 0 outside the support: log2 h is -inf or NaN there, so it is
@@ -1033,21 +1052,82 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
             myy = fmaf(dy * dy, r0, myy);
         }
         if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
-            float4* gp = reinterpret_cast<float4*>(isect_grads + kPartialStride * size_t(s.g));
+            float* out = isect_grads + partial_stride<BETA, VAR>() * size_t(s.g);
             const float wq = s.w * s.w * vq;  // sum v_var alpha^2 (the uncertainty's gradient)
             if constexpr (BETA) {
                 const float a = s.A, b = 0.5f * s.B, c = s.C, k = s.w * s.b2d;
+                float4* gp = reinterpret_cast<float4*>(out);
                 gp[0] = make_float4(k * (2.f * a * mx + 2.f * b * my), k * (2.f * b * mx + 2.f * c * my), -k * mxx,
                                     -2.f * k * mxy);
                 gp[1] = make_float4(-k * myy, se, s.w * 0.6931471805599453f * sl, wq);
             } else {
                 // conic entries in pixel units: a = 2 A / log2 e, b = B / log2 e, c = 2 C / log2 e
                 const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
-                gp[0] = make_float4(s.w * (a * mx + b * my), s.w * (b * mx + c * my), -0.5f * s.w * mxx,
-                                    -s.w * mxy);
-                gp[1] = make_float4(-0.5f * s.w * myy, m0, 0.f, wq);
+                const float g0 = s.w * (a * mx + b * my), g1 = s.w * (b * mx + c * my), g2 = -0.5f * s.w * mxx,
+                            g3 = -s.w * mxy, g4 = -0.5f * s.w * myy;
+                if constexpr (VAR) {
+                    float4* gp = reinterpret_cast<float4*>(out);
+                    gp[0] = make_float4(g0, g1, g2, g3);
+                    gp[1] = make_float4(g4, m0, 0.f, wq);
+                } else {  // 24 bytes, three 8-byte stores
+                    float2* gp = reinterpret_cast<float2*>(out);
+                    gp[0] = make_float2(g0, g1);
+                    gp[1] = make_float2(g2, g3);
+                    gp[2] = make_float2(g4, m0);
+                }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Keep masks (diagnostic): which (list entry, pixel) pairs pass the cut-off in
+// the forward and in the backward, decided by the same operations as those
+// kernels (row_span, their row-group / row skipping, form_by_columns /
+// form_by_rows, contributes). One workgroup per tile; one thread per
+// (entry, 32-pixel word). Bit b of word w of an entry is pixel p = 32 w + b =
+// row * T + column of the entry's tile. Tests evaluate the kept pairs in
+// float64 to separate arithmetic error from pairs at the cut-off.
+// ---------------------------------------------------------------------------
+template <int T, bool BETA, bool VAR>
+__global__ void __launch_bounds__(256)
+fused_keep_mask_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+                       const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                       uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t* __restrict__ keep_fwd,
+                       uint32_t* __restrict__ keep_bwd) {
+    constexpr int WORDS = T * T / 32, STEP = kWave / T;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t n_live = live(d_n, capacity);
+    int32_t start, end;
+    fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
+    const uint32_t nt = ntx * nty, t = tile - (tile / nt) * nt;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    const float y_base = float(ty * T);
+    const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f;
+    for (int64_t i = int64_t(start) * WORDS + threadIdx.x; i < int64_t(end) * WORDS; i += blockDim.x) {
+        const int32_t entry = int32_t(i / WORDS), word = int32_t(i % WORDS);
+        const Rec<BETA, VAR> s = recs[entries[entry].x];
+        int r_lo, r_hi;
+        row_span(s, y_base, T, r_lo, r_hi);
+        const int k_lo = r_lo / STEP, k_hi = (r_hi + STEP - 1) / STEP;  // the forward's row groups
+        uint32_t bits_fwd = 0u, bits_bwd = 0u;
+        for (int b = 0; b < 32; ++b) {
+            const int pix = word * 32 + b, r = pix / T, col = pix % T;
+            if (r_lo < r_hi && r / STEP >= k_lo && r / STEP < k_hi) {
+                const float dx = (float(tx * T + uint32_t(col)) + 0.5f) - s.mx;
+                const float adx2 = s.A * dx * dx, bdx = s.B * dx;
+                const float dy = (float(ty * T + uint32_t(r)) + 0.5f) - s.my;
+                if (contributes(form_by_columns(adx2, bdx, s.C, dy), s.qlim)) bits_fwd |= 1u << b;
+            }
+            if (r >= r_lo && r < r_hi) {
+                const float dx = (x0 - s.mx) + float(col);
+                const float dy = y0 + float(r) - s.my;
+                const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
+                if (contributes(form_by_rows(s.A, dx, bdy, cdy2), s.qlim)) bits_bwd |= 1u << b;
+            }
+        }
+        keep_fwd[i] = bits_fwd;
+        keep_bwd[i] = bits_bwd;
     }
 }
 
@@ -1124,27 +1204,31 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             for (uint32_t base = wlo; base < whi; base += kVjpStageParts) {
                 const uint32_t cnt = min(kVjpStageParts, whi - base);
                 wave_lds_sync();  // every lane is done with the previous chunk
-                for (uint32_t i = lane; i < cnt; i += kWave) {
-                    const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(base + i));
-                    float4* sp = reinterpret_cast<float4*>(stage + kPartialStride * i);
-                    sp[0] = gp[0];
-                    if constexpr (BETA || VAR)
+                if constexpr (BETA || VAR) {
+                    for (uint32_t i = lane; i < cnt; i += kWave) {
+                        const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(base + i));
+                        float4* sp = reinterpret_cast<float4*>(stage + kPartialStride * i);
+                        sp[0] = gp[0];
                         sp[1] = gp[1];
-                    else
-                        *reinterpret_cast<float2*>(sp + 1) = *reinterpret_cast<const float2*>(gp + 1);
+                    }
+                } else {  // the range's packed 6-float partials as a flat run of 8-byte pairs
+                    const float2* src = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(base));
+                    float2* dst = reinterpret_cast<float2*>(stage);
+                    for (uint32_t i = lane; i < 3 * cnt; i += kWave) dst[i] = src[i];
                 }
                 wave_lds_sync();
                 const uint32_t j1 = min(hi, base + cnt);
                 for (uint32_t j = max(lo, base); j < j1; ++j) {
-                    const float4* gp = reinterpret_cast<const float4*>(stage + kPartialStride * (j - base));
-                    const float4 a = gp[0];
-                    g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
                     if constexpr (BETA || VAR) {
-                        const float4 b = gp[1];
+                        const float4* gp = reinterpret_cast<const float4*>(stage + kPartialStride * (j - base));
+                        const float4 a = gp[0], b = gp[1];
+                        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
                         g4 += b.x; g5 += b.y; g6 += b.z; g7 += b.w;
                     } else {
-                        const float2 b = *reinterpret_cast<const float2*>(gp + 1);
-                        g4 += b.x; g5 += b.y;
+                        const float2* gp = reinterpret_cast<const float2*>(stage + 6 * (j - base));
+                        const float2 a = gp[0], b = gp[1], c = gp[2];
+                        g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y;
+                        g4 += c.x; g5 += c.y;
                     }
                 }
             }
@@ -1380,6 +1464,18 @@ struct LaunchBwd {
     }
 };
 
+template <int T, int WAVES, bool BETA, bool VAR>
+struct LaunchKeep {
+    static void run(uint32_t n_tiles, hipStream_t stream, const void* packed, const int32_t* offsets,
+                    const uint32_t* counters, uint32_t capacity, uint32_t ntx, uint32_t nty, uint32_t* keep_fwd,
+                    uint32_t* keep_bwd) {
+        fused_keep_mask_kernel<T, BETA, VAR><<<n_tiles, 256, 0, stream>>>(
+            static_cast<const uint2*>(packed),
+            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            offsets, counters, capacity, n_tiles, ntx, nty, keep_fwd, keep_bwd);
+    }
+};
+
 template <bool BETA, bool VAR>
 static int prepare_impl(const float* means, const float* quats, const float* scales, const float* densities,
                         const float* betas, const float* beta_ratio, const float* uncertainties,
@@ -1552,6 +1648,22 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     launch_tiled<LaunchBwd>(tile_size, waves, kind, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
                             counters, capacity, ntx, nty, width, height, v_image, v_image_var, detach, isect_grads);
     return check_launch("fused_raster_bwd");
+}
+
+extern "C" int r2s_fused_keep_masks(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,
+                                    uint32_t capacity, uint32_t C, uint32_t width, uint32_t height,
+                                    uint32_t tile_size, int kind, uint32_t* keep_fwd, uint32_t* keep_bwd,
+                                    r2s_stream_t stream) {
+    if (int rc = check_tile(tile_size, 0)) return rc;
+    R2S_REQUIRE(keep_fwd && keep_bwd, "keep_masks: both mask buffers are required");
+    const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
+    const uint64_t n_tiles = uint64_t(C) * ntx * nty;
+    R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles for one launch");
+    if (n_tiles == 0) return R2S_OK;
+    // the wave count is not part of the decisions; the 8-wave instantiation is a placeholder
+    launch_tiled<LaunchKeep>(tile_size, 8, kind, uint32_t(n_tiles), stream, packed, tile_offsets, counters,
+                             capacity, ntx, nty, keep_fwd, keep_bwd);
+    return check_launch("fused_keep_masks");
 }
 
 // The one-pass parameter VJP needs no workspace; the size query and the

@@ -106,6 +106,12 @@ template <typename S>
 struct VoxArgs {
     const S* means3d; const S* conics; const S* opac; const S* betas; const S* unc;
     const int32_t* offsets; const int32_t* flatten; uint32_t n_isects, N, dx, dy, dz, T, ntx, nty, ntz;
+    // fused float32 kernels: the world-frame means and the volume frame, from
+    // which each voxel-frame mean is recomputed in double precision to carry
+    // the part that its float32 value (means3d) rounds off
+    const float* world = nullptr;
+    const float* ranges = nullptr;
+    const float* centers = nullptr;
 };
 
 template <typename S>
@@ -319,7 +325,44 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-constexpr uint32_t kVoxPartial = 12;  // floats per intersection partial (11 used, 16-byte aligned)
+// Voxel-frame mean m = (world - center) dims / extent + dims / 2 split into the
+// float32 mean the binning used (means3d) and the float32 remainder lo = m -
+// means3d, with m in double precision. Voxel offsets are then formed as
+// (v - hi) - lo: the float32 rounding of the mean itself (up to half an ulp of
+// the voxel coordinate, 1.5e-5 voxels at 256) never enters the profile, whose
+// compact Beta form (1 - d^2)^beta with beta < 1 is steep at its support edge.
+struct VoxFrame {
+    double scale[3], shift[3], centre[3];
+    bool on;
+};
+
+__device__ __forceinline__ VoxFrame vox_frame(const VoxArgs<float>& A) {
+    VoxFrame f = {};
+    f.on = A.world != nullptr;
+    if (!f.on) return f;
+    const double dims[3] = {double(A.dx), double(A.dy), double(A.dz)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f.scale[k] = dims[k] / double(A.ranges[k]);
+        f.shift[k] = dims[k] * 0.5;
+        f.centre[k] = double(A.centers[k]);
+    }
+    return f;
+}
+
+__device__ __forceinline__ void vox_mean_lo(const VoxArgs<float>& A, const VoxFrame& f, size_t G, const float hi[3],
+                                            float lo[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = 0.f;
+        if (f.on) {
+            const double m = (double(A.world[3 * G + k]) - f.centre[k]) * f.scale[k] + f.shift[k];
+            lo[k] = float(m - double(hi[k]));
+        }
+    }
+}
+
+constexpr uint32_t kVoxPartial = 12;  // floats per intersection partial (11, or 12 with a variance)
 constexpr uint32_t kMomThreads = 256;
 constexpr float kL2E = 1.4426950408889634f;
 
@@ -334,14 +377,19 @@ __device__ __forceinline__ uint32_t vox_qlim(float opac, float beta, bool is_bet
     return __float_as_uint(fminf(D, 0.99999994f)) + 1u;
 }
 
-template <bool BETA>
+// VAR: the voxel adjoint becomes v + var_scale q alpha v_var (var_scale 2, or
+// 0 when the variance is detached) and the partial's twelfth slot carries
+// v_q = sum v_var alpha^2 (variance = sum q alpha^2).
+template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kMomThreads)
 vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, const float* __restrict__ v_render,
+                       const float* __restrict__ v_render_var, float var_scale,
                        float* __restrict__ v_xyz, float* __restrict__ v_c, float* __restrict__ v_opac,
                        float* __restrict__ v_beta, const int32_t* __restrict__ radii,
                        const uint32_t* __restrict__ cum, float* __restrict__ partials) {
     constexpr uint32_t T = kMomT, T3 = T * T * T;
     __shared__ __align__(16) float sv[T3];
+    __shared__ __align__(16) float svv[VAR ? T3 : 4];
     __shared__ int64_t ids[kMomThreads];
     // a chunk of kMomThreads consecutive (tile-sorted) intersections, one per thread; the chunk's
     // tiles are visited in order, each one's adjoint staged in LDS in turn
@@ -352,6 +400,7 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
     __syncthreads();
     const uint32_t n_here = e1 - e0;
     const int64_t nt = int64_t(A.ntx) * A.nty * A.ntz;
+    const VoxFrame frame = vox_frame(A);
     uint32_t pos = 0;  // first chunk entry of the current tile
     while (pos < n_here) {
         const int64_t tile = ids[pos];
@@ -366,7 +415,10 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
         if (valid_tile)
             for (uint32_t p = threadIdx.x; p < T3; p += blockDim.x) {
                 const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
-                sv[p] = (i < A.dz && j < A.dy && k < A.dx) ? v_render[(size_t(i) * A.dy + j) * A.dx + k] : 0.f;
+                const bool in = i < A.dz && j < A.dy && k < A.dx;
+                const size_t o = (size_t(i) * A.dy + j) * A.dx + k;
+                sv[p] = in ? v_render[o] : 0.f;
+                if constexpr (VAR) svv[p] = in ? v_render_var[o] : 0.f;
             }
         __syncthreads();
         if (valid_tile && mine == tile) {
@@ -375,26 +427,37 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
             if (g >= 0 && uint32_t(g) < A.N) {
                 const size_t G = size_t(g);
                 const float mx = A.means3d[3 * G], my = A.means3d[3 * G + 1], mz = A.means3d[3 * G + 2];
+                const float hi3[3] = {mx, my, mz};
+                float lo3[3];
+                vox_mean_lo(A, frame, G, hi3, lo3);
                 float P[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) P[k] = A.conics[6 * G + k];
                 const float opac = A.opac[g], beta = BETA ? A.betas[g] : 0.f;
                 const uint32_t qlim = vox_qlim(opac, beta, BETA);
+                const float kvar = VAR ? var_scale * A.unc[g] * opac : 0.f;  // d variance / d (alpha / w) factor
                 // q = a dx^2 + bxy dx dy + bxz dx dz + cyy dy^2 + cyz dy dz + czz dz^2
                 const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
                 const float a = s * P[0], bxy = t2 * P[1], bxz = t2 * P[2], cyy = s * P[3], cyz = t2 * P[4], czz = s * P[5];
                 const float bm1 = beta - 1.f;
                 float m0 = 0.f, m_x = 0.f, m_y = 0.f, m_z = 0.f, mxx = 0.f, myy = 0.f, mzz = 0.f, mxy = 0.f, mxz = 0.f,
-                      myz = 0.f, se = 0.f, sl = 0.f;
-                const float dx0 = x0 - mx;
+                      myz = 0.f, se = 0.f, sl = 0.f, vq = 0.f;
+                const float dx0 = (x0 - mx) - lo3[0];
                 if (qlim != 0u) {
                     for (uint32_t z = 0; z < T; ++z) {
-                        const float dz = z0 + float(z) - mz;
+                        const float dz = ((z0 + float(z)) - mz) - lo3[2];
                         for (uint32_t y = 0; y < T; ++y) {
-                            const float dy = y0 + float(y) - my;
+                            const float dy = ((y0 + float(y)) - my) - lo3[1];
                             const float4* row = reinterpret_cast<const float4*>(sv + (z * T + y) * T);
                             const float4 va = row[0], vb = row[1];
                             const float v[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+                            float vv[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                            if constexpr (VAR) {
+                                const float4* rowv = reinterpret_cast<const float4*>(svv + (z * T + y) * T);
+                                const float4 wa = rowv[0], wb = rowv[1];
+                                vv[0] = wa.x; vv[1] = wa.y; vv[2] = wa.z; vv[3] = wa.w;
+                                vv[4] = wb.x; vv[5] = wb.y; vv[6] = wb.z; vv[7] = wb.w;
+                            }
                             const float c1 = fmaf(bxy, dy, bxz * dz);
                             const float c0 = fmaf(dy, fmaf(cyy, dy, cyz * dz), czz * dz * dz);
                             float r0 = 0.f, r1 = 0.f, r2 = 0.f, re = 0.f, rl = 0.f;
@@ -406,12 +469,29 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                                 float gv;
                                 if constexpr (BETA) {
                                     const float h = 1.f - q, lg = __log2f(h);
-                                    gv = in ? v[x] * __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;
-                                    const float ve = gv * h;
+                                    const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(beta - 1)
+                                    float ve;
+                                    if constexpr (VAR) {
+                                        const float he = hb * h;  // h^beta = alpha / w
+                                        const float vj = fmaf(kvar * he, vv[x], v[x]);
+                                        gv = vj * hb;
+                                        ve = vj * he;
+                                        vq = fmaf(vv[x] * he, he, vq);
+                                    } else {
+                                        gv = v[x] * hb;
+                                        ve = gv * h;
+                                    }
                                     re += ve;
                                     rl = fmaf(ve, in ? lg : 0.f, rl);
                                 } else {
-                                    gv = in ? v[x] * __builtin_amdgcn_exp2f(-q) : 0.f;
+                                    const float e = in ? __builtin_amdgcn_exp2f(-q) : 0.f;
+                                    if constexpr (VAR) {
+                                        const float vj = fmaf(kvar * e, vv[x], v[x]);
+                                        gv = vj * e;
+                                        vq = fmaf(vv[x] * e, e, vq);
+                                    } else {
+                                        gv = v[x] * e;
+                                    }
                                 }
                                 r0 += gv;
                                 const float gx = gv * dx;
@@ -460,7 +540,7 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                             float4* out = reinterpret_cast<float4*>(partials + kVoxPartial * size_t(w));
                             out[0] = make_float4(gm[0], gm[1], gm[2], gc[0]);
                             out[1] = make_float4(gc[1], gc[2], gc[3], gc[4]);
-                            out[2] = make_float4(gc[5], go, gb, 0.f);
+                            out[2] = make_float4(gc[5], go, gb, VAR ? opac * opac * vq : 0.f);
                         }
                     }
                 } else {
@@ -493,11 +573,13 @@ constexpr int kVoxFwdWaves = 4;
 struct __align__(16) VoxRec {
     float mx, my, mz, a, bxy, bxz, cyy, cyz, czz, w, beta;
     uint32_t qlim;
+    float lx, ly, lz, pad;  // the mean's float32 remainder (vox_mean_lo)
 };
-static_assert(sizeof(VoxRec) == 48, "voxel record must be 48 bytes");
+static_assert(sizeof(VoxRec) == 64, "voxel record must be 64 bytes");
 
 template <bool BETA>
-__device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, int32_t i, int32_t hi) {
+__device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, const VoxFrame& frame, int32_t i,
+                                               int32_t hi) {
     VoxRec r = {};
     if (i >= hi) return r;
     const int32_t g = A.flatten[i];
@@ -508,6 +590,12 @@ __device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, int32_t 
     r.mx = A.means3d[3 * G];
     r.my = A.means3d[3 * G + 1];
     r.mz = A.means3d[3 * G + 2];
+    const float hi3[3] = {r.mx, r.my, r.mz};
+    float lo3[3];
+    vox_mean_lo(A, frame, G, hi3, lo3);
+    r.lx = lo3[0];
+    r.ly = lo3[1];
+    r.lz = lo3[2];
     r.a = s * c[0];
     r.bxy = t2 * c[1];
     r.bxz = t2 * c[2];
@@ -520,11 +608,21 @@ __device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, int32_t 
     return r;
 }
 
-template <bool BETA>
+// The uncertainty of the primitive a list entry names (0 past the list).
+__device__ __forceinline__ float load_vox_unc(const VoxArgs<float>& A, int32_t i, int32_t hi) {
+    if (i >= hi) return 0.f;
+    const int32_t g = A.flatten[i];
+    return (g < 0 || uint32_t(g) >= A.N) ? 0.f : A.unc[g];
+}
+
+// VAR also accumulates the variance sum q alpha^2 (q the primitive's
+// uncertainty, staged beside the record) and sums it in a second LDS phase.
+template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kVoxFwdWaves * kWave)
-vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
+vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out, float* __restrict__ out_var) {
     constexpr uint32_t T = kMomT;
     __shared__ VoxRec stage[kVoxFwdWaves][kWave];
+    __shared__ float stage_q[VAR ? kVoxFwdWaves : 1][kWave];
     __shared__ float part[kVoxFwdWaves][T * T * T];
     const uint32_t blk = blockIdx.x;
     const uint32_t nt = A.ntx * A.nty * A.ntz;
@@ -538,21 +636,29 @@ vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
     const uint32_t tx = blk % A.ntx, rest = blk / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
     const uint32_t ly = lane % T, lz = lane / T;
     const float py = float(ty * T + ly) + 0.5f, pz = float(tz * T + lz) + 0.5f, px0 = float(tx * T) + 0.5f;
-    float acc[T];
+    float acc[T], acc_var[T];
 #pragma unroll
-    for (uint32_t x = 0; x < T; ++x) acc[x] = 0.f;
+    for (uint32_t x = 0; x < T; ++x) {
+        acc[x] = 0.f;
+        acc_var[x] = 0.f;
+    }
     VoxRec* slot = stage[wave];
-    VoxRec next = load_vox_rec<BETA>(A, lo + int32_t(lane), hi);
+    const VoxFrame frame = vox_frame(A);
+    VoxRec next = load_vox_rec<BETA>(A, frame, lo + int32_t(lane), hi);
+    float next_q = VAR ? load_vox_unc(A, lo + int32_t(lane), hi) : 0.f;
     for (int32_t base = lo; base < hi; base += kWave) {
         wave_sync_lds();
         slot[lane] = next;
+        if constexpr (VAR) stage_q[wave][lane] = next_q;
         wave_sync_lds();
-        next = load_vox_rec<BETA>(A, base + kWave + int32_t(lane), hi);
+        next = load_vox_rec<BETA>(A, frame, base + kWave + int32_t(lane), hi);
+        if constexpr (VAR) next_q = load_vox_unc(A, base + kWave + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         for (int32_t j = 0; j < n; ++j) {
             const VoxRec r = slot[j];
             if (r.qlim == 0u) continue;  // wave-uniform (every lane reads the same record)
-            const float dy = py - r.my, dz = pz - r.mz, dx0 = px0 - r.mx;
+            const float unc = VAR ? stage_q[VAR ? wave : 0][j] : 0.f;
+            const float dy = (py - r.my) - r.ly, dz = (pz - r.mz) - r.lz, dx0 = (px0 - r.mx) - r.lx;
             const float c1 = fmaf(r.bxy, dy, r.bxz * dz);
             const float c0 = fmaf(dy, fmaf(r.cyy, dy, r.cyz * dz), r.czz * dz * dz);
 #pragma unroll
@@ -564,7 +670,14 @@ vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
                     e = __builtin_amdgcn_exp2f(r.beta * __log2f(1.f - q));
                 else
                     e = __builtin_amdgcn_exp2f(-q);
-                acc[x] = fmaf(__float_as_uint(q) < r.qlim ? e : 0.f, r.w, acc[x]);
+                const float ek = __float_as_uint(q) < r.qlim ? e : 0.f;
+                if constexpr (VAR) {
+                    const float alpha = ek * r.w;
+                    acc[x] += alpha;
+                    acc_var[x] = fmaf(unc * alpha, alpha, acc_var[x]);
+                } else {
+                    acc[x] = fmaf(ek, r.w, acc[x]);
+                }
             }
         }
     }
@@ -578,6 +691,19 @@ vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
         const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
         if (i < A.dz && j < A.dy && k < A.dx) out[(size_t(i) * A.dy + j) * A.dx + k] = v;
     }
+    if constexpr (VAR) {
+        __syncthreads();  // the mean's partial volumes are consumed
+#pragma unroll
+        for (uint32_t x = 0; x < T; ++x) part[wave][(lz * T + ly) * T + x] = acc_var[x];
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < T * T * T; p += blockDim.x) {
+            float v = 0.f;
+#pragma unroll
+            for (int k = 0; k < kVoxFwdWaves; ++k) v += part[k][p];
+            const uint32_t k = tx * T + p % T, j = ty * T + (p / T) % T, i = tz * T + p / (T * T);
+            if (i < A.dz && j < A.dy && k < A.dx) out_var[(size_t(i) * A.dy + j) * A.dx + k] = v;
+        }
+    }
 }
 
 // Per-primitive sums of the intersection partials [cum[n], cum[n+1]) in
@@ -585,17 +711,17 @@ vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out) {
 __global__ void __launch_bounds__(256)
 vox_partials_reduce_kernel(const uint32_t* __restrict__ cum, const float* __restrict__ partials, uint32_t N,
                            uint32_t n_isects, float* __restrict__ v_xyz, float* __restrict__ v_c,
-                           float* __restrict__ v_opac, float* __restrict__ v_beta) {
+                           float* __restrict__ v_opac, float* __restrict__ v_beta, float* __restrict__ v_unc) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    float acc[11] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float acc[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const uint32_t lo = min(cum[n], n_isects), hi = min(cum[n + 1], n_isects);
     for (uint32_t j = lo; j < hi; ++j) {
         const float4* p = reinterpret_cast<const float4*>(partials + kVoxPartial * size_t(j));
         const float4 a = p[0], b = p[1], c = p[2];
         acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
         acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
-        acc[8] += c.x; acc[9] += c.y; acc[10] += c.z;
+        acc[8] += c.x; acc[9] += c.y; acc[10] += c.z; acc[11] += c.w;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) v_xyz[3 * size_t(n) + k] = acc[k];
@@ -603,6 +729,7 @@ vox_partials_reduce_kernel(const uint32_t* __restrict__ cum, const float* __rest
     for (int k = 0; k < 6; ++k) v_c[6 * size_t(n) + k] = acc[3 + k];
     v_opac[n] = acc[9];
     if (v_beta) v_beta[n] = acc[10];
+    if (v_unc) v_unc[n] = acc[11];
 }
 
 inline int check_vox_tile(uint32_t T, uint32_t ntx, uint32_t nty, uint32_t ntz, uint32_t dx,
@@ -717,10 +844,10 @@ extern "C" int r2s_voxelize_to_voxels_bwd_per_gaussian(
         if constexpr (std::is_same<S, float>::value) {
             const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
             if (tile_size == kMomT && !v_render_var && n_tiles < (1ull << 31)) {  // moments form
-                auto km = beta ? vox_bwd_moments_kernel<true> : vox_bwd_moments_kernel<false>;
+                auto km = beta ? vox_bwd_moments_kernel<true, false> : vox_bwd_moments_kernel<false, false>;
                 km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(
-                    A, isect_ids, (const float*)v_render, (float*)v_means3d, (float*)v_conics, (float*)v_opacities,
-                    (float*)v_betas, nullptr, nullptr, nullptr);
+                    A, isect_ids, (const float*)v_render, nullptr, 0.f, (float*)v_means3d, (float*)v_conics,
+                    (float*)v_opacities, (float*)v_betas, nullptr, nullptr, nullptr);
                 return check_launch("voxelize_to_voxels_bwd_per_gaussian");
             }
         }
@@ -738,48 +865,69 @@ extern "C" size_t r2s_voxel_partials_bytes(uint32_t n_isects) {
     return size_t(n_isects) * kVoxPartial * sizeof(float);
 }
 
-extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* conics, const float* opacities,
-                                      const float* betas, const int32_t* radii, const uint32_t* tile_cum,
-                                      const int64_t* isect_ids, const int32_t* flatten_ids, uint32_t n_isects,
-                                      uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, uint32_t tile_size,
-                                      uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
-                                      const float* v_render, float* partials, size_t partials_bytes,
-                                      float* v_means3d, float* v_conics, float* v_opacities, float* v_betas,
-                                      r2s_stream_t stream) {
+extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* means, const float* ranges,
+                                      const float* centers, const float* conics, const float* opacities,
+                                      const float* betas, const float* uncertainties, const int32_t* radii,
+                                      const uint32_t* tile_cum, const int64_t* isect_ids, const int32_t* flatten_ids,
+                                      uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
+                                      uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
+                                      const float* v_render, const float* v_render_var, int detach, float* partials,
+                                      size_t partials_bytes, float* v_means3d, float* v_conics, float* v_opacities,
+                                      float* v_betas, float* v_uncertainties, r2s_stream_t stream) {
     if (int rc = check_vox_tile(tile_size, n_tiles_x, n_tiles_y, n_tiles_z, dim_x, dim_y, dim_z)) return rc;
     R2S_REQUIRE(tile_size == kMomT, "the fused voxel VJP takes %u^3 tiles (got %u)", kMomT, tile_size);
     R2S_REQUIRE(uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z < (1ull << 31), "too many tiles");
     R2S_REQUIRE(partials_bytes >= r2s_voxel_partials_bytes(n_isects), "partials buffer too small: %zu < %zu",
                 partials_bytes, r2s_voxel_partials_bytes(n_isects));
-    const bool beta = betas != nullptr;
+    const bool beta = betas != nullptr, var = v_render_var != nullptr;
     R2S_REQUIRE(!beta || v_betas != nullptr, "v_betas is required with betas");
+    R2S_REQUIRE(!var || (uncertainties != nullptr && v_uncertainties != nullptr),
+                "a variance adjoint needs uncertainties and v_uncertainties");
+    R2S_REQUIRE((means == nullptr) == (ranges == nullptr) && (means == nullptr) == (centers == nullptr),
+                "means, ranges and centers go together");
     if (N == 0) return R2S_OK;
     if (n_isects > 0) {
-        auto A = make_vox_args<float>(means3d, conics, opacities, betas, nullptr, nullptr, flatten_ids, n_isects, N,
-                                      dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
-        auto km = beta ? vox_bwd_moments_kernel<true> : vox_bwd_moments_kernel<false>;
-        km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(A, isect_ids, v_render, nullptr, nullptr,
-                                                                         nullptr, nullptr, radii, tile_cum, partials);
+        auto A = make_vox_args<float>(means3d, conics, opacities, betas, uncertainties, nullptr, flatten_ids,
+                                      n_isects, N, dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
+        A.world = means;
+        A.ranges = ranges;
+        A.centers = centers;
+        auto km = beta ? (var ? vox_bwd_moments_kernel<true, true> : vox_bwd_moments_kernel<true, false>)
+                       : (var ? vox_bwd_moments_kernel<false, true> : vox_bwd_moments_kernel<false, false>);
+        km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(
+            A, isect_ids, v_render, v_render_var, detach ? 0.f : 2.f, nullptr, nullptr, nullptr, nullptr, radii,
+            tile_cum, partials);
         if (int rc = check_launch("voxelize_fused_vjp")) return rc;
     }
     vox_partials_reduce_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(tile_cum, partials, N, n_isects, v_means3d,
-                                                                     v_conics, v_opacities, beta ? v_betas : nullptr);
+                                                                     v_conics, v_opacities, beta ? v_betas : nullptr,
+                                                                     var ? v_uncertainties : nullptr);
     return check_launch("voxelize_fused_vjp_reduce");
 }
 
-extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const float* conics, const float* opacities,
-                                      const float* betas, const int32_t* tile_offsets, const int32_t* flatten_ids,
-                                      uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
-                                      uint32_t tile_size, uint32_t n_tiles_x, uint32_t n_tiles_y, uint32_t n_tiles_z,
-                                      float* render, r2s_stream_t stream) {
+extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float* ranges,
+                                      const float* centers, const float* conics, const float* opacities,
+                                      const float* betas, const float* uncertainties, const int32_t* tile_offsets,
+                                      const int32_t* flatten_ids, uint32_t n_isects, uint32_t N, uint32_t dim_x,
+                                      uint32_t dim_y, uint32_t dim_z, uint32_t tile_size, uint32_t n_tiles_x,
+                                      uint32_t n_tiles_y, uint32_t n_tiles_z, float* render, float* render_var,
+                                      r2s_stream_t stream) {
     if (int rc = check_vox_tile(tile_size, n_tiles_x, n_tiles_y, n_tiles_z, dim_x, dim_y, dim_z)) return rc;
     R2S_REQUIRE(tile_size == kMomT, "the fused voxel forward takes %u^3 tiles (got %u)", kMomT, tile_size);
+    const bool var = render_var != nullptr;
+    R2S_REQUIRE(!var || uncertainties != nullptr, "render_var requires uncertainties");
     const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles");
     if (n_tiles == 0) return R2S_OK;
-    auto A = make_vox_args<float>(means3d, conics, opacities, betas, nullptr, tile_offsets, flatten_ids, n_isects, N,
-                                  dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
-    auto k = betas ? vox_fwd_fused_kernel<true> : vox_fwd_fused_kernel<false>;
-    k<<<uint32_t(n_tiles), kVoxFwdWaves * kWave, 0, stream>>>(A, render);
+    R2S_REQUIRE((means == nullptr) == (ranges == nullptr) && (means == nullptr) == (centers == nullptr),
+                "means, ranges and centers go together");
+    auto A = make_vox_args<float>(means3d, conics, opacities, betas, uncertainties, tile_offsets, flatten_ids,
+                                  n_isects, N, dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
+    A.world = means;
+    A.ranges = ranges;
+    A.centers = centers;
+    auto k = betas ? (var ? vox_fwd_fused_kernel<true, true> : vox_fwd_fused_kernel<true, false>)
+                   : (var ? vox_fwd_fused_kernel<false, true> : vox_fwd_fused_kernel<false, false>);
+    k<<<uint32_t(n_tiles), kVoxFwdWaves * kWave, 0, stream>>>(A, render, render_var);
     return check_launch("voxelize_fused_fwd");
 }

@@ -44,8 +44,8 @@ def _use_fused(means) -> bool:
 
 
 def _use_fused_voxels(means) -> bool:
-    """float32 voxel forwards (mean only) and mean-adjoint VJPs (8^3 tiles) run the
-    fused kernels (fused.voxel_render / fused.voxel_vjp)."""
+    """float32 voxel forwards and VJPs (mean or (mean, variance), 8^3 tiles) run
+    the fused kernels (fused.voxel_render / fused.voxel_vjp)."""
     return BACKEND == "fused" and means.dtype == torch.float32 and VOXEL_TILE == 8
 
 
@@ -357,7 +357,7 @@ def _voxel_state(prims, extent, center, shape, eps, radius_clip) -> dict:
         _, isect_ids, flatten_ids, tile_cum = _ops_impl._isect_tiles_3d(mapped, radii, T, tx, ty, tz)
         offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
     return dict(covars=covars, mapped=mapped, conics=conics, radii=radii, tile_cum=tile_cum, offsets=offsets,
-                isect_ids=isect_ids, flatten_ids=flatten_ids)
+                isect_ids=isect_ids, flatten_ids=flatten_ids, means=means)
 
 
 def _voxel_render(b, prims, shape):
@@ -367,8 +367,8 @@ def _voxel_render(b, prims, shape):
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
     with stage("voxelize_fwd"):
-        if uncertainty is None and _use_fused_voxels(means):
-            return fused.voxel_render(b, densities, beta, (width, height, depth), T)
+        if _use_fused_voxels(means):
+            return fused.voxel_render(b, densities, beta, (width, height, depth), T, uncertainty=uncertainty)
         if uncertainty is None:
             return getattr(ops, prefix + "voxelize_to_voxels_fwd")(*args, b["offsets"], b["flatten_ids"], width,
                                                                    height, depth, T)
@@ -445,14 +445,18 @@ def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Opti
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], densities) + (() if beta is None else (beta,))
     with stage("voxelize_bwd"):
-        if v_variance is not None:
+        if _use_fused_voxels(means):
+            grads = fused.voxel_vjp(b, densities, beta, v_volume, (width, height, depth), T, uncertainty=uncertainty,
+                                    v_variance=v_variance, detach_variance=context.detach_variance)
+            if v_variance is not None:
+                v_uncertainty = grads[-1]
+            else:
+                v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
+        elif v_variance is not None:
             grads = getattr(ops, prefix + "voxelize_to_voxels_mean_var_bwd_per_gaussian")(
                 *args, b["offsets"], b["isect_ids"], b["flatten_ids"], v_volume, v_variance, uncertainty,
                 context.detach_variance, width, height, depth, T)
             v_uncertainty = grads[-1]
-        elif _use_fused_voxels(means):
-            grads = fused.voxel_vjp(b, densities, beta, v_volume, (width, height, depth), T)
-            v_uncertainty = None if uncertainty is None else torch.zeros_like(uncertainty)
         else:
             if context.backward_mode == "primitive":
                 grads = getattr(ops, prefix + "voxelize_to_voxels_bwd_per_gaussian")(

@@ -215,11 +215,23 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     return out
 
 
-def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8):
+def _frame(state: dict):
+    """World means and volume frame of a binning state (None when it has none):
+    the kernels recompute the voxel-frame means from them in double precision."""
+    if "means" in state and "extent" in state and "center" in state:
+        return state["means"], state["extent"], state["center"]
+    return None, None, None
+
+
+def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8, *, uncertainty=None,
+              v_variance=None, detach_variance: bool = False):
     """Per-primitive voxelize VJP (float32, 8^3 tiles) from a voxel binning
-    state (mapped, conics, radii, tile_cum, isect_ids, flatten_ids): one partial
+    state (mapped, conics, radii, tile_cum, isect_ids, flatten_ids, and the
+    world means with the volume's extent and center): one partial
     per intersection, summed per primitive in a fixed order (no atomics).
-    -> (v_mapped [N,3], v_conics [N,6], v_densities [N][, v_beta [N]])."""
+    With a variance adjoint (``v_variance``, and the forward's ``uncertainty``)
+    it also returns v_uncertainty.
+    -> (v_mapped [N,3], v_conics [N,6], v_densities [N][, v_beta [N]][, v_uncertainty [N]])."""
     mapped = state["mapped"]
     device = mapped.device
     N = mapped.shape[0]
@@ -232,25 +244,37 @@ def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8):
     v_conics = torch.empty((N, 6), dtype=torch.float32, device=device)
     v_dens = torch.empty((N,), dtype=torch.float32, device=device)
     v_beta = torch.empty((N,), dtype=torch.float32, device=device) if beta is not None else None
+    v_unc = torch.empty((N,), dtype=torch.float32, device=device) if v_variance is not None else None
     with torch.cuda.device(device):
-        call("r2s_voxelize_fused_vjp", ptr(mapped), ptr(state["conics"]), ptr(densities), ptr(beta),
-             ptr(state["radii"]), ptr(state["tile_cum"]), ptr(ids), ptr(flat), n_isects, N, W, H, D, T,
-             -(-W // T), -(-H // T), -(-D // T), ptr(v_volume), ptr(partials), partials.numel(), ptr(v_mapped),
-             ptr(v_conics), ptr(v_dens), ptr(v_beta), be.stream_of(device))
+        world, extent, center = _frame(state)
+        call("r2s_voxelize_fused_vjp", ptr(mapped), ptr(world), ptr(extent), ptr(center), ptr(state["conics"]),
+             ptr(densities), ptr(beta),
+             ptr(uncertainty if v_variance is not None else None), ptr(state["radii"]), ptr(state["tile_cum"]),
+             ptr(ids), ptr(flat), n_isects, N, W, H, D, T, -(-W // T), -(-H // T), -(-D // T), ptr(v_volume),
+             ptr(v_variance), int(bool(detach_variance)), ptr(partials), partials.numel(), ptr(v_mapped),
+             ptr(v_conics), ptr(v_dens), ptr(v_beta), ptr(v_unc), be.stream_of(device))
+    out = (v_mapped, v_conics, v_dens)
     if beta is not None:
-        return v_mapped, v_conics, v_dens, v_beta
-    return v_mapped, v_conics, v_dens
+        out = out + (v_beta,)
+    if v_unc is not None:
+        out = out + (v_unc,)
+    return out
 
 
-def voxel_render(state: dict, densities, beta, dims, tile_size: int = 8):
-    """Volume [D, H, W] of a voxel binning state (float32, 8^3 tiles)."""
+def voxel_render(state: dict, densities, beta, dims, tile_size: int = 8, *, uncertainty=None):
+    """Volume [D, H, W] of a voxel binning state (float32, 8^3 tiles), or
+    (volume, variance) with ``uncertainty`` [N]."""
     mapped = state["mapped"]
     device = mapped.device
     W, H, D = dims
     T = int(tile_size)
     out = torch.empty((D, H, W), dtype=torch.float32, device=device)
+    var = torch.empty_like(out) if uncertainty is not None else None
     with torch.cuda.device(device):
-        call("r2s_voxelize_fused_fwd", ptr(mapped), ptr(state["conics"]), ptr(densities), ptr(beta),
-             ptr(state["offsets"]), ptr(state["flatten_ids"]), state["flatten_ids"].numel(), mapped.shape[0], W, H, D,
-             T, -(-W // T), -(-H // T), -(-D // T), ptr(out), be.stream_of(device))
-    return out
+        world, extent, center = _frame(state)
+        call("r2s_voxelize_fused_fwd", ptr(mapped), ptr(world), ptr(extent), ptr(center), ptr(state["conics"]),
+             ptr(densities), ptr(beta),
+             ptr(uncertainty), ptr(state["offsets"]), ptr(state["flatten_ids"]), state["flatten_ids"].numel(),
+             mapped.shape[0], W, H, D, T, -(-W // T), -(-H // T), -(-D // T), ptr(out), ptr(var),
+             be.stream_of(device))
+    return out if var is None else (out, var)

@@ -348,37 +348,45 @@ def test_fused_beta_large_exponent_centre_pixel(device):
 
 
 # --------------------------------------------------------------------------- voxels
-@pytest.mark.parametrize("use_beta", [False, True])
-def test_fused_voxel_vjp_matches_staged_path_and_is_reproducible(device, use_beta):
+@pytest.mark.parametrize("use_beta,uncertainty,detach", [(False, False, False), (True, False, False),
+                                                         (False, True, False), (True, True, False),
+                                                         (True, True, True)])
+def test_fused_voxel_vjp_matches_staged_path_and_is_reproducible(device, use_beta, uncertainty, detach):
     """float32 fused voxelize forward and backward (no atomics: one partial per
-    intersection, summed per primitive in a fixed order) against the staged
-    operators, on a volume whose edges cut tiles; bitwise reproducible."""
+    intersection, summed per primitive in a fixed order), mean or (mean,
+    variance) with detach_variance either way, against the staged operators,
+    on a volume whose edges cut tiles; bitwise reproducible."""
     sc = workloads.scene(20000, 1, 8, 8, scale_range=(0.005, 0.05), seed=2, device=device,
-                         beta_range=(0.5, 4.0) if use_beta else None)
+                         beta_range=(0.5, 4.0) if use_beta else None, uncertainty_max=0.1 if uncertainty else None)
     shape = (90, 77, 64)
-    kw = dict(volume_extent=(1.0, 1.1, 0.9), beta=sc.beta)
-    v = torch.randn(shape, generator=torch.Generator(device=device).manual_seed(3), device=device)
+    kw = dict(volume_extent=(1.0, 1.1, 0.9), beta=sc.beta, uncertainty=sc.uncertainty, detach_variance=detach)
+    g = torch.Generator(device=device).manual_seed(3)
+    v = torch.randn(shape, generator=g, device=device)
+    vv = torch.randn(shape, generator=g, device=device) if uncertainty else None
 
     def run(backend):
         old = explicit.BACKEND
         explicit.BACKEND = backend
         try:
-            vol, ctx = r2splat_amd.voxelize_fwd(*sc.params(), shape, **kw)
-            return vol, r2splat_amd.voxelize_bwd(ctx, v)
+            out, ctx = r2splat_amd.voxelize_fwd(*sc.params(), shape, **kw)
+            return out, r2splat_amd.voxelize_bwd(ctx, v, vv)
         finally:
             explicit.BACKEND = old
 
-    vol_a, a = run("fused")
-    vol_b, b = run("ops")
-    assert O.relative_error(vol_a, vol_b) <= 1e-5
-    names = ("means", "quats", "scales", "densities") + (("beta",) if use_beta else ())
+    out_a, a = run("fused")
+    out_b, b = run("ops")
+    for x, y in zip(out_a if uncertainty else (out_a,), out_b if uncertainty else (out_b,)):
+        assert O.relative_error(x, y) <= 1e-5
+    names = ("means", "quats", "scales", "densities") + (("beta",) if use_beta else ()) + \
+        (("uncertainty",) if uncertainty else ())
     for name in names:
         x, y = getattr(a, name).double(), getattr(b, name).double()
         l2 = float((x - y).norm() / y.norm())
         assert l2 <= 1e-5, f"{name}: relative L2 {l2:.2e}"
         assert O.relative_error(x, y) <= 1e-4, f"{name}: {O.relative_error(x, y):.2e}"
-    vol_a2, a2 = run("fused")
-    assert torch.equal(vol_a, vol_a2)
+    out_a2, a2 = run("fused")
+    for x, y in zip(out_a if uncertainty else (out_a,), out_a2 if uncertainty else (out_a2,)):
+        assert torch.equal(x, y)
     for name in names:
         assert torch.equal(getattr(a, name), getattr(a2, name)), name
 

@@ -303,6 +303,7 @@ def test_rasterize_ops_forward_and_all_vjps(device, dtype, beta, tile):
     v = torch.linspace(-0.4, 1.3, ref.numel(), dtype=torch.float64).reshape(ref.shape)
     vv = torch.linspace(0.9, -0.2, ref.numel(), dtype=torch.float64).reshape(ref.shape)
     exp_mean = torch.autograd.grad((ref * v).sum(), leaves, retain_graph=True)
+    exp_abs = O.abs_mean2d_grad(m2, conics, opac, mus, W, H, v, beta2d=b2, mask=mask)
     gt = tol(dtype, 1e-4, 1e-9)
     for name, fn, extra in (("pixel", prefix + "rasterize_to_pixels_bwd", (off, flat)),
                             ("primitive", prefix + "rasterize_to_pixels_bwd_per_gaussian", (off, ids, flat))):
@@ -310,12 +311,18 @@ def test_rasterize_ops_forward_and_all_vjps(device, dtype, beta, tile):
         assert grads[0] is not None and len(grads) == 5 + int(beta)
         for k, (a, e) in enumerate(zip(grads[1:], exp_mean)):
             assert_rel(a, e, gt, f"{name} grad {k}")
+        # absgrad: sum over pixels of |d(v alpha)/d mean2d| (RasterizeToPixelsBwd.cu:219-221, 250-255)
+        assert_rel(grads[0], exp_abs, gt, f"{name} absgrad")
     exp_joint = torch.autograd.grad((ref * v + ref_var * vv).sum(), leaves + [ur], retain_graph=True)
     fused = getattr(ops, prefix + "rasterize_to_pixels_mean_var_bwd_per_gaussian")(
         *args, off, ids, flat, d(v), d(vv), d(unc), False, W, H, tile, False)
     assert fused[0] is None and len(fused) == 6 + int(beta)
     for k, (a, e) in enumerate(zip(fused[1:], exp_joint)):
         assert_rel(a, e, gt, f"fused grad {k}")
+    with_abs = getattr(ops, prefix + "rasterize_to_pixels_mean_var_bwd_per_gaussian")(
+        *args, off, ids, flat, d(v), d(vv), d(unc), False, W, H, tile, True)
+    exp_abs_var = O.abs_mean2d_grad(m2, conics, opac, mus, W, H, v, beta2d=b2, unc=unc, v_var=vv, mask=mask)
+    assert_rel(with_abs[0], exp_abs_var, gt, "mean_var absgrad")
     vq = getattr(ops, prefix + "rasterize_to_pixels_uncertainty_bwd")(*args, off, flat, d(vv), W, H, tile)
     assert_rel(vq, exp_joint[-1], gt, "uncertainty bwd")
     # detached overload: mean adjoint only from v, uncertainty gradient unchanged

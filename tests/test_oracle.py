@@ -544,3 +544,59 @@ def test_voxelize_cutoffs_1e6_gaussian_1e5_beta():
     b_edge = 1 - 1e-5 ** (1 / beta)  # (1 - d2)^2 = 1e-5
     assert one(b_edge * (1 - 1e-9), beta) > 0 and one(b_edge * (1 + 1e-9), beta) == 0.0
     assert one(1.0, 0.5) == 0.0 and one(0.999999, 0.5) > 0  # support ends at d2 = 1
+
+
+# --------------------------------------------------------------------------- absgrad
+def _abs_scene(beta):
+    g = torch.Generator().manual_seed(21)
+    C, N, W, H = 2, 5, 9, 7
+    m2 = torch.rand(C, N, 2, generator=g) * torch.tensor([W, H])
+    A = torch.randn(C, N, 2, 2, generator=g)
+    P = A @ A.transpose(-1, -2) * 0.1 + 0.05 * torch.eye(2)
+    conics = P[..., [0, 0, 1], [0, 1, 1]]
+    opac, mus = 0.3 + torch.rand(C, N, generator=g), 0.5 + torch.rand(C, N, generator=g)
+    b2 = 1 + 2 * torch.rand(C, N, generator=g) if beta else None
+    return m2, conics, opac, mus, b2, W, H
+
+
+@pytest.mark.parametrize("beta", [False, True])
+def test_abs_mean2d_grad_signed_parts_sum_to_the_gradient(beta):
+    """The per-pixel partials behind absgrad, summed with their signs, are the
+    gradient of sum(v * image) with respect to means2d (autograd of rasterize)."""
+    m2, conics, opac, mus, b2, W, H = _abs_scene(beta)
+    v = torch.linspace(-1.0, 1.5, 2 * H * W).reshape(2, H, W)
+    leaf = m2.clone().requires_grad_()
+    img = O.rasterize(leaf, conics, opac, mus, W, H, beta2d=b2)
+    (grad,) = torch.autograd.grad((img * v).sum(), leaf)
+    ab = O.abs_mean2d_grad(m2, conics, opac, mus, W, H, v, beta2d=b2)
+    assert bool((ab >= grad.abs() - 1e-12).all())  # |sum| <= sum of |parts|
+    assert float((ab - grad.abs()).max()) > 1e-3    # and strictly larger somewhere: signs differ across pixels
+    # a positive adjoint and every kept pixel right of / below the mean: all parts share a sign
+    one = torch.tensor([[[-0.5, -0.5]]])
+    cone = torch.tensor([[[0.4, 0.0, 0.6]]])
+    vp = torch.ones(1, H, W)
+    leaf = one.clone().requires_grad_()
+    bb = None if b2 is None else torch.tensor([[1.5]])
+    img = O.rasterize(leaf, cone, torch.tensor([[1.0]]), torch.tensor([[1.0]]), W, H, beta2d=bb)
+    if beta:  # the Beta support of conic 0.4 / 0.6 reaches ~1.6 px: pick a wider one
+        cone = torch.tensor([[[0.05, 0.0, 0.08]]])
+        img = O.rasterize(leaf, cone, torch.tensor([[1.0]]), torch.tensor([[1.0]]), W, H, beta2d=bb)
+    (grad,) = torch.autograd.grad((img * vp).sum(), leaf)
+    ab = O.abs_mean2d_grad(one, cone, torch.tensor([[1.0]]), torch.tensor([[1.0]]), W, H, vp, beta2d=bb)
+    assert float(grad.abs().min()) > 0
+    torch.testing.assert_close(ab, grad.abs(), atol=1e-14, rtol=1e-12)
+
+
+def test_abs_mean2d_grad_variance_adjoint_enters_unless_detached():
+    m2, conics, opac, mus, _, W, H = _abs_scene(False)
+    v = torch.zeros(2, H, W)
+    vv = torch.ones(2, H, W)
+    unc = torch.full((5,), 0.5)
+    with_var = O.abs_mean2d_grad(m2, conics, opac, mus, W, H, v, unc=unc, v_var=vv)
+    detached = O.abs_mean2d_grad(m2, conics, opac, mus, W, H, v, unc=unc, v_var=vv, detach=True)
+    assert float(detached.abs().max()) == 0.0 and float(with_var.min()) >= 0 and float(with_var.max()) > 0
+    # with v = 0 the pixel adjoint is 2 q alpha v_var: the parts are those of the variance sum q alpha^2
+    leaf = m2.clone().requires_grad_()
+    _, var = O.rasterize(leaf, conics, opac, mus, W, H, unc=unc)
+    (g,) = torch.autograd.grad(var.sum(), leaf)
+    assert bool((with_var >= g.abs() - 1e-12).all())

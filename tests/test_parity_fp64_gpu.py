@@ -1,0 +1,146 @@
+"""Float32 fused path against float64 at working size, with cut-off flips
+separated from arithmetic error.
+
+For every (list entry, pixel) pair of the fused path's tile lists the float64
+side is evaluated twice (tests/_fp64_pairs.py): over the pairs the float32
+kernels kept (their keep masks), and with its own float64 decisions. The
+first comparison holds the float32 arithmetic to the contract's fixed bounds
+(max |a - b| / max |b|: 1e-5 for images, 1e-4 for every gradient); the
+second is the float64 reference itself (it reproduces the float64 operator
+path, which the oracle pins at config 0), and the gap between the two is the
+contribution of pairs at the 1e-5 threshold, reported with their count.
+
+Config 1: 100,000 Gaussians x 64 views of 256^2, MSE adjoint against a second
+primitive set (its target images), through the explicit API.
+"""
+
+import pytest
+import torch
+
+import r2splat_amd
+from oracle import splat as O
+from r2splat_amd import workloads
+
+from _fp64_pairs import evaluate, fp64_scene, pair_table
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("means", "quats", "scales", "densities")
+
+
+def _counts(C, N, table):
+    return torch.bincount(table.cam * N + table.prim, minlength=C * N)
+
+
+@pytest.fixture(scope="module")
+def config1(device):
+    sc = workloads.config1(device=device)
+    tg = workloads.config1(device=device, seed=1)
+    C, N = sc.viewmats.shape[0], sc.means.shape[0]
+    target, _ = r2splat_amd.rasterize_fwd(*tg.params(), tg.viewmats, tg.Ks, tg.image_size)
+    image, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+    assert "fused" in ctx.buffers
+    table = pair_table(ctx.buffers["fused"], C, N)
+    v = (2.0 * (image - target) / image.numel()).contiguous()
+    grads = r2splat_amd.rasterize_bwd(ctx, v)
+    sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+    # the float64 path's own tile lists hold the same (camera, primitive) pairs as the float32 lists
+    counts64 = torch.bincount(sc64.buffers["flatten_ids"].long(), minlength=C * N)
+    coverage_diff = int((counts64 != _counts(C, N, table)).sum())
+    g64 = r2splat_amd.rasterize_bwd(sc64.ctx, v.double())
+    kept = evaluate(table, sc64, "kept", v_image=v)
+    ref = evaluate(table, sc64, "ref", v_image=v)
+    return dict(C=C, N=N, image=image, grads=grads, sc64=sc64, g64=g64, kept=kept, ref=ref, table=table,
+                coverage_diff=coverage_diff)
+
+
+def test_config1_fp64_pair_evaluation_reproduces_the_float64_operator_path(config1):
+    """The float64 pair evaluation with float64 decisions equals the float64
+    operator path (rasterize_fwd / rasterize_bwd on float64 inputs)."""
+    ref, sc64, g64 = config1["ref"], config1["sc64"], config1["g64"]
+    print(f"config1 (camera, primitive) records whose float32 and float64 tile counts differ: "
+          f"{config1['coverage_diff']}", flush=True)
+    assert config1["coverage_diff"] == 0
+    err = O.relative_error(ref.image, sc64.image)
+    print(f"config1 fp64 pair evaluation vs fp64 operators: image {err:.2e}", flush=True)
+    assert err <= 1e-12
+    for name in FIELDS:
+        gerr = O.relative_error(ref.grads[name], getattr(g64, name))
+        print(f"  grad {name}: {gerr:.2e}", flush=True)
+        assert gerr <= 1e-9, name
+
+
+def test_config1_float32_image_within_1e5_of_float64_on_its_kept_pairs(config1):
+    kept, ref, image = config1["kept"], config1["ref"], config1["image"]
+    arith = O.relative_error(image, kept.image)
+    flips = O.relative_error(kept.image, ref.image)
+    total = O.relative_error(image, ref.image)
+    print(f"config1 image: float32 vs float64 on kept pairs {arith:.2e}; cut-off flips {kept.flips_fwd} of "
+          f"{kept.pairs} pairs, their contribution {flips:.2e}; float32 vs float64 reference {total:.2e}",
+          flush=True)
+    assert arith <= 1e-5
+
+
+def test_config1_float32_gradients_within_1e4_of_float64_on_their_kept_pairs(config1):
+    kept, ref, grads = config1["kept"], config1["ref"], config1["grads"]
+    print(f"config1 gradients: cut-off flips in the backward {kept.flips_bwd} of {kept.pairs} pairs", flush=True)
+    for name in FIELDS:
+        arith = O.relative_error(getattr(grads, name), kept.grads[name])
+        flips = O.relative_error(kept.grads[name], ref.grads[name])
+        total = O.relative_error(getattr(grads, name), ref.grads[name])
+        print(f"  {name}: float32 vs float64 on kept pairs {arith:.2e}; flips {flips:.2e}; total {total:.2e}",
+              flush=True)
+        assert arith <= 1e-4, name
+
+
+# --------------------------------------------------------------------------- config 2
+# Beta primitives with uncertainty: all 500,000 primitives of config 2 seen by
+# 32 of its 256 views (every eighth) at 512^2, the views' share of its 274M
+# intersections; adjoints N(0, 1e-3) for the mean and N(0, 1e-4) for the variance.
+BETA_FIELDS = FIELDS + ("beta", "uncertainty")
+
+
+@pytest.fixture(scope="module")
+def config2_views(device):
+    sc = workloads.config2(device=device)
+    sel = torch.arange(0, 256, 8, device=device)
+    sc.viewmats, sc.Ks = sc.viewmats[sel].contiguous(), sc.Ks[sel].contiguous()
+    g = torch.Generator(device=device).manual_seed(5)
+    shape = (sel.numel(),) + sc.image_size
+    v = torch.randn(shape, generator=g, device=device) * 1e-3
+    vv = torch.randn(shape, generator=g, device=device) * 1e-4
+    return sc, v, vv
+
+
+@pytest.mark.parametrize("detach", [False, True])
+def test_config2_beta_mean_variance_vs_float64_on_kept_pairs(config2_views, detach):
+    sc, v, vv = config2_views
+    C, N = sc.viewmats.shape[0], sc.means.shape[0]
+    kw = dict(beta=sc.beta, uncertainty=sc.uncertainty, detach_variance=detach)
+    (mean, var), ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, **kw)
+    assert "fused" in ctx.buffers
+    table = pair_table(ctx.buffers["fused"], C, N)
+    grads = r2splat_amd.rasterize_bwd(ctx, v, vv)
+    sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta, uncertainty=sc.uncertainty,
+                      detach=detach)
+    counts64 = torch.bincount(sc64.buffers["flatten_ids"].long(), minlength=C * N)
+    assert int((counts64 != _counts(C, N, table)).sum()) == 0
+    g64 = r2splat_amd.rasterize_bwd(sc64.ctx, v.double(), vv.double())
+    kept = evaluate(table, sc64, "kept", v_image=v, v_variance=vv, detach=detach)
+    ref = evaluate(table, sc64, "ref", v_image=v, v_variance=vv, detach=detach)
+    mean64, var64 = sc64.image
+    # the pair evaluation with float64 decisions is the float64 operator path
+    assert O.relative_error(ref.image, mean64) <= 1e-12 and O.relative_error(ref.variance, var64) <= 1e-12
+    for name in BETA_FIELDS:
+        assert O.relative_error(ref.grads[name], getattr(g64, name)) <= 1e-9, name
+    arith = (O.relative_error(mean, kept.image), O.relative_error(var, kept.variance))
+    total = (O.relative_error(mean, mean64), O.relative_error(var, var64))
+    print(f"config2 (32 views, detach={detach}): mean / variance vs float64 on kept pairs {arith[0]:.2e} / "
+          f"{arith[1]:.2e}; forward flips {kept.flips_fwd} of {kept.pairs} pairs; vs the float64 reference "
+          f"{total[0]:.2e} / {total[1]:.2e}", flush=True)
+    assert arith[0] <= 1e-5 and arith[1] <= 1e-5
+    print(f"  backward flips {kept.flips_bwd}", flush=True)
+    for name in BETA_FIELDS:
+        a = O.relative_error(getattr(grads, name), kept.grads[name])
+        t = O.relative_error(getattr(grads, name), ref.grads[name])
+        print(f"  {name}: float32 vs float64 on kept pairs {a:.2e}; vs the float64 reference {t:.2e}", flush=True)
+        assert a <= 1e-4, name
