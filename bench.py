@@ -72,6 +72,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-voxel", action="store_true", help="skip the separate 128^3 voxelization timing")
     p.add_argument("--no-graph", action="store_true", help="time the eager step instead of its hipGraph replay")
+    p.add_argument("--no-parity", action="store_true", help="skip the float64 error report of the timed workload")
     p.add_argument("--serve-window", type=float, default=10.0,
                    help="config1: seconds of Poisson fit requests (4 req/s) served for the p50/p99 fit latency "
                         "(0 skips it)")
@@ -176,6 +177,7 @@ def main():
                                    "(launch time x 2.4 GHz x 1,024 SIMDs)); a 16-lane SIMD completes one "
                                    "full-rate wave64 instruction per 4 cycles"}}
 
+    parity = None if args.no_parity else parity_vs_fp64(sc, target)
     voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
     latency = fit_latency(args.rate, args.serve_window, args.serve_steps, args.streams) if args.serve_window > 0 \
         else None
@@ -212,6 +214,7 @@ def main():
             "stages_ms": {k: round(v[0], 4) for k, v in plain[1].items()}},
         "voxelize_128": voxel,
         "fit_latency": latency,
+        "parity_fp64": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -486,6 +489,48 @@ def serving_main(args):
         "plain_path": plain,
     }
     print(json.dumps(result), flush=True)
+
+
+def parity_vs_fp64(sc, target):
+    """Max relative error (max |a - b| / max |b|) of the benchmarked float32 step
+    (image and every gradient, MSE adjoint against the target) against float64
+    on the same inputs: over the pairs the float32 kernels kept (the float32
+    arithmetic) and against the float64 operator path itself (which also
+    counts pairs at the 1e-5 cut-off that the two precisions decide
+    differently, and footprints whose radius rounds across an integer), with
+    the number of each. Evaluated once, outside the timed region."""
+    import torch
+
+    import r2splat_amd
+    from r2splat_amd import parity
+
+    C, N = sc.viewmats.shape[0], sc.means.shape[0]
+    with torch.no_grad():
+        img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+        table = parity.pair_table(ctx.buffers["fused"], C, N)
+        v = (2.0 * (img - target) / img.numel()).contiguous()
+        grads = r2splat_amd.rasterize_bwd(ctx, v)
+        sc64 = parity.fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
+        kept = parity.evaluate(table, sc64, "kept", v_image=v)
+        coverage = parity.coverage_diff(table, parity.fp64_table(sc64), C, N)
+        img64 = sc64.image
+        g64 = r2splat_amd.rasterize_bwd(sc64.ctx, v.double())
+
+    def rel(a, b):
+        a, b = a.double(), b.double()
+        return float(f"{float((a - b).abs().max() / b.abs().max()):.3e}")
+
+    out = {"image": {"kept_pairs": rel(img, kept.image), "float64_reference": rel(img, img64)}}
+    for name in ("means", "quats", "scales", "densities"):
+        out[name] = {"kept_pairs": rel(getattr(grads, name), kept.grads[name]),
+                     "float64_reference": rel(getattr(grads, name), getattr(g64, name))}
+    out["pairs"] = kept.pairs
+    out["cut_off_flips"] = {"forward": kept.flips_fwd, "backward": kept.flips_bwd}
+    out["coverage_differences"] = coverage
+    out["bounds"] = "images 1e-5, gradients 1e-4 (max |a - b| / max |b|)"
+    del table, sc64, kept, g64
+    torch.cuda.empty_cache()
+    return out
 
 
 def fit_latency(rate, duration, steps, streams):

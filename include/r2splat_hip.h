@@ -37,9 +37,10 @@ typedef struct ihipStream_t* r2s_stream_t; /* identical to hipStream_t */
 #define R2S_F32 0
 #define R2S_F64 1
 
-/* Library identification and error reporting. Version 3: the fused voxel
- * entry points take uncertainties and the variance buffers; the TV and L1
- * loss backward take lambda as a float (version 2: the fused raster entry
+/* Library identification and error reporting. Version 4: the fused voxel
+ * entry points take the double-precision frame of r2s_voxel_frame (version 3:
+ * they take uncertainties and the variance buffers; the TV and L1 loss
+ * backward take lambda as a float; version 2: the fused raster entry
  * points take the record kind and the variance buffers, and `packed` holds
  * tile-list entries followed by the projected records). */
 int r2s_abi_version(void);
@@ -335,7 +336,7 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * = intersections, counters[1] = 1 when capacity was exceeded (that
  * forward's outputs are then incomplete and the call must be repeated with
  * a larger capacity). isect_grads: float [capacity * 8], one VJP partial per
- * intersection (6 floats for Gaussians without uncertainty, 8 otherwise),
+ * intersection (8 floats: v_mx, v_my, v_a, v_b, v_c, v_w, v_b2d, v_q),
  * summed per primitive in a fixed order (the backward is deterministic and
  * uses no atomics).
  *
@@ -425,21 +426,36 @@ int r2s_fused_param_vjp(const float* means, const float* quats, const float* sca
                         float* v_means, float* v_quats, float* v_scales, float* v_densities,
                         float* v_betas, float* v_uncertainties, r2s_stream_t stream);
 
+/* Double-precision voxel frame of the fused voxel kernels: frame [N,9]
+ * float64 = per primitive the voxel-frame mean (x, y, z) and the inverse of
+ * the voxel-frame covariance (xx, xy, xz, yy, yz, zz), computed in double
+ * from float32 means [N,3], quats [N,4], scales [N,3] and float64 ranges [3]
+ * and centers [3] (the volume frame as given) by the chain
+ * r2s_parameter_transform_fwd ->
+ * r2s_world_to_volume_mapping_fwd (ParameterTransformFwd.cu,
+ * WorldToVolumeMappingFwd.cu; eps added to the voxel-frame diagonal; zeros
+ * for a singular covariance). No counterpart in the reference: the fused
+ * float32 kernels take their means and conics from it in place of the
+ * float32 mapping's, because the Beta profile (1 - d^2)^beta with beta < 1 is
+ * steep at its support edge (a change of 1e-7 in d^2 moves a term by up to
+ * ~3e-4 there). */
+int r2s_voxel_frame(const float* means, const float* quats, const float* scales, const double* ranges,
+                    const double* centers, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, double eps,
+                    uint32_t N, double* frame, r2s_stream_t stream);
+
 /* Fused float32 voxel forward (8^3 tiles) over the 3D binning: render
  * [dim_z, dim_y, dim_x] and, with uncertainties [N] and render_var, the
  * variance sum q alpha^2 (both overwritten). Replaces voxelize_to_voxels_fwd
  * and voxelize_to_voxels_mean_var_fwd (VoxelizeToVoxelsFwd.cu:181-219, :297
  * and the Beta kernels) for float32; the summation order within a voxel
- * differs from the operator (rounding only). means3d [N,3] are the voxel-frame
- * means the binning used (r2s_world_to_volume_mapping_fwd); when the world
- * means [N,3], ranges [3] and centers [3] of that mapping are given too (all
- * three or none), each voxel-frame mean is recomputed in double precision and
- * the part its float32 value rounds off enters every voxel offset, so the
- * profile sees the mean to float32 precision of the offset rather than of the
- * voxel coordinate (the Beta profile (1 - d^2)^beta with beta < 1 is steep at
- * its support edge). */
-int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float* ranges,
-                           const float* centers, const float* conics, const float* opacities,
+ * differs from the operator (rounding only). means3d [N,3] are the float32
+ * voxel-frame means the binning used (r2s_world_to_volume_mapping_fwd); frame
+ * (required) is r2s_voxel_frame's. Gaussian terms take the mean from the frame
+ * as a float pair hi + lo (voxel offsets (v - hi) - lo) and the conic rounded
+ * to float32; Beta terms form d^2 and 1 - d^2 in double from the frame and
+ * decide the cut-off (0 <= d^2, |alpha| >= 1e-5) in double. */
+int r2s_voxelize_fused_fwd(const float* means3d, const double* frame, const float* conics,
+                           const float* opacities,
                            const float* betas, const float* uncertainties, const int32_t* tile_offsets,
                            const int32_t* flatten_ids, uint32_t n_isects, uint32_t N, uint32_t dim_x,
                            uint32_t dim_y, uint32_t dim_z, uint32_t tile_size, uint32_t n_tiles_x,
@@ -451,7 +467,8 @@ int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float
  * called by explicit.py:327-350) without atomics. radii int32 [N,3] and
  * tile_cum uint32 [N+1] (exclusive scan of the per-primitive tile counts) are
  * the binning's own (r2s_isect_tiles_3d_count); isect_ids / flatten_ids its
- * sorted output; means, ranges and centers as for r2s_voxelize_fused_fwd.
+ * sorted output; means3d and frame as for r2s_voxelize_fused_fwd (the same
+ * terms and cut-off decisions).
  * With v_render_var the voxel adjoint becomes v_render + 2 q
  * alpha v_render_var (v_render alone when `detach`, the reference's
  * detach_variance) and v_uncertainties [N] = sum v_render_var alpha^2. One
@@ -461,8 +478,8 @@ int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float
  * v_means3d [N,3], v_conics [N,6], v_opacities [N], v_betas [N] (iff betas),
  * v_uncertainties [N] (iff v_render_var). */
 size_t r2s_voxel_partials_bytes(uint32_t n_isects);
-int r2s_voxelize_fused_vjp(const float* means3d, const float* means, const float* ranges,
-                           const float* centers, const float* conics, const float* opacities,
+int r2s_voxelize_fused_vjp(const float* means3d, const double* frame, const float* conics,
+                           const float* opacities,
                            const float* betas, const float* uncertainties, const int32_t* radii,
                            const uint32_t* tile_cum, const int64_t* isect_ids, const int32_t* flatten_ids,
                            uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
@@ -484,6 +501,7 @@ size_t r2s_mse_partials(uint64_t n);
  * fixed order. float32. */
 int r2s_mse_adjoint(const float* image, const float* target, uint64_t n, float* v_image,
                     float* partials, float* loss, r2s_stream_t stream);
+
 
 /* Adam over a flat float32 buffer (torch.optim.Adam semantics, no weight
  * decay). state uint32 [2]: applied steps (bias correction uses state[0]+1)

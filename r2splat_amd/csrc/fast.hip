@@ -50,29 +50,49 @@ static_assert(sizeof(Splat) == 32, "splat record must be 32 bytes");
 
 // Wide record, for Beta primitives and for primitives with an uncertainty:
 // Beta: d^2 = A dx^2 + B dx dy + C dy^2 (A = a, B = 2 b, C = c of the conic),
-// alpha = w (1 - d^2)^b2d with w = opacity mu ratio(beta), b2d = beta + 1/2;
-// Gaussian with uncertainty: the Splat fields (b2d unused). unc is the
-// primitive's uncertainty q (variance = sum q alpha^2).
+// alpha = w (1 - d^2)^b2d with w = opacity mu ratio(beta), b2d = beta + 1/2,
+// and the mean is mx + lx, my + ly (its float32 value and remainder, from the
+// projection in double; offsets are formed as (p - mx) - lx); Gaussian with
+// uncertainty: the Splat fields (b2d, lx, ly unused). unc is the primitive's
+// uncertainty q (variance = sum q alpha^2).
 struct __align__(16) BetaSplat {
     float mx, my, A, B, C, w;
     uint32_t qlim;  // Beta: bits(D) + 1; Gaussian: bits(S) + 1; or 0
     int32_t g;
-    float b2d, unc, pad1, pad2;
+    float b2d, unc, lx, ly;
 };
 static_assert(sizeof(BetaSplat) == 48, "wide record must be 48 bytes");
 
 template <bool BETA, bool VAR = false>
 using Rec = typename std::conditional<BETA || VAR, BetaSplat, Splat>::type;
 
+// Pixel-minus-mean offsets of a record: Beta records carry their mean's
+// float32 remainder (the profile's slope at its support edge would pass the
+// rounding of a pixel coordinate to the image), the others do not.
+template <bool BETA, typename R>
+__device__ __forceinline__ float offset_x(const R& s, float px) {
+    if constexpr (BETA)
+        return (px - s.mx) - s.lx;
+    else
+        return px - s.mx;
+}
+template <bool BETA, typename R>
+__device__ __forceinline__ float offset_y(const R& s, float py) {
+    if constexpr (BETA)
+        return (py - s.my) - s.ly;
+    else
+        return py - s.my;
+}
+
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2Inv1e5 = 16.609640474436812f;  // log2(1e5)
 constexpr int kDefaultWaves = 8;                     // waves sharing one tile unless chosen otherwise
-// Floats per intersection partial: (v_mx, v_my, v_a, v_b, v_c, v_w) for plain
-// Gaussians, packed back to back (8-byte aligned); Beta and uncertainty kinds
-// add v_b2d and v_q in a 32-byte slot.
-template <bool BETA, bool VAR>
-constexpr int partial_stride() { return (BETA || VAR) ? 8 : 6; }
-constexpr int kPartialStride = 8;  // the widest partial (the buffer is sized for it)
+// Floats per intersection partial, a 32-byte slot: (v_mx, v_my, v_a, v_b,
+// v_c, v_w, v_b2d, v_q), the last two 0 where the kind has no Beta exponent or
+// uncertainty. (Packing plain Gaussians' six floats into 24 bytes measured
+// ~80 us slower in the parameter VJP at 27M intersections: 8-byte loads and
+// LDS reads in place of 16-byte ones.)
+constexpr int kPartialStride = 8;
 
 __device__ __forceinline__ uint32_t qlim_of(float w) {
     const float S = __log2f(fabsf(w)) + kLog2Inv1e5;
@@ -87,6 +107,13 @@ __device__ __forceinline__ uint32_t beta_qlim_of(float w, float b2d) {
 }
 
 __device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
+
+// LDS written by some lanes of a wave is visible to its other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // The record's quadratic form q (Gaussian: sigma log2 e; Beta: d^2) at one
 // pixel, in the two evaluation orders of the raster kernels. The forward
@@ -299,8 +326,35 @@ permute_params_kernel(const uint32_t* __restrict__ perm, uint32_t N, const float
     if (unc) o_unc[p] = unc[n];
 }
 
+// Projected mean and 2D covariance of one primitive for one camera in double
+// precision (its float32 parameters are exact in double): the operator chain
+// parameter transform -> world-to-image projection evaluated in double.
+// Returns false when the primitive is culled (depth <= 0.01).
+__device__ __forceinline__ bool project_f64(const float* __restrict__ viewmat, const float* __restrict__ K,
+                                         const float* __restrict__ mean, const float* __restrict__ quat,
+                                         const float* __restrict__ scale, uint32_t W, uint32_t H, double m2[2],
+                                         double c2[6]) {
+    Camera<double> cam;
+    cam.load(viewmat, K);
+    const double raw[4] = {double(quat[0]), double(quat[1]), double(quat[2]), double(quat[3])};
+    double q[4], mag, norm, R[9], cov[6];
+    unit_quat(raw, q, mag, norm);
+    quat_to_rot(q, R);
+    const double s[3] = {double(scale[0]), double(scale[1]), double(scale[2])};
+    covariance_from(R, s, cov);
+    const double p[3] = {double(mean[0]), double(mean[1]), double(mean[2])};
+    ProjState<double> st;
+    if (!project_state(cam, p, cov, double(W), double(H), st)) return false;
+    m2[0] = cam.fx * st.x / st.z + cam.cx;
+    m2[1] = cam.fy * st.y / st.z + cam.cy;
+    projected_cov(st, c2);
+    return true;
+}
+
 // ---------------------------------------------------------------------------
-// Fused projection + tile count. grid (ceil(N/256), C).
+// Fused projection + tile count. grid (ceil(N/256), C). Culling, radii and the
+// tile rectangle follow the float32 projection (as the operators do); Beta
+// records take their mean, conic and mu from the projection in double.
 // ---------------------------------------------------------------------------
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(256)
@@ -342,9 +396,21 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
         double cd[6], P[3], mu;
 #pragma unroll
         for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
-        conic_mu_f64(cd, double(eps), P, mu);
         rec.mx = mx;
         rec.my = my;
+        if constexpr (BETA) {
+            double md[2], c2d[6];
+            if (project_f64(viewmats + 16 * size_t(c), Ks + 9 * size_t(c), p, quats + 4 * size_t(n),
+                            scales + 3 * size_t(n), W, H, md, c2d)) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) cd[k] = c2d[k];
+                rec.mx = float(md[0]);
+                rec.my = float(md[1]);
+                rec.lx = float(md[0] - double(rec.mx));
+                rec.ly = float(md[1] - double(rec.my));
+            }
+        }
+        conic_mu_f64(cd, double(eps), P, mu);
         if constexpr (BETA) {
             const float mus = float(mu) * ratio[n];  // mu_beta = mu ratio(beta), in float32
             const float w = dens[n] * mus;
@@ -367,7 +433,7 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
             tile_span(my, ry, tile_size, nty, y0, y1)) {
             count = (x1 - x0) * (y1 - y0);
             rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
-            rr = make_float2(my, row_reach(rec));
+            rr = make_float2(rec.my, row_reach(rec));
         }
     }
     recs[idx] = rec;
@@ -432,10 +498,26 @@ __device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, 
     end = max(start, min(end, int32_t(n_live)));
 }
 
-// Sort keys that order tiles by decreasing intersection count (in size classes
-// of 2^(1/8)): key = 255 - floor(8 log2(count + 1)). The raster kernels walk
-// tiles in that order (longest first), so the few heavy tiles of a view do
-// not all land on the same compute units at the end of the launch.
+// The raster kernels deal their n tiles to the 8 XCDs (workgroup b runs on
+// XCD b % 8) as contiguous bands of the camera-major tile index: band x holds
+// per + (x < rem) tiles (per = n / 8, rem = n % 8), so the tiles that gather
+// the same records (neighbours in one view) share one XCD's L2. xcd_band is a
+// tile's band; xcd_slot the position of workgroup b in the band-major order.
+__device__ __forceinline__ uint32_t xcd_band(uint32_t t, uint32_t n) {
+    const uint32_t per = n / 8u, rem = n % 8u, big = rem * (per + 1u);
+    return t < big ? t / (per + 1u) : rem + (t - big) / per;  // per >= 1 whenever t >= big
+}
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
+    const uint32_t per = n / 8u, rem = n % 8u, x = b % 8u;
+    return x * per + min(x, rem) + b / 8u;
+}
+
+// Sort keys that order tiles by XCD band, then by decreasing intersection
+// count (in size classes of 2^(1/8)): key = band << 8 | (255 - floor(8
+// log2(count + 1))). Each XCD walks its band's tiles longest first, so the
+// few heavy tiles of a view do not all land on the same compute units at the
+// end of the launch.
+constexpr uint32_t kTileOrderBits = 11;
 __global__ void __launch_bounds__(256)
 fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n,
                              uint32_t capacity, uint32_t C, uint32_t nt, uint32_t* __restrict__ keys,
@@ -446,7 +528,7 @@ fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t
     int32_t start, end;
     fused_tile(offsets, t, C * nt, n_live, start, end);
     const float cls = floorf(8.f * __log2f(float(end - start) + 1.f));
-    keys[t] = 255u - min(255u, uint32_t(cls));
+    keys[t] = (xcd_band(t, C * nt) << 8) | (255u - min(255u, uint32_t(cls)));
     vals[t] = t;
 }
 
@@ -746,12 +828,12 @@ __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, floa
     constexpr uint32_t kAll = PX >= 32 ? 0xffffffffu : (1u << PX) - 1u;
     if (rows == 0u) return;
     const bool all = rows == kAll;
-    const float dx = px - s.mx;
+    const float dx = offset_x<BETA>(s, px);
     const float adx2 = s.A * dx * dx, bdx = s.B * dx;
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         if (!all && !((rows >> k) & 1u)) continue;
-        const float dy = py[k] - s.my;
+        const float dy = offset_y<BETA>(s, py[k]);
         const float q = form_by_columns(adx2, bdx, s.C, dy);  // Gaussian: sigma log2 e; Beta: d^2
         float e;
         if constexpr (BETA)
@@ -767,12 +849,6 @@ __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, floa
             acc[k] = fmaf(ek, s.w, acc[k]);
         }
     }
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // List entry i ({record index, emission index}), or an empty entry past the list.
@@ -800,7 +876,7 @@ __device__ __forceinline__ R gather_splat(const R* __restrict__ recs, uint2 e) {
 template <typename R>
 __device__ __forceinline__ void settle(const R& s) {
     asm volatile("" ::"v"(s.mx), "v"(s.my), "v"(s.A), "v"(s.B), "v"(s.C), "v"(s.w), "v"(s.qlim), "v"(s.g));
-    if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d), "v"(s.unc));
+    if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d), "v"(s.unc), "v"(s.lx), "v"(s.ly));
 }
 
 template <int T, int WAVES, bool BETA, bool VAR>
@@ -816,7 +892,7 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
     using R = Rec<BETA, VAR>;
     __shared__ R stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
+    const uint32_t tile = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -926,7 +1002,7 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
 // of 6 (its 75-VGPR default); the other variants keep the default (forcing
 // the next occupancy step spills them).
 template <int T, bool BETA, bool VAR>
-constexpr int bwd_waves_per_eu() { return (T == 16 && !BETA && !VAR) ? 7 : 1; }
+constexpr int bwd_waves_per_eu() { return (T == 16 && !BETA && !VAR) ? 5 : 1; }
 
 template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
@@ -938,7 +1014,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
                         float* __restrict__ isect_grads) {
     __shared__ __align__(16) float sv[VAR ? 2 : 1][T * T];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
+    const uint32_t tile = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -971,7 +1047,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
         settle(s);
         next = gather_splat(recs, ahead);
         ahead = load_entry(entries, chunk + 2 * kStride + int32_t(lane), end);
-        const float dx0 = x0 - s.mx;
+        const float dx0 = offset_x<BETA>(s, x0);
         float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f, vq = 0.f;
         float bm1 = 0.f, kvar = 0.f;
         if constexpr (BETA) bm1 = s.b2d - 1.f;
@@ -980,8 +1056,15 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
         // the wave runs as many rows as its tallest record needs
         int r_lo = 0, r_hi = 0;
         if (active) row_span(s, y0 - 0.5f, T, r_lo, r_hi);
+        // the columns' offsets and their squares are the same in every row
+        float dxc[T], dxsq[T];
+#pragma unroll
+        for (int j = 0; j < T; ++j) {
+            dxc[j] = dx0 + float(j);
+            dxsq[j] = dxc[j] * dxc[j];
+        }
         for (int r = r_lo; r < r_hi; ++r) {
-            const float dy = y0 + float(r) - s.my;
+            const float dy = offset_y<BETA>(s, y0 + float(r));
             const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
             const float4* row = reinterpret_cast<const float4*>(sv[0] + r * T);
             float v[T], vv[VAR ? T : 1];
@@ -1007,7 +1090,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
             float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
             for (int j = 0; j < T; ++j) {
-                const float dx = dx0 + float(j);
+                const float dx = dxc[j];
                 const float q = form_by_rows(s.A, dx, bdy, cdy2);
                 float g;
                 if constexpr (BETA) {
@@ -1040,9 +1123,8 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
                     }
                 }
                 r0 += g;
-                const float gx = g * dx;
-                r1 += gx;
-                r2 = fmaf(gx, dx, r2);
+                r1 = fmaf(g, dx, r1);
+                r2 = fmaf(g, dxsq[j], r2);
             }
             m0 += r0;
             mx += r1;
@@ -1052,7 +1134,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
             myy = fmaf(dy * dy, r0, myy);
         }
         if (active && s.g >= 0 && uint32_t(s.g) < n_live) {
-            float* out = isect_grads + partial_stride<BETA, VAR>() * size_t(s.g);
+            float* out = isect_grads + kPartialStride * size_t(s.g);
             const float wq = s.w * s.w * vq;  // sum v_var alpha^2 (the uncertainty's gradient)
             if constexpr (BETA) {
                 const float a = s.A, b = 0.5f * s.B, c = s.C, k = s.w * s.b2d;
@@ -1065,16 +1147,9 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
                 const float a = 2.f * s.A / kLog2e, b = s.B / kLog2e, c = 2.f * s.C / kLog2e;
                 const float g0 = s.w * (a * mx + b * my), g1 = s.w * (b * mx + c * my), g2 = -0.5f * s.w * mxx,
                             g3 = -s.w * mxy, g4 = -0.5f * s.w * myy;
-                if constexpr (VAR) {
-                    float4* gp = reinterpret_cast<float4*>(out);
-                    gp[0] = make_float4(g0, g1, g2, g3);
-                    gp[1] = make_float4(g4, m0, 0.f, wq);
-                } else {  // 24 bytes, three 8-byte stores
-                    float2* gp = reinterpret_cast<float2*>(out);
-                    gp[0] = make_float2(g0, g1);
-                    gp[1] = make_float2(g2, g3);
-                    gp[2] = make_float2(g4, m0);
-                }
+                float4* gp = reinterpret_cast<float4*>(out);
+                gp[0] = make_float4(g0, g1, g2, g3);
+                gp[1] = make_float4(g4, m0, 0.f, VAR ? wq : 0.f);
             }
         }
     }
@@ -1114,14 +1189,14 @@ fused_keep_mask_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* 
         for (int b = 0; b < 32; ++b) {
             const int pix = word * 32 + b, r = pix / T, col = pix % T;
             if (r_lo < r_hi && r / STEP >= k_lo && r / STEP < k_hi) {
-                const float dx = (float(tx * T + uint32_t(col)) + 0.5f) - s.mx;
+                const float dx = offset_x<BETA>(s, float(tx * T + uint32_t(col)) + 0.5f);
                 const float adx2 = s.A * dx * dx, bdx = s.B * dx;
-                const float dy = (float(ty * T + uint32_t(r)) + 0.5f) - s.my;
+                const float dy = offset_y<BETA>(s, float(ty * T + uint32_t(r)) + 0.5f);
                 if (contributes(form_by_columns(adx2, bdx, s.C, dy), s.qlim)) bits_fwd |= 1u << b;
             }
             if (r >= r_lo && r < r_hi) {
-                const float dx = (x0 - s.mx) + float(col);
-                const float dy = y0 + float(r) - s.my;
+                const float dx = offset_x<BETA>(s, x0) + float(col);
+                const float dy = offset_y<BETA>(s, y0 + float(r));
                 const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
                 if (contributes(form_by_rows(s.A, dx, bdy, cdy2), s.qlim)) bits_bwd |= 1u << b;
             }
@@ -1204,31 +1279,22 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             for (uint32_t base = wlo; base < whi; base += kVjpStageParts) {
                 const uint32_t cnt = min(kVjpStageParts, whi - base);
                 wave_lds_sync();  // every lane is done with the previous chunk
-                if constexpr (BETA || VAR) {
-                    for (uint32_t i = lane; i < cnt; i += kWave) {
-                        const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(base + i));
-                        float4* sp = reinterpret_cast<float4*>(stage + kPartialStride * i);
-                        sp[0] = gp[0];
-                        sp[1] = gp[1];
-                    }
-                } else {  // the range's packed 6-float partials as a flat run of 8-byte pairs
-                    const float2* src = reinterpret_cast<const float2*>(isect_grads + 6 * size_t(base));
-                    float2* dst = reinterpret_cast<float2*>(stage);
-                    for (uint32_t i = lane; i < 3 * cnt; i += kWave) dst[i] = src[i];
+                for (uint32_t i = lane; i < cnt; i += kWave) {
+                    const float4* gp = reinterpret_cast<const float4*>(isect_grads + kPartialStride * size_t(base + i));
+                    float4* sp = reinterpret_cast<float4*>(stage + kPartialStride * i);
+                    sp[0] = gp[0];
+                    sp[1] = gp[1];
                 }
                 wave_lds_sync();
                 const uint32_t j1 = min(hi, base + cnt);
                 for (uint32_t j = max(lo, base); j < j1; ++j) {
+                    const float4* gp = reinterpret_cast<const float4*>(stage + kPartialStride * (j - base));
+                    const float4 a = gp[0], b = gp[1];
+                    g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
+                    g4 += b.x; g5 += b.y;
                     if constexpr (BETA || VAR) {
-                        const float4* gp = reinterpret_cast<const float4*>(stage + kPartialStride * (j - base));
-                        const float4 a = gp[0], b = gp[1];
-                        g0 += a.x; g1 += a.y; g2 += a.z; g3 += a.w;
-                        g4 += b.x; g5 += b.y; g6 += b.z; g7 += b.w;
-                    } else {
-                        const float2* gp = reinterpret_cast<const float2*>(stage + 6 * (j - base));
-                        const float2 a = gp[0], b = gp[1], c = gp[2];
-                        g0 += a.x; g1 += a.y; g2 += b.x; g3 += b.y;
-                        g4 += c.x; g5 += c.y;
+                        g6 += b.z;
+                        g7 += b.w;
                     }
                 }
             }
@@ -1593,7 +1659,8 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
             tile_offsets, counters, capacity, C, ntx * nty, sort_keys_buffer(order_ws),
             sort_vals_buffer(order_ws, n_tiles));
         if (int rc = check_launch("fused_tile_order")) return rc;
-        if (int rc = sort_pairs_u32(order_ws, L.order_sort_bytes, n_tiles, nullptr, 8, at<uint32_t>(scratch, L.order_keys),
+        if (int rc = sort_pairs_u32(order_ws, L.order_sort_bytes, n_tiles, nullptr, kTileOrderBits,
+                                    at<uint32_t>(scratch, L.order_keys),
                                     tile_order, stream))
             return rc;
     }

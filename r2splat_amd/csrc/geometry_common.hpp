@@ -75,17 +75,20 @@ __device__ __forceinline__ void covariance_from(const S R[9], const S s[3], S co
 template <typename S>
 struct Camera {
     S R[9], t[3], fx, fy, cx, cy;
-    __device__ __forceinline__ void load(const S* __restrict__ viewmat, const S* __restrict__ K) {
+    // from a viewmat [4,4] and K [3,3] of any floating type (float32 cameras
+    // are exact in double)
+    template <typename T>
+    __device__ __forceinline__ void load(const T* __restrict__ viewmat, const T* __restrict__ K) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j) R[3 * i + j] = viewmat[4 * i + j];
-            t[i] = viewmat[4 * i + 3];
+            for (int j = 0; j < 3; ++j) R[3 * i + j] = S(viewmat[4 * i + j]);
+            t[i] = S(viewmat[4 * i + 3]);
         }
-        fx = K[0];
-        fy = K[4];
-        cx = K[2];
-        cy = K[5];
+        fx = S(K[0]);
+        fy = S(K[4]);
+        cx = S(K[2]);
+        cy = S(K[5]);
     }
 };
 

@@ -25,6 +25,6 @@ int check_launch(const char* where) {
 
 }  // namespace r2s
 
-extern "C" int r2s_abi_version(void) { return 3; }
+extern "C" int r2s_abi_version(void) { return 4; }
 
 extern "C" const char* r2s_last_error(void) { return r2s::g_error; }

@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "binning.hpp"
+#include "geometry_common.hpp"
 #include "raster_common.hpp"
 
 namespace r2s {
@@ -106,12 +107,9 @@ template <typename S>
 struct VoxArgs {
     const S* means3d; const S* conics; const S* opac; const S* betas; const S* unc;
     const int32_t* offsets; const int32_t* flatten; uint32_t n_isects, N, dx, dy, dz, T, ntx, nty, ntz;
-    // fused float32 kernels: the world-frame means and the volume frame, from
-    // which each voxel-frame mean is recomputed in double precision to carry
-    // the part that its float32 value (means3d) rounds off
-    const float* world = nullptr;
-    const float* ranges = nullptr;
-    const float* centers = nullptr;
+    // fused float32 kernels: each primitive's voxel-frame mean and inverse
+    // covariance in double precision (kFrame doubles per primitive)
+    const double* frame = nullptr;
 };
 
 template <typename S>
@@ -325,41 +323,73 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// Voxel-frame mean m = (world - center) dims / extent + dims / 2 split into the
-// float32 mean the binning used (means3d) and the float32 remainder lo = m -
-// means3d, with m in double precision. Voxel offsets are then formed as
-// (v - hi) - lo: the float32 rounding of the mean itself (up to half an ulp of
-// the voxel coordinate, 1.5e-5 voxels at 256) never enters the profile, whose
-// compact Beta form (1 - d^2)^beta with beta < 1 is steep at its support edge.
-struct VoxFrame {
-    double scale[3], shift[3], centre[3];
-    bool on;
-};
 
-__device__ __forceinline__ VoxFrame vox_frame(const VoxArgs<float>& A) {
-    VoxFrame f = {};
-    f.on = A.world != nullptr;
-    if (!f.on) return f;
-    const double dims[3] = {double(A.dx), double(A.dy), double(A.dz)};
+// The fused float32 kernels read each primitive's voxel-frame mean and
+// inverse covariance from a double-precision frame, kFrame doubles per
+// primitive (mean x y z, then the conic xx xy xz yy yz zz), computed from the
+// world-frame parameters in double (vox_frame_kernel: the operator chain
+// parameter transform -> volume mapping, in double). Gaussian offsets are
+// formed as (v - hi) - lo with hi + lo the mean split into two floats, so the
+// float32 rounding of a voxel coordinate (up to 1.5e-5 voxels at 256) never
+// enters the profile. The Beta profile (1 - d^2)^beta is steep at its support
+// edge when beta < 1 (there a change of 1e-7 in d^2 moves a term by up to
+// ~3e-4), so Beta kernels form d^2 and 1 - d^2 in double from the frame.
+constexpr int kFrame = 9;
+
+__global__ void __launch_bounds__(256)
+vox_frame_kernel(const float* __restrict__ means, const float* __restrict__ quats, const float* __restrict__ scales,
+                 const double* __restrict__ ranges, const double* __restrict__ centers, uint32_t dx, uint32_t dy,
+                 uint32_t dz, double eps, uint32_t N, double* __restrict__ frame) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double raw[4] = {double(quats[4 * size_t(n)]), double(quats[4 * size_t(n) + 1]),
+                           double(quats[4 * size_t(n) + 2]), double(quats[4 * size_t(n) + 3])};
+    double q[4], mag, norm, R[9], cov[6];
+    unit_quat(raw, q, mag, norm);
+    quat_to_rot(q, R);
+    const double s[3] = {double(scales[3 * size_t(n)]), double(scales[3 * size_t(n) + 1]),
+                         double(scales[3 * size_t(n) + 2])};
+    covariance_from(R, s, cov);
+    const double D[3] = {double(dx), double(dy), double(dz)};
+    const double r[3] = {D[0] / ranges[0], D[1] / ranges[1], D[2] / ranges[2]};
+    double* f = frame + kFrame * size_t(n);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        f.scale[k] = dims[k] / double(A.ranges[k]);
-        f.shift[k] = dims[k] * 0.5;
-        f.centre[k] = double(A.centers[k]);
-    }
-    return f;
+    for (int k = 0; k < 3; ++k) f[k] = (double(means[3 * size_t(n) + k]) - centers[k]) * r[k] + D[k] * 0.5;
+    const double sv[6] = {r[0] * r[0] * cov[0] + eps, r[0] * r[1] * cov[1], r[0] * r[2] * cov[2],
+                          r[1] * r[1] * cov[3] + eps, r[1] * r[2] * cov[4], r[2] * r[2] * cov[5] + eps};
+    double inv[6];
+    spd3_inverse(sv, inv);  // zeros when singular (such primitives have zero radii and no tiles)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) f[3 + k] = inv[k];
 }
 
-__device__ __forceinline__ void vox_mean_lo(const VoxArgs<float>& A, const VoxFrame& f, size_t G, const float hi[3],
-                                            float lo[3]) {
+// Mean of primitive G split into float hi + lo (lo = 0 without a frame).
+__device__ __forceinline__ void vox_mean_split(const VoxArgs<float>& A, size_t G, float hi[3], float lo[3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        lo[k] = 0.f;
-        if (f.on) {
-            const double m = (double(A.world[3 * G + k]) - f.centre[k]) * f.scale[k] + f.shift[k];
+        if (A.frame) {
+            const double m = A.frame[kFrame * G + k];
+            hi[k] = float(m);
             lo[k] = float(m - double(hi[k]));
+        } else {
+            hi[k] = A.means3d[3 * G + k];
+            lo[k] = 0.f;
         }
     }
+}
+
+// Conic of primitive G (from the frame when there is one).
+__device__ __forceinline__ void vox_conic(const VoxArgs<float>& A, size_t G, float P[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P[k] = A.frame ? float(A.frame[kFrame * G + 3 + k]) : A.conics[6 * G + k];
+}
+
+// Largest Beta d^2 whose term passes the cut-off, in double: |alpha| =
+// |w| (1 - d^2)^beta >= 1e-5 for 0 <= d^2 <= qmax; negative when none does.
+__device__ __forceinline__ double beta_qmax(float w, float beta) {
+    const double aw = fabs(double(w));
+    if (!(aw >= 1e-5) || !(beta > 0.f)) return -1.0;
+    return 1.0 - pow(1e-5 / aw, 1.0 / double(beta));
 }
 
 constexpr uint32_t kVoxPartial = 12;  // floats per intersection partial (11, or 12 with a variance)
@@ -379,8 +409,10 @@ __device__ __forceinline__ uint32_t vox_qlim(float opac, float beta, bool is_bet
 
 // VAR: the voxel adjoint becomes v + var_scale q alpha v_var (var_scale 2, or
 // 0 when the variance is detached) and the partial's twelfth slot carries
-// v_q = sum v_var alpha^2 (variance = sum q alpha^2).
-template <bool BETA, bool VAR>
+// v_q = sum v_var alpha^2 (variance = sum q alpha^2). WIDE (Beta with a
+// frame): q and h = 1 - q are formed in double from the frame, with the
+// cut-off 0 <= q <= qmax decided in double, exactly as the fused forward does.
+template <bool BETA, bool VAR, bool WIDE = false>
 __global__ void __launch_bounds__(kMomThreads)
 vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, const float* __restrict__ v_render,
                        const float* __restrict__ v_render_var, float var_scale,
@@ -400,7 +432,7 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
     __syncthreads();
     const uint32_t n_here = e1 - e0;
     const int64_t nt = int64_t(A.ntx) * A.nty * A.ntz;
-    const VoxFrame frame = vox_frame(A);
+    static_assert(!WIDE || BETA, "the double-precision form is for Beta profiles");
     uint32_t pos = 0;  // first chunk entry of the current tile
     while (pos < n_here) {
         const int64_t tile = ids[pos];
@@ -426,15 +458,24 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
             const int32_t g = A.flatten[e];
             if (g >= 0 && uint32_t(g) < A.N) {
                 const size_t G = size_t(g);
-                const float mx = A.means3d[3 * G], my = A.means3d[3 * G + 1], mz = A.means3d[3 * G + 2];
-                const float hi3[3] = {mx, my, mz};
-                float lo3[3];
-                vox_mean_lo(A, frame, G, hi3, lo3);
-                float P[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) P[k] = A.conics[6 * G + k];
+                float hi3[3], lo3[3], P[6];
+                vox_mean_split(A, G, hi3, lo3);
+                vox_conic(A, G, P);
+                const float mx = hi3[0], my = hi3[1], mz = hi3[2];
                 const float opac = A.opac[g], beta = BETA ? A.betas[g] : 0.f;
-                const uint32_t qlim = vox_qlim(opac, beta, BETA);
+                uint32_t qlim = vox_qlim(opac, beta, BETA);
+                double md[3] = {0.0, 0.0, 0.0}, Pd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, qmax = -1.0;
+                if constexpr (WIDE) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) md[k] = A.frame[kFrame * G + k];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) Pd[k] = A.frame[kFrame * G + 3 + k];
+                    qmax = beta_qmax(opac, beta);
+                    qlim = qmax >= 0.0 ? 1u : 0u;  // liveness only: the cut-off is q <= qmax
+                }
+                const double ad = Pd[0], bxyd = 2.0 * Pd[1], bxzd = 2.0 * Pd[2], cyyd = Pd[3], cyzd = 2.0 * Pd[4],
+                             czzd = Pd[5];
+                const double dx0d = double(float(tx * T) + 0.5f) - md[0];
                 const float kvar = VAR ? var_scale * A.unc[g] * opac : 0.f;  // d variance / d (alpha / w) factor
                 // q = a dx^2 + bxy dx dy + bxz dx dz + cyy dy^2 + cyz dy dz + czz dz^2
                 const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
@@ -445,9 +486,11 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                 const float dx0 = (x0 - mx) - lo3[0];
                 if (qlim != 0u) {
                     for (uint32_t z = 0; z < T; ++z) {
-                        const float dz = ((z0 + float(z)) - mz) - lo3[2];
+                        const double dzd = double(z0 + float(z)) - md[2];
+                        const float dz = WIDE ? float(dzd) : ((z0 + float(z)) - mz) - lo3[2];
                         for (uint32_t y = 0; y < T; ++y) {
-                            const float dy = ((y0 + float(y)) - my) - lo3[1];
+                            const double dyd = double(y0 + float(y)) - md[1];
+                            const float dy = WIDE ? float(dyd) : ((y0 + float(y)) - my) - lo3[1];
                             const float4* row = reinterpret_cast<const float4*>(sv + (z * T + y) * T);
                             const float4 va = row[0], vb = row[1];
                             const float v[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
@@ -460,15 +503,29 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
                             }
                             const float c1 = fmaf(bxy, dy, bxz * dz);
                             const float c0 = fmaf(dy, fmaf(cyy, dy, cyz * dz), czz * dz * dz);
+                            const double c1d = fma(bxyd, dyd, bxzd * dzd);
+                            const double c0d = fma(dyd, fma(cyyd, dyd, cyzd * dzd), czzd * dzd * dzd);
                             float r0 = 0.f, r1 = 0.f, r2 = 0.f, re = 0.f, rl = 0.f;
 #pragma unroll
                             for (int x = 0; x < 8; ++x) {
-                                const float dx = dx0 + float(x);
-                                const float q = fmaf(dx, fmaf(a, dx, c1), c0);
-                                const bool in = __float_as_uint(q) < qlim;
+                                float dx, q, h;
+                                bool in;
+                                if constexpr (WIDE) {
+                                    const double dxd = dx0d + double(x);
+                                    const double qd = fma(dxd, fma(ad, dxd, c1d), c0d);
+                                    in = qd >= 0.0 && qd <= qmax;
+                                    h = float(1.0 - qd);
+                                    q = 0.f;
+                                    dx = float(dxd);
+                                } else {
+                                    dx = dx0 + float(x);
+                                    q = fmaf(dx, fmaf(a, dx, c1), c0);
+                                    in = __float_as_uint(q) < qlim;
+                                    h = 1.f - q;
+                                }
                                 float gv;
                                 if constexpr (BETA) {
-                                    const float h = 1.f - q, lg = __log2f(h);
+                                    const float lg = __log2f(h);
                                     const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(beta - 1)
                                     float ve;
                                     if constexpr (VAR) {
@@ -570,42 +627,82 @@ vox_bwd_moments_kernel(VoxArgs<float> A, const int64_t* __restrict__ isect_ids, 
 // ---------------------------------------------------------------------------
 constexpr int kVoxFwdWaves = 4;
 
+// Gaussian record: the quadratic form in log2 units, the mean as hi + lo floats.
 struct __align__(16) VoxRec {
     float mx, my, mz, a, bxy, bxz, cyy, cyz, czz, w, beta;
     uint32_t qlim;
-    float lx, ly, lz, pad;  // the mean's float32 remainder (vox_mean_lo)
+    float lx, ly, lz, pad;  // the mean's float32 remainder (vox_mean_split)
 };
 static_assert(sizeof(VoxRec) == 64, "voxel record must be 64 bytes");
 
+// Beta record: the mean and the conic (off-diagonals doubled) in double; a
+// term passes the cut-off when 0 <= d^2 <= qmax (qlim = 1 when some term can).
+struct __align__(16) VoxRecB {
+    double mx, my, mz, a, bxy, bxz, cyy, cyz, czz, qmax;
+    float w, beta;
+    uint32_t qlim, pad;
+};
+static_assert(sizeof(VoxRecB) == 96, "Beta voxel record must be 96 bytes");
+
 template <bool BETA>
-__device__ __forceinline__ VoxRec load_vox_rec(const VoxArgs<float>& A, const VoxFrame& frame, int32_t i,
-                                               int32_t hi) {
+using VoxRecOf = typename std::conditional<BETA, VoxRecB, VoxRec>::type;
+
+__device__ __forceinline__ VoxRec load_gauss_rec(const VoxArgs<float>& A, int32_t i, int32_t hi) {
     VoxRec r = {};
     if (i >= hi) return r;
     const int32_t g = A.flatten[i];
     if (g < 0 || uint32_t(g) >= A.N) return r;
     const size_t G = size_t(g);
-    const float* c = A.conics + 6 * G;
-    const float s = BETA ? 1.f : 0.5f * kL2E, t2 = BETA ? 2.f : kL2E;
-    r.mx = A.means3d[3 * G];
-    r.my = A.means3d[3 * G + 1];
-    r.mz = A.means3d[3 * G + 2];
-    const float hi3[3] = {r.mx, r.my, r.mz};
-    float lo3[3];
-    vox_mean_lo(A, frame, G, hi3, lo3);
-    r.lx = lo3[0];
-    r.ly = lo3[1];
-    r.lz = lo3[2];
-    r.a = s * c[0];
-    r.bxy = t2 * c[1];
-    r.bxz = t2 * c[2];
-    r.cyy = s * c[3];
-    r.cyz = t2 * c[4];
-    r.czz = s * c[5];
+    float h3[3], l3[3], P[6];
+    vox_mean_split(A, G, h3, l3);
+    vox_conic(A, G, P);
+    constexpr float s = 0.5f * kL2E, t2 = kL2E;
+    r.mx = h3[0];
+    r.my = h3[1];
+    r.mz = h3[2];
+    r.lx = l3[0];
+    r.ly = l3[1];
+    r.lz = l3[2];
+    r.a = s * P[0];
+    r.bxy = t2 * P[1];
+    r.bxz = t2 * P[2];
+    r.cyy = s * P[3];
+    r.cyz = t2 * P[4];
+    r.czz = s * P[5];
     r.w = A.opac[g];
-    r.beta = BETA ? A.betas[g] : 0.f;
-    r.qlim = vox_qlim(r.w, r.beta, BETA);
+    r.qlim = vox_qlim(r.w, 0.f, false);
     return r;
+}
+
+// Needs the frame (the fused entry points require it).
+__device__ __forceinline__ VoxRecB load_beta_rec(const VoxArgs<float>& A, int32_t i, int32_t hi) {
+    VoxRecB r = {};
+    if (i >= hi) return r;
+    const int32_t g = A.flatten[i];
+    if (g < 0 || uint32_t(g) >= A.N) return r;
+    const double* f = A.frame + kFrame * size_t(g);
+    r.mx = f[0];
+    r.my = f[1];
+    r.mz = f[2];
+    r.a = f[3];
+    r.bxy = 2.0 * f[4];
+    r.bxz = 2.0 * f[5];
+    r.cyy = f[6];
+    r.cyz = 2.0 * f[7];
+    r.czz = f[8];
+    r.w = A.opac[g];
+    r.beta = A.betas[g];
+    r.qmax = beta_qmax(r.w, r.beta);
+    r.qlim = r.qmax >= 0.0 ? 1u : 0u;
+    return r;
+}
+
+template <bool BETA>
+__device__ __forceinline__ VoxRecOf<BETA> load_vox_rec(const VoxArgs<float>& A, int32_t i, int32_t hi) {
+    if constexpr (BETA)
+        return load_beta_rec(A, i, hi);
+    else
+        return load_gauss_rec(A, i, hi);
 }
 
 // The uncertainty of the primitive a list entry names (0 past the list).
@@ -617,11 +714,14 @@ __device__ __forceinline__ float load_vox_unc(const VoxArgs<float>& A, int32_t i
 
 // VAR also accumulates the variance sum q alpha^2 (q the primitive's
 // uncertainty, staged beside the record) and sums it in a second LDS phase.
+// Beta terms form d^2 and 1 - d^2 in double (the record's mean and conic),
+// decide the cut-off on d^2 in double and take the power in float32.
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kVoxFwdWaves * kWave)
 vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out, float* __restrict__ out_var) {
     constexpr uint32_t T = kMomT;
-    __shared__ VoxRec stage[kVoxFwdWaves][kWave];
+    using R = VoxRecOf<BETA>;
+    __shared__ R stage[kVoxFwdWaves][kWave];
     __shared__ float stage_q[VAR ? kVoxFwdWaves : 1][kWave];
     __shared__ float part[kVoxFwdWaves][T * T * T];
     const uint32_t blk = blockIdx.x;
@@ -642,41 +742,53 @@ vox_fwd_fused_kernel(VoxArgs<float> A, float* __restrict__ out, float* __restric
         acc[x] = 0.f;
         acc_var[x] = 0.f;
     }
-    VoxRec* slot = stage[wave];
-    const VoxFrame frame = vox_frame(A);
-    VoxRec next = load_vox_rec<BETA>(A, frame, lo + int32_t(lane), hi);
+    R* slot = stage[wave];
+    R next = load_vox_rec<BETA>(A, lo + int32_t(lane), hi);
     float next_q = VAR ? load_vox_unc(A, lo + int32_t(lane), hi) : 0.f;
     for (int32_t base = lo; base < hi; base += kWave) {
         wave_sync_lds();
         slot[lane] = next;
         if constexpr (VAR) stage_q[wave][lane] = next_q;
         wave_sync_lds();
-        next = load_vox_rec<BETA>(A, frame, base + kWave + int32_t(lane), hi);
+        next = load_vox_rec<BETA>(A, base + kWave + int32_t(lane), hi);
         if constexpr (VAR) next_q = load_vox_unc(A, base + kWave + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         for (int32_t j = 0; j < n; ++j) {
-            const VoxRec r = slot[j];
+            const R r = slot[j];
             if (r.qlim == 0u) continue;  // wave-uniform (every lane reads the same record)
             const float unc = VAR ? stage_q[VAR ? wave : 0][j] : 0.f;
-            const float dy = (py - r.my) - r.ly, dz = (pz - r.mz) - r.lz, dx0 = (px0 - r.mx) - r.lx;
-            const float c1 = fmaf(r.bxy, dy, r.bxz * dz);
-            const float c0 = fmaf(dy, fmaf(r.cyy, dy, r.cyz * dz), r.czz * dz * dz);
+            float ek[T];
+            if constexpr (BETA) {
+                const double dy = double(py) - r.my, dz = double(pz) - r.mz, dx0 = double(px0) - r.mx;
+                const double c1 = fma(r.bxy, dy, r.bxz * dz);
+                const double c0 = fma(dy, fma(r.cyy, dy, r.cyz * dz), r.czz * dz * dz);
+#pragma unroll
+                for (uint32_t x = 0; x < T; ++x) {
+                    const double dx = dx0 + double(x);
+                    const double q = fma(dx, fma(r.a, dx, c1), c0);
+                    const float e = __builtin_amdgcn_exp2f(r.beta * __log2f(float(1.0 - q)));
+                    ek[x] = (q >= 0.0 && q <= r.qmax) ? e : 0.f;
+                }
+            } else {
+                const float dy = (py - r.my) - r.ly, dz = (pz - r.mz) - r.lz, dx0 = (px0 - r.mx) - r.lx;
+                const float c1 = fmaf(r.bxy, dy, r.bxz * dz);
+                const float c0 = fmaf(dy, fmaf(r.cyy, dy, r.cyz * dz), r.czz * dz * dz);
+#pragma unroll
+                for (uint32_t x = 0; x < T; ++x) {
+                    const float dx = dx0 + float(x);
+                    const float q = fmaf(dx, fmaf(r.a, dx, c1), c0);
+                    const float e = __builtin_amdgcn_exp2f(-q);
+                    ek[x] = __float_as_uint(q) < r.qlim ? e : 0.f;
+                }
+            }
 #pragma unroll
             for (uint32_t x = 0; x < T; ++x) {
-                const float dx = dx0 + float(x);
-                const float q = fmaf(dx, fmaf(r.a, dx, c1), c0);
-                float e;
-                if constexpr (BETA)
-                    e = __builtin_amdgcn_exp2f(r.beta * __log2f(1.f - q));
-                else
-                    e = __builtin_amdgcn_exp2f(-q);
-                const float ek = __float_as_uint(q) < r.qlim ? e : 0.f;
                 if constexpr (VAR) {
-                    const float alpha = ek * r.w;
+                    const float alpha = ek[x] * r.w;
                     acc[x] += alpha;
                     acc_var[x] = fmaf(unc * alpha, alpha, acc_var[x]);
                 } else {
-                    acc[x] = fmaf(ek, r.w, acc[x]);
+                    acc[x] = fmaf(ek[x], r.w, acc[x]);
                 }
             }
         }
@@ -865,8 +977,18 @@ extern "C" size_t r2s_voxel_partials_bytes(uint32_t n_isects) {
     return size_t(n_isects) * kVoxPartial * sizeof(float);
 }
 
-extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* means, const float* ranges,
-                                      const float* centers, const float* conics, const float* opacities,
+extern "C" int r2s_voxel_frame(const float* means, const float* quats, const float* scales, const double* ranges,
+                               const double* centers, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z, double eps,
+                               uint32_t N, double* frame, r2s_stream_t stream) {
+    if (N == 0) return R2S_OK;
+    R2S_REQUIRE(means && quats && scales && ranges && centers && frame, "voxel_frame: null argument");
+    vox_frame_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(means, quats, scales, ranges, centers, dim_x, dim_y,
+                                                           dim_z, eps, N, frame);
+    return check_launch("voxel_frame");
+}
+
+extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const double* frame, const float* conics,
+                                      const float* opacities,
                                       const float* betas, const float* uncertainties, const int32_t* radii,
                                       const uint32_t* tile_cum, const int64_t* isect_ids, const int32_t* flatten_ids,
                                       uint32_t n_isects, uint32_t N, uint32_t dim_x, uint32_t dim_y, uint32_t dim_z,
@@ -883,16 +1005,13 @@ extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* means, 
     R2S_REQUIRE(!beta || v_betas != nullptr, "v_betas is required with betas");
     R2S_REQUIRE(!var || (uncertainties != nullptr && v_uncertainties != nullptr),
                 "a variance adjoint needs uncertainties and v_uncertainties");
-    R2S_REQUIRE((means == nullptr) == (ranges == nullptr) && (means == nullptr) == (centers == nullptr),
-                "means, ranges and centers go together");
+    R2S_REQUIRE(frame != nullptr, "voxelize_fused_vjp: the double-precision frame is required");
     if (N == 0) return R2S_OK;
     if (n_isects > 0) {
         auto A = make_vox_args<float>(means3d, conics, opacities, betas, uncertainties, nullptr, flatten_ids,
                                       n_isects, N, dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
-        A.world = means;
-        A.ranges = ranges;
-        A.centers = centers;
-        auto km = beta ? (var ? vox_bwd_moments_kernel<true, true> : vox_bwd_moments_kernel<true, false>)
+        A.frame = frame;
+        auto km = beta ? (var ? vox_bwd_moments_kernel<true, true, true> : vox_bwd_moments_kernel<true, false, true>)
                        : (var ? vox_bwd_moments_kernel<false, true> : vox_bwd_moments_kernel<false, false>);
         km<<<ceil_div(n_isects, kMomThreads), kMomThreads, 0, stream>>>(
             A, isect_ids, v_render, v_render_var, detach ? 0.f : 2.f, nullptr, nullptr, nullptr, nullptr, radii,
@@ -905,8 +1024,8 @@ extern "C" int r2s_voxelize_fused_vjp(const float* means3d, const float* means, 
     return check_launch("voxelize_fused_vjp_reduce");
 }
 
-extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const float* means, const float* ranges,
-                                      const float* centers, const float* conics, const float* opacities,
+extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const double* frame, const float* conics,
+                                      const float* opacities,
                                       const float* betas, const float* uncertainties, const int32_t* tile_offsets,
                                       const int32_t* flatten_ids, uint32_t n_isects, uint32_t N, uint32_t dim_x,
                                       uint32_t dim_y, uint32_t dim_z, uint32_t tile_size, uint32_t n_tiles_x,
@@ -919,13 +1038,10 @@ extern "C" int r2s_voxelize_fused_fwd(const float* means3d, const float* means, 
     const uint64_t n_tiles = uint64_t(n_tiles_x) * n_tiles_y * n_tiles_z;
     R2S_REQUIRE(n_tiles < (1ull << 31), "too many tiles");
     if (n_tiles == 0) return R2S_OK;
-    R2S_REQUIRE((means == nullptr) == (ranges == nullptr) && (means == nullptr) == (centers == nullptr),
-                "means, ranges and centers go together");
+    R2S_REQUIRE(frame != nullptr, "voxelize_fused_fwd: the double-precision frame is required");
     auto A = make_vox_args<float>(means3d, conics, opacities, betas, uncertainties, tile_offsets, flatten_ids,
                                   n_isects, N, dim_x, dim_y, dim_z, tile_size, n_tiles_x, n_tiles_y, n_tiles_z);
-    A.world = means;
-    A.ranges = ranges;
-    A.centers = centers;
+    A.frame = frame;
     auto k = betas ? (var ? vox_fwd_fused_kernel<true, true> : vox_fwd_fused_kernel<true, false>)
                    : (var ? vox_fwd_fused_kernel<false, true> : vox_fwd_fused_kernel<false, false>);
     k<<<uint32_t(n_tiles), kVoxFwdWaves * kWave, 0, stream>>>(A, render, render_var);

@@ -166,24 +166,31 @@ def _cameras(viewmats, Ks, means, height: int, width: int, checks: check.ValueCh
 
 
 def _volume_frame(extent, center, means, checks: check.ValueChecks):
-    """volume_extent (positive) and volume_center: 3-vectors, constant."""
-    out = []
+    """volume_extent (positive) and volume_center: 3-vectors, constant. ->
+    (extent, center) in the primitives' dtype, and both in float64 as given
+    (Python numbers are not rounded to float32 there)."""
+    out, exact = [], []
     for name, value in (("volume_extent", extent), ("volume_center", center)):
         check.constant(name, value)
+        given = value
         if not isinstance(value, Tensor):
             value = torch.as_tensor(value, device=means.device, dtype=means.dtype)
         value = check.tensor(name, value, (3,), means)
         checks.finite(name, value)
         out.append(value)
+        if isinstance(given, Tensor):
+            exact.append(value.double().contiguous())
+        else:
+            exact.append(torch.as_tensor(given, device=means.device, dtype=torch.float64).reshape(3).contiguous())
     extent = out[0]
     checks.add("volume_extent must contain positive physical lengths", lambda: (extent <= 0).any())
-    return out
+    return out[0], out[1], exact[0], exact[1]
 
 
 # ---------------------------------------------------------------------------
 # projection
 # ---------------------------------------------------------------------------
-_STATE_KEEP = ("viewmats", "Ks", "extent", "center")
+_STATE_KEEP = ("viewmats", "Ks", "extent", "center", "extent64", "center64")
 
 
 def _drop_state(buffers, eps, radius_clip):
@@ -343,9 +350,11 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
 # ---------------------------------------------------------------------------
 # voxelization
 # ---------------------------------------------------------------------------
-def _voxel_state(prims, extent, center, shape, eps, radius_clip) -> dict:
-    """Volume mapping and 3D tile binning."""
+def _voxel_state(prims, vol: dict, shape, eps, radius_clip) -> dict:
+    """Volume mapping and 3D tile binning (vol: the volume frame, extent /
+    center in the primitives' dtype and extent64 / center64 in float64)."""
     means, quats, scales, densities, beta, uncertainty = prims
+    extent, center = vol["extent"], vol["center"]
     depth, height, width = shape
     T = VOXEL_TILE
     with stage("voxel_map"):
@@ -356,8 +365,13 @@ def _voxel_state(prims, extent, center, shape, eps, radius_clip) -> dict:
     with stage("voxel_bin"):
         _, isect_ids, flatten_ids, tile_cum = _ops_impl._isect_tiles_3d(mapped, radii, T, tx, ty, tz)
         offsets = ops.isect_offset_encode_3d(isect_ids, tx, ty, tz)
-    return dict(covars=covars, mapped=mapped, conics=conics, radii=radii, tile_cum=tile_cum, offsets=offsets,
-                isect_ids=isect_ids, flatten_ids=flatten_ids, means=means)
+    state = dict(covars=covars, mapped=mapped, conics=conics, radii=radii, tile_cum=tile_cum, offsets=offsets,
+                 isect_ids=isect_ids, flatten_ids=flatten_ids)
+    if _use_fused_voxels(means):
+        with stage("voxel_map"):
+            state["frame"] = fused.voxel_frame(means, quats, scales, vol["extent64"], vol["center64"],
+                                               (width, height, depth), eps)
+    return state
 
 
 def _voxel_render(b, prims, shape):
@@ -410,15 +424,15 @@ def voxelize_fwd(
     means, quats, scales, densities, beta, uncertainty = prims
     eps, radius_clip = check.options(beta, eps, radius_clip, detach_variance)
     mode = _mode(backward_mode)
-    extent, center = _volume_frame(volume_extent, volume_center, means, values)
+    extent, center, extent64, center64 = _volume_frame(volume_extent, volume_center, means, values)
     values.run()
     shape = (depth, height, width)
-    buffers = {"extent": extent, "center": center}
+    buffers = {"extent": extent, "center": center, "extent64": extent64, "center64": center64}
     if means.shape[0] == 0:
         volume = means.new_zeros(shape)
         output = volume if uncertainty is None else (volume, volume.clone())
     else:
-        buffers.update(_voxel_state(prims, extent, center, shape, eps, radius_clip))
+        buffers.update(_voxel_state(prims, buffers, shape, eps, radius_clip))
         output = _voxel_render(buffers, prims, shape)
         if recompute:
             _drop_state(buffers, eps, radius_clip)
@@ -439,7 +453,7 @@ def voxelize_bwd(context: _Context, v_volume: Optional[Tensor], v_variance: Opti
         v_volume = means.new_zeros(context.shape)
     b = context.buffers
     if "recompute" in b:
-        b.update(_voxel_state(context.primitives, b["extent"], b["center"], context.shape, *b.pop("recompute")))
+        b.update(_voxel_state(context.primitives, b, context.shape, *b.pop("recompute")))
     depth, height, width = context.shape
     T = VOXEL_TILE
     prefix = "beta_" if beta is not None else ""

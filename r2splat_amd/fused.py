@@ -215,19 +215,24 @@ def backward(fwd: FusedForward, means, quats, scales, densities, viewmats, Ks, v
     return out
 
 
-def _frame(state: dict):
-    """World means and volume frame of a binning state (None when it has none):
-    the kernels recompute the voxel-frame means from them in double precision."""
-    if "means" in state and "extent" in state and "center" in state:
-        return state["means"], state["extent"], state["center"]
-    return None, None, None
+def voxel_frame(means, quats, scales, extent, center, dims, eps: float) -> torch.Tensor:
+    """Per-primitive voxel-frame mean and inverse covariance in double
+    precision ([N, 9] float64: mean x y z, conic xx xy xz yy yz zz), from
+    float32 parameters and the volume frame (extent [3], center [3], float64)."""
+    N = means.shape[0]
+    W, H, D = dims
+    frame = torch.empty((N, 9), dtype=torch.float64, device=means.device)
+    with torch.cuda.device(means.device):
+        call("r2s_voxel_frame", ptr(means), ptr(quats), ptr(scales), ptr(extent), ptr(center), W, H, D, float(eps),
+             N, ptr(frame), be.stream_of(means.device))
+    return frame
 
 
 def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8, *, uncertainty=None,
               v_variance=None, detach_variance: bool = False):
     """Per-primitive voxelize VJP (float32, 8^3 tiles) from a voxel binning
     state (mapped, conics, radii, tile_cum, isect_ids, flatten_ids, and the
-    world means with the volume's extent and center): one partial
+    double-precision frame of voxel_frame): one partial
     per intersection, summed per primitive in a fixed order (no atomics).
     With a variance adjoint (``v_variance``, and the forward's ``uncertainty``)
     it also returns v_uncertainty.
@@ -246,8 +251,7 @@ def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8, 
     v_beta = torch.empty((N,), dtype=torch.float32, device=device) if beta is not None else None
     v_unc = torch.empty((N,), dtype=torch.float32, device=device) if v_variance is not None else None
     with torch.cuda.device(device):
-        world, extent, center = _frame(state)
-        call("r2s_voxelize_fused_vjp", ptr(mapped), ptr(world), ptr(extent), ptr(center), ptr(state["conics"]),
+        call("r2s_voxelize_fused_vjp", ptr(mapped), ptr(state["frame"]), ptr(state["conics"]),
              ptr(densities), ptr(beta),
              ptr(uncertainty if v_variance is not None else None), ptr(state["radii"]), ptr(state["tile_cum"]),
              ptr(ids), ptr(flat), n_isects, N, W, H, D, T, -(-W // T), -(-H // T), -(-D // T), ptr(v_volume),
@@ -262,8 +266,9 @@ def voxel_vjp(state: dict, densities, beta, v_volume, dims, tile_size: int = 8, 
 
 
 def voxel_render(state: dict, densities, beta, dims, tile_size: int = 8, *, uncertainty=None):
-    """Volume [D, H, W] of a voxel binning state (float32, 8^3 tiles), or
-    (volume, variance) with ``uncertainty`` [N]."""
+    """Volume [D, H, W] of a voxel binning state (float32, 8^3 tiles, with the
+    double-precision frame of voxel_frame), or (volume, variance) with
+    ``uncertainty`` [N]."""
     mapped = state["mapped"]
     device = mapped.device
     W, H, D = dims
@@ -271,8 +276,7 @@ def voxel_render(state: dict, densities, beta, dims, tile_size: int = 8, *, unce
     out = torch.empty((D, H, W), dtype=torch.float32, device=device)
     var = torch.empty_like(out) if uncertainty is not None else None
     with torch.cuda.device(device):
-        world, extent, center = _frame(state)
-        call("r2s_voxelize_fused_fwd", ptr(mapped), ptr(world), ptr(extent), ptr(center), ptr(state["conics"]),
+        call("r2s_voxelize_fused_fwd", ptr(mapped), ptr(state["frame"]), ptr(state["conics"]),
              ptr(densities), ptr(beta),
              ptr(uncertainty), ptr(state["offsets"]), ptr(state["flatten_ids"]), state["flatten_ids"].numel(),
              mapped.shape[0], W, H, D, T, -(-W // T), -(-H // T), -(-D // T), ptr(out), ptr(var),

@@ -68,9 +68,10 @@ SIGNATURES = {
     "r2s_fused_param_workspace_bytes": [U32, U32],
     "r2s_fused_param_vjp": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, F, P, P, U32, P, P, SZ, P, P, P, P, P, P,
                             P],
-    "r2s_voxelize_fused_fwd": [P, P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, P],
+    "r2s_voxel_frame": [P, P, P, P, P, U32, U32, U32, D, U32, P, P],
+    "r2s_voxelize_fused_fwd": [P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P, P, P],
     "r2s_voxel_partials_bytes": [U32],
-    "r2s_voxelize_fused_vjp": [P, P, P, P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P,
+    "r2s_voxelize_fused_vjp": [P, P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, U32, U32, U32, U32, P,
                                P, I32, P, SZ, P, P, P, P, P, P],
     "r2s_mse_partials": [U64],
     "r2s_mse_adjoint": [P, P, U64, P, P, P, P],

@@ -1,6 +1,6 @@
 """Float64 evaluation of the fused float32 path's own (list entry, pixel) pairs.
 
-Test helper (not collected). The fused float32 rasterizer keeps a pair when
+A precision audit of the fused float32 rasterizer. It keeps a pair when
 its float32 alpha passes the 1e-5 cut-off; a float64 evaluation of the same
 scene can decide a pair at the threshold the other way (each such term is
 about 1e-5). To hold the float32 arithmetic to the contract's bounds on its
@@ -12,7 +12,9 @@ path's tile lists:
   gradients), with float64 projections (the staged operator path in float64:
   means2d, conics, mu, opacity);
 * ``ref``: the float64 path's own decisions (sigma >= 0 and |alpha| >= 1e-5,
-  Beta 0 <= d^2 < 1), i.e. the float64 reference on the same tile lists.
+  Beta 0 <= d^2 < 1). Over the float64 path's own tile lists (``fp64_table``)
+  this is the float64 reference itself; the two paths' lists can differ where
+  a footprint radius rounds across an integer (``coverage_diff``).
 
 Per-pair VJPs follow the reference's formulas (RasterizeToPixelsBwd.cu:
 Gaussian sigma = 1/2 (a dx^2 + c dy^2) + b dx dy with dx = pixel - mean; Beta
@@ -20,8 +22,9 @@ d^2 = delta^T P delta with delta = mean - pixel and alpha = w (1 - d^2)^b2d;
 variance sum q alpha^2, whose adjoint reaches alpha unless detached) and are
 chained through the float64 conic/mu, projection and parameter-transform VJP
 operators exactly as the explicit API chains them. ``ref`` reproduces the
-float64 operator path's image and gradients (checked by the tests), which pins
-this evaluation itself.
+float64 operator path's image and gradients (checked by the GPU tests), which
+pins this evaluation itself. Used by the float64 parity tests and by the
+benchmark's parity report.
 """
 
 from __future__ import annotations
@@ -31,11 +34,11 @@ from typing import Optional
 
 import torch
 
-from r2splat_amd import explicit
-from r2splat_amd import hip as ops
-from r2splat_amd._beta import beta_path_log_grad
-from r2splat_amd.hip import _backend as be
-from r2splat_amd.hip._backend import ptr
+from . import explicit
+from . import hip as ops
+from ._beta import beta_path_log_grad
+from .hip import _backend as be
+from .hip._backend import ptr
 
 MODES = ("kept", "ref")
 
@@ -49,8 +52,8 @@ class PairTable:
     prim: torch.Tensor
     ty: torch.Tensor
     tx: torch.Tensor
-    mask_fwd: torch.Tensor  # int32 [n, T*T/32] bit words
-    mask_bwd: torch.Tensor
+    mask_fwd: Optional[torch.Tensor]  # int32 [n, T*T/32] bit words (None: a table with no float32 decisions)
+    mask_bwd: Optional[torch.Tensor]
     tile: int
 
 
@@ -77,6 +80,27 @@ def pair_table(fwd, C: int, N: int) -> PairTable:
     t = tile % (ntx * nty)
     assert bool((tile // (ntx * nty) == cam).all()), "an entry sits in another camera's list"
     return PairTable(cam, prim, t // ntx, t % ntx, mask_f[:n], mask_b[:n], T)
+
+
+def list_table(buffers: dict, C: int, N: int, width: int, height: int, tile: int) -> PairTable:
+    """Entries of the staged operator path's tile lists (isect_ids = camera *
+    n_tiles + ty * n_tiles_x + tx, flatten_ids = camera * N + primitive), with
+    no keep masks: a table for mode 'ref' on a path's own coverage."""
+    ids = buffers["isect_ids"].long()
+    flat = buffers["flatten_ids"].long()
+    ntx = -(-width // tile)
+    nty = -(-height // tile)
+    cam = ids // (ntx * nty)
+    t = ids % (ntx * nty)
+    assert bool((cam == flat // N).all()), "an entry's key and primitive name different cameras"
+    return PairTable(cam, flat % N, t // ntx, t % ntx, None, None, tile)
+
+
+def coverage_diff(a: PairTable, b: PairTable, C: int, N: int) -> int:
+    """(camera, primitive) records whose tile counts differ between two tables."""
+    ca = torch.bincount(a.cam * N + a.prim, minlength=C * N)
+    cb = torch.bincount(b.cam * N + b.prim, minlength=C * N)
+    return int((ca != cb).sum())
 
 
 def _bits(words: torch.Tensor, T: int) -> torch.Tensor:
@@ -113,6 +137,12 @@ def fp64_scene(means, quats, scales, densities, viewmats, Ks, image_size, *, bet
                      (viewmats.shape[0],) + tuple(image_size), eps)
 
 
+def fp64_table(sc: "Fp64Scene") -> PairTable:
+    """The float64 operator path's own tile lists as a table (mode 'ref')."""
+    C, H, W = sc.shape
+    return list_table(sc.buffers, C, sc.means.shape[0], W, H, explicit.RASTER_TILE)
+
+
 @dataclass
 class PairResult:
     image: torch.Tensor
@@ -130,6 +160,7 @@ def evaluate(table: PairTable, sc: Fp64Scene, mode: str, v_image=None, v_varianc
     table's pairs, kept by the float32 masks (mode 'kept') or by the float64
     decisions (mode 'ref')."""
     assert mode in MODES
+    assert mode == "ref" or table.mask_fwd is not None, "mode 'kept' needs a table with keep masks"
     b = sc.buffers
     C, H, W = sc.shape
     T = table.tile
@@ -181,13 +212,15 @@ def evaluate(table: PairTable, sc: Fp64Scene, mode: str, v_image=None, v_varianc
             alpha = w * e
             keep64 = (sigma >= 0) & (alpha.abs() >= 1e-5)
         keep64 &= inimg
-        kf = _bits(table.mask_fwd[lo:hi], T) & inimg
-        kb = _bits(table.mask_bwd[lo:hi], T) & inimg
-        flips_f += int((kf != keep64).sum())
-        flips_b += int((kb != keep64).sum())
         pairs += int(inimg.sum())
-        keep_f = kf if mode == "kept" else keep64
-        keep_b = kb if mode == "kept" else keep64
+        keep_f = keep_b = keep64
+        if table.mask_fwd is not None:
+            kf = _bits(table.mask_fwd[lo:hi], T) & inimg
+            kb = _bits(table.mask_bwd[lo:hi], T) & inimg
+            flips_f += int((kf != keep64).sum())
+            flips_b += int((kb != keep64).sum())
+            if mode == "kept":
+                keep_f, keep_b = kf, kb
         af = torch.where(keep_f, alpha, torch.zeros_like(alpha))
         image.index_add_(0, pix.reshape(-1), af.reshape(-1))
         if var_on:

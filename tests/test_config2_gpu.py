@@ -5,9 +5,10 @@ properties that hold at any size: the mean of the (mean, variance) path
 equals the mean-only path; the variance is non-negative and exactly linear in
 the uncertainty; with detach_variance the mean-parameter gradients are those
 of the mean alone and the uncertainty gradient is unchanged; the backward is
-linear in the adjoints; the fused and staged backward agree (mean-only and
-(mean, variance)) and match central differences; voxel mean/variance
-likewise; TV loss and gradient against the FP64 oracle on the full volume."""
+linear in the adjoints and matches central differences; voxel mean/variance
+likewise; TV loss and gradient against the FP64 oracle on the full volume.
+(Against float64, image / variance / gradients and the 256^3 volume are
+held to fixed bounds in test_parity_fp64_gpu.py.)"""
 
 import pytest
 import torch
@@ -53,15 +54,6 @@ def test_config2_mean_and_variance(scene):
     assert float(mean.abs().max()) > 0 and float(var.max()) > 0
 
 
-def _ops(fn):
-    old = explicit.BACKEND
-    explicit.BACKEND = "ops"
-    try:
-        return fn()
-    finally:
-        explicit.BACKEND = old
-
-
 def test_config2_backward_detach_and_linearity(scene, adjoints):
     """Fused (mean, variance) path: detach_variance semantics and linearity."""
     v, vv = adjoints
@@ -88,29 +80,14 @@ def test_config2_backward_detach_and_linearity(scene, adjoints):
         assert err <= 1e-5, f"linearity {name}: {err:.2e}"
 
 
-def test_config2_fused_backward_matches_ops_and_finite_differences(scene, adjoints):
-    """The fused Beta backward against the staged operator path over all
-    274M intersections, and both against central differences of the
-    (deterministic) forward on the primitives with the largest gradients.
-    Max-norm gaps on positions and shapes stay loose: at these scales a few
-    primitives sit entirely at the 1e-5 cut-off, where two float32 evaluations
-    keep different pixels (the operator path differs from its own float64
-    evaluation by up to 1e-2 there)."""
+def test_config2_fused_backward_matches_finite_differences(scene, adjoints):
+    """The fused Beta backward over all 274M intersections against central
+    differences of the (deterministic) forward on the primitives with the
+    largest density and beta gradients. (Against float64 the fused path is held
+    to 1e-5 / 1e-4 on its kept pairs in test_parity_fp64_gpu.py.)"""
     v, _ = adjoints
     _, ctx = _fwd(scene, None)
     fz = r2splat_amd.rasterize_bwd(ctx, v)
-
-    def ops_bwd():
-        _, c = _fwd(scene, None)
-        return r2splat_amd.rasterize_bwd(c, v)
-
-    op = _ops(ops_bwd)
-    for name in FIELDS:
-        a, b = getattr(fz, name).double(), getattr(op, name).double()
-        l2 = float((a - b).norm() / b.norm())
-        assert l2 <= 1e-3, f"{name}: relative L2 {l2:.2e}"
-    assert O.relative_error(fz.densities, op.densities) <= 1e-3
-    assert O.relative_error(fz.beta, op.beta) <= 1e-4
     vd = v.double()
 
     def loss(**kw):
@@ -129,7 +106,6 @@ def test_config2_fused_backward_matches_ops_and_finite_differences(scene, adjoin
             minus[k] -= h
             fd = (loss(**{name: plus}) - loss(**{name: minus})) / (2 * h)
             assert abs(fd - float(grad[k])) <= 2e-3 * abs(float(grad[k])), (name, k, fd, float(grad[k]))
-            assert abs(float(getattr(op, name)[k]) - float(grad[k])) <= 1e-4 * abs(float(grad[k]))
 
 
 def test_config2_voxels_and_isotropic_tv(scene, device):
@@ -137,10 +113,8 @@ def test_config2_voxels_and_isotropic_tv(scene, device):
     kw = dict(beta=scene.beta, volume_extent=(1.0, 1.0, 1.0))
     (vmean, vvar), _ = r2splat_amd.voxelize_fwd(*scene.params(), shape, uncertainty=scene.uncertainty, **kw)
     mean_only, _ = r2splat_amd.voxelize_fwd(*scene.params(), shape, **kw)
-    # the mean-only volume runs the fused kernel, the (mean, variance) one the staged operator:
-    # same function, different float32 rounding (each is 3e-4 from a float64 evaluation here)
-    assert O.relative_error(vmean, mean_only) <= 1e-4
-    assert float((vmean.double() - mean_only.double()).norm() / mean_only.double().norm()) <= 1e-6
+    # the same fused kernels with and without the variance (alpha added, not fused into a multiply-add)
+    assert O.relative_error(vmean, mean_only) <= 1e-6
     assert float(vvar.min()) >= 0.0
     (_, vvar2), _ = r2splat_amd.voxelize_fwd(*scene.params(), shape, uncertainty=(2.0 * scene.uncertainty).contiguous(),
                                              **kw)
@@ -157,26 +131,3 @@ def test_config2_voxels_and_isotropic_tv(scene, device):
     assert O.relative_error(g, g_ref) <= 1e-4
     aniso, _, _ = ops.volume_loss_fwd(vol, 1e-3)
     assert abs(aniso.item() - 1e-3 * O.volume_tv(ref.detach()).item()) <= 5e-5 * aniso.item()
-
-
-def test_config2_fused_variance_backward_matches_ops(scene, adjoints):
-    """The fused (mean, variance) backward against the staged operator path
-    over all 274M intersections (relative L2; max-norm on the per-primitive
-    sums that cut-off flips barely move)."""
-    v, vv = adjoints
-    _, ctx = _fwd(scene, scene.uncertainty)
-    assert "fused" in ctx.buffers
-    fz = r2splat_amd.rasterize_bwd(ctx, v, vv)
-
-    def ops_bwd():
-        _, c = _fwd(scene, scene.uncertainty)
-        return r2splat_amd.rasterize_bwd(c, v, vv)
-
-    op = _ops(ops_bwd)
-    for name in FIELDS + ("uncertainty",):
-        a, b = getattr(fz, name).double(), getattr(op, name).double()
-        l2 = float((a - b).norm() / b.norm())
-        assert l2 <= 1e-3, f"{name}: relative L2 {l2:.2e}"
-    for name in ("densities", "uncertainty"):
-        assert O.relative_error(getattr(fz, name), getattr(op, name)) <= 1e-3, name
-    assert O.relative_error(fz.beta, op.beta) <= 1e-4

@@ -11,6 +11,8 @@ from oracle import splat as O
 from r2splat_amd import explicit, fused, workloads
 from r2splat_amd import hip as ops
 
+from r2splat_amd.parity import evaluate, fp64_scene, pair_table
+
 pytestmark = pytest.mark.gpu
 
 
@@ -310,22 +312,45 @@ def _run_beta(sc, backend, v_fn=None):
 
 
 @pytest.mark.parametrize("n,views,side,seed", [(1000, 4, 64, 0), (20_000, 8, 128, 3)])
-def test_fused_beta_matches_staged_ops_path(device, n, views, side, seed):
-    """The explicit API's Beta path through the fused pipeline agrees with the
-    staged operator path (each operator pinned to the oracle separately)."""
+def test_fused_beta_matches_float64_and_staged_ops_path(device, n, views, side, seed):
+    """The explicit API's Beta path through the fused pipeline against float64
+    over the pairs its kernels kept (r2splat_amd.parity): image within 1e-5,
+    every gradient within 1e-4; bitwise reproducible backward. The staged
+    float32 operators pass float32 means2d and conics between them (as the
+    reference's operators do); the Beta profile's slope at its support edge
+    (w b2d) carries that rounding into the image and gradients (measured up to
+    1.5e-5 and 9e-5 at these sizes), so they are held to 3e-5 of the fused
+    image and 2e-4 of its gradients."""
     sc = workloads.scene(n, views, side, side, scale_range=(0.005, 0.08), seed=seed, device=device,
                          beta_range=(0.5, 4.0))
-    a_img, a, _, ctx = _run_beta(sc, "fused")
-    b_img, b, _, _ = _run_beta(sc, "ops")
-    assert O.relative_error(a_img, b_img) <= 2e-6
-    for name in ("means", "quats", "scales", "densities", "beta"):
-        err = O.relative_error(getattr(a, name), getattr(b, name))
-        assert err <= 2e-5, f"{name}: {err:.2e}"
-    # bitwise reproducible backward
+    names = ("means", "quats", "scales", "densities", "beta")
+    C, N = sc.viewmats.shape[0], sc.means.shape[0]
     img, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
-    a2 = r2splat_amd.rasterize_bwd(ctx, (2.0 * (img - 0.8 * img) / img.numel()).contiguous())
-    assert torch.equal(img, a_img)
-    for name in ("means", "quats", "scales", "densities", "beta"):
+    assert "fused" in ctx.buffers
+    table = pair_table(ctx.buffers["fused"], C, N)
+    v = (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
+    a = r2splat_amd.rasterize_bwd(ctx, v)
+    sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
+    kept = evaluate(table, sc64, "kept", v_image=v)
+    err = O.relative_error(img, kept.image)
+    print(f"Beta {n} x {views} x {side}^2: image vs float64 on kept pairs {err:.2e}", flush=True)
+    assert err <= 1e-5
+    for name in names:
+        err = O.relative_error(getattr(a, name), kept.grads[name])
+        print(f"  {name}: {err:.2e}", flush=True)
+        assert err <= 1e-4, f"{name}: {err:.2e}"
+    b_img, b, _, _ = _run_beta(sc, "ops", lambda _: v)
+    err = O.relative_error(b_img, img)
+    print(f"  staged float32 operators vs fused: image {err:.2e}", flush=True)
+    assert err <= 3e-5
+    for name in names:
+        err = O.relative_error(getattr(b, name), getattr(a, name))
+        print(f"  staged {name}: {err:.2e}", flush=True)
+        assert err <= 2e-4, f"staged {name}: {err:.2e}"
+    img2, ctx = r2splat_amd.rasterize_fwd(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta)
+    a2 = r2splat_amd.rasterize_bwd(ctx, v)
+    assert torch.equal(img2, img)
+    for name in names:
         assert torch.equal(getattr(a2, name), getattr(a, name)), name
 
 
@@ -351,41 +376,59 @@ def test_fused_beta_large_exponent_centre_pixel(device):
 @pytest.mark.parametrize("use_beta,uncertainty,detach", [(False, False, False), (True, False, False),
                                                          (False, True, False), (True, True, False),
                                                          (True, True, True)])
-def test_fused_voxel_vjp_matches_staged_path_and_is_reproducible(device, use_beta, uncertainty, detach):
+def test_fused_voxel_vjp_matches_float64_and_is_reproducible(device, use_beta, uncertainty, detach):
     """float32 fused voxelize forward and backward (no atomics: one partial per
     intersection, summed per primitive in a fixed order), mean or (mean,
-    variance) with detach_variance either way, against the staged operators,
-    on a volume whose edges cut tiles; bitwise reproducible."""
+    variance) with detach_variance either way, on a volume whose edges cut
+    tiles: volume (and variance) within 1e-5 and every gradient within 1e-4 of
+    the float64 operator path; bitwise reproducible. The staged float32
+    operators (which pass float32 conics between them and form d^2 in float32,
+    as the reference's operators do) agree with the fused kernels to 1e-5 /
+    1e-4 for Gaussians. For Beta primitives with beta < 1 the slope of
+    (1 - d^2)^beta at the support edge (of h^(beta - 1) in the VJP) turns that
+    rounding into 2e-5 on the volume and up to ~2e-2 on the gradients: their
+    volume is held to 1e-4 of float64 and their gradient errors are printed."""
     sc = workloads.scene(20000, 1, 8, 8, scale_range=(0.005, 0.05), seed=2, device=device,
                          beta_range=(0.5, 4.0) if use_beta else None, uncertainty_max=0.1 if uncertainty else None)
     shape = (90, 77, 64)
-    kw = dict(volume_extent=(1.0, 1.1, 0.9), beta=sc.beta, uncertainty=sc.uncertainty, detach_variance=detach)
     g = torch.Generator(device=device).manual_seed(3)
     v = torch.randn(shape, generator=g, device=device)
     vv = torch.randn(shape, generator=g, device=device) if uncertainty else None
 
-    def run(backend):
+    def run(backend, dtype=torch.float32):
+        cast = lambda t: None if t is None else t.to(dtype)  # noqa: E731
         old = explicit.BACKEND
         explicit.BACKEND = backend
         try:
-            out, ctx = r2splat_amd.voxelize_fwd(*sc.params(), shape, **kw)
-            return out, r2splat_amd.voxelize_bwd(ctx, v, vv)
+            out, ctx = r2splat_amd.voxelize_fwd(*(cast(t) for t in sc.params()), shape,
+                                                volume_extent=(1.0, 1.1, 0.9), beta=cast(sc.beta),
+                                                uncertainty=cast(sc.uncertainty), detach_variance=detach)
+            return out, r2splat_amd.voxelize_bwd(ctx, cast(v), cast(vv))
         finally:
             explicit.BACKEND = old
 
-    out_a, a = run("fused")
-    out_b, b = run("ops")
-    for x, y in zip(out_a if uncertainty else (out_a,), out_b if uncertainty else (out_b,)):
-        assert O.relative_error(x, y) <= 1e-5
+    def outputs(out):
+        return out if uncertainty else (out,)
+
     names = ("means", "quats", "scales", "densities") + (("beta",) if use_beta else ()) + \
         (("uncertainty",) if uncertainty else ())
+    out_a, a = run("fused")
+    out_64, g64 = run("fused", torch.float64)
+    for x, y in zip(outputs(out_a), outputs(out_64)):
+        assert O.relative_error(x, y) <= 1e-5
     for name in names:
-        x, y = getattr(a, name).double(), getattr(b, name).double()
-        l2 = float((x - y).norm() / y.norm())
-        assert l2 <= 1e-5, f"{name}: relative L2 {l2:.2e}"
-        assert O.relative_error(x, y) <= 1e-4, f"{name}: {O.relative_error(x, y):.2e}"
+        err = O.relative_error(getattr(a, name), getattr(g64, name))
+        assert err <= 1e-4, f"{name}: {err:.2e}"
+    out_b, b = run("ops")
+    ref_out, ref_g = (outputs(out_64), g64) if use_beta else (outputs(out_a), a)
+    for x, y in zip(outputs(out_b), ref_out):
+        assert O.relative_error(x, y) <= (1e-4 if use_beta else 1e-5)
+    for name in names:
+        err = O.relative_error(getattr(b, name), getattr(ref_g, name))
+        print(f"staged float32 operators, {name}: {err:.2e} from {'float64' if use_beta else 'fused'}", flush=True)
+        assert use_beta or err <= 1e-4, f"staged {name}: {err:.2e}"
     out_a2, a2 = run("fused")
-    for x, y in zip(out_a if uncertainty else (out_a,), out_a2 if uncertainty else (out_a2,)):
+    for x, y in zip(outputs(out_a), outputs(out_a2)):
         assert torch.equal(x, y)
     for name in names:
         assert torch.equal(getattr(a, name), getattr(a2, name)), name
@@ -448,56 +491,43 @@ def test_fused_variance_matches_oracle(device, use_beta, detach, tile, waves):
 
 
 @pytest.mark.parametrize("use_beta,detach", [(False, False), (True, False), (True, True)])
-def test_fused_variance_matches_staged_ops_path(device, use_beta, detach):
-    """The explicit API's (mean, variance) path through the fused pipeline
-    against the staged operators evaluated in float64 on the same inputs
-    (20k primitives, 8 views of 128^2; each operator pinned to the oracle
-    separately). At this size the float32 staged path is itself up to 1e-4
-    (max-norm) from that reference on quaternion gradients, the fused path
-    3e-5. The fused backward is bitwise reproducible and linear in the
-    adjoints."""
+def test_fused_variance_matches_float64_on_kept_pairs(device, use_beta, detach):
+    """The explicit API's (mean, variance) path through the fused pipeline (20k
+    primitives, 8 views of 128^2) against float64 over the pairs the float32
+    kernels kept (r2splat_amd.parity; float64 projections from the staged
+    operators in float64, each pinned to the oracle separately): mean and
+    variance within 1e-5, every gradient including the uncertainty's within
+    1e-4 (max |a - b| / max |b|). The fused backward is bitwise reproducible
+    and linear in the adjoints."""
     sc = workloads.scene(20_000, 8, 128, 128, scale_range=(0.005, 0.08), seed=3, device=device,
                          beta_range=(0.5, 4.0) if use_beta else None, uncertainty_max=0.1)
     names = ("means", "quats", "scales", "densities") + (("beta",) if use_beta else ()) + ("uncertainty",)
-    f64 = {k: (None if t is None else t.double()) for k, t in
-           dict(zip(("means", "quats", "scales", "densities"), sc.params()), beta=sc.beta,
-                uncertainty=sc.uncertainty, viewmats=sc.viewmats, Ks=sc.Ks).items()}
-
-    def run(backend, p, scale=1.0):
-        old = explicit.BACKEND
-        explicit.BACKEND = backend
-        try:
-            (img, var), ctx = r2splat_amd.rasterize_fwd(p["means"], p["quats"], p["scales"], p["densities"],
-                                                        p["viewmats"], p["Ks"], sc.image_size, beta=p["beta"],
-                                                        uncertainty=p["uncertainty"], detach_variance=detach)
-            assert ("fused" in ctx.buffers) == (backend == "fused")
-            v, vv = (t.to(img.dtype) for t in _adjoint_pair(img.float(), 13))
-            return img, var, r2splat_amd.rasterize_bwd(ctx, (scale * v).contiguous(), (scale * vv).contiguous())
-        finally:
-            explicit.BACKEND = old
-
+    C, N = sc.viewmats.shape[0], sc.means.shape[0]
     p32 = dict(zip(("means", "quats", "scales", "densities"), sc.params()), beta=sc.beta,
                uncertainty=sc.uncertainty, viewmats=sc.viewmats, Ks=sc.Ks)
-    a_img, a_var, a = run("fused", p32)
-    b_img, b_var, b = run("ops", f64)
-    s_img, s_var, s = run("ops", p32)
 
-    def l2(x, y):
-        return float((x.double() - y).norm() / y.norm())
+    def run(p, scale=1.0, keep_table=False):
+        (img, var), ctx = r2splat_amd.rasterize_fwd(p["means"], p["quats"], p["scales"], p["densities"],
+                                                    p["viewmats"], p["Ks"], sc.image_size, beta=p["beta"],
+                                                    uncertainty=p["uncertainty"], detach_variance=detach)
+        assert "fused" in ctx.buffers
+        table = pair_table(ctx.buffers["fused"], C, N) if keep_table else None
+        v, vv = _adjoint_pair(img, 13)
+        return img, var, r2splat_amd.rasterize_bwd(ctx, (scale * v).contiguous(), (scale * vv).contiguous()), \
+            table, (v, vv)
 
-    # no worse than the float32 staged path against the same reference (floors: the
-    # tolerances of the oracle parity tests)
-    for x, sx, y, floor in ((a_img, s_img, b_img, 1e-5), (a_var, s_var, b_var, 1e-5)):
-        assert O.relative_error(x, y) <= max(floor, 1.5 * O.relative_error(sx, y))
-        assert l2(x, y) <= max(1e-6, 1.5 * l2(sx, y))
+    a_img, a_var, a, table, (v, vv) = run(p32, keep_table=True)
+    sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta, uncertainty=sc.uncertainty,
+                      detach=detach)
+    kept = evaluate(table, sc64, "kept", v_image=v, v_variance=vv, detach=detach)
+    assert O.relative_error(a_img, kept.image) <= 1e-5
+    assert O.relative_error(a_var, kept.variance) <= 1e-5
     for name in names:
-        x, sx, y = getattr(a, name), getattr(s, name), getattr(b, name)
-        err, ref_err = O.relative_error(x, y), O.relative_error(sx, y)
-        assert err <= max(1e-4, 1.5 * ref_err), f"{name}: {err:.2e} (float32 staged {ref_err:.2e})"
-        assert l2(x, y) <= max(1e-5, 1.5 * l2(sx, y)), f"{name}: relative L2 {l2(x, y):.2e}"
-    c_img, c_var, c = run("fused", p32)
+        err = O.relative_error(getattr(a, name), kept.grads[name])
+        assert err <= 1e-4, f"{name}: {err:.2e}"
+    c_img, c_var, c, _, _ = run(p32)
     assert torch.equal(c_img, a_img) and torch.equal(c_var, a_var)
-    _, _, d = run("fused", p32, 2.0)
+    _, _, d, _, _ = run(p32, 2.0)
     for name in names:
         assert torch.equal(getattr(c, name), getattr(a, name)), name
         assert torch.equal(getattr(d, name), 2.0 * getattr(a, name)), name

@@ -1,14 +1,16 @@
 """Float32 fused path against float64 at working size, with cut-off flips
 separated from arithmetic error.
 
-For every (list entry, pixel) pair of the fused path's tile lists the float64
-side is evaluated twice (tests/_fp64_pairs.py): over the pairs the float32
-kernels kept (their keep masks), and with its own float64 decisions. The
-first comparison holds the float32 arithmetic to the contract's fixed bounds
-(max |a - b| / max |b|: 1e-5 for images, 1e-4 for every gradient); the
-second is the float64 reference itself (it reproduces the float64 operator
-path, which the oracle pins at config 0), and the gap between the two is the
-contribution of pairs at the 1e-5 threshold, reported with their count.
+The float64 side is evaluated pair by pair (r2splat_amd.parity) twice: over
+the (list entry, pixel) pairs the float32 kernels kept (their keep masks, on
+the fused path's tile lists), and over the float64 path's own tile lists with
+its own float64 decisions. The first comparison holds the float32 arithmetic
+to the contract's fixed bounds (max |a - b| / max |b|: 1e-5 for images, 1e-4
+for every gradient); the second reproduces the float64 operator path (which
+the oracle pins at config 0), and the gap between the two is the contribution
+of pairs decided differently at the 1e-5 threshold or at the support edge, and
+of footprints whose radius rounds across an integer (a tile more or less),
+reported with their counts.
 
 Config 1: 100,000 Gaussians x 64 views of 256^2, MSE adjoint against a second
 primitive set (its target images), through the explicit API.
@@ -21,14 +23,16 @@ import r2splat_amd
 from oracle import splat as O
 from r2splat_amd import workloads
 
-from _fp64_pairs import evaluate, fp64_scene, pair_table
+from r2splat_amd.parity import coverage_diff, evaluate, fp64_scene, fp64_table, pair_table
 
 pytestmark = pytest.mark.gpu
 FIELDS = ("means", "quats", "scales", "densities")
 
 
-def _counts(C, N, table):
-    return torch.bincount(table.cam * N + table.prim, minlength=C * N)
+# (camera, primitive) records whose float32 and float64 tile counts may differ:
+# a footprint radius ceil(3 sqrt(lambda_max)) lands on the other side of an
+# integer about once per 10^6 records at these scales.
+COVERAGE_SHARE = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -43,23 +47,23 @@ def config1(device):
     v = (2.0 * (image - target) / image.numel()).contiguous()
     grads = r2splat_amd.rasterize_bwd(ctx, v)
     sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size)
-    # the float64 path's own tile lists hold the same (camera, primitive) pairs as the float32 lists
-    counts64 = torch.bincount(sc64.buffers["flatten_ids"].long(), minlength=C * N)
-    coverage_diff = int((counts64 != _counts(C, N, table)).sum())
+    table64 = fp64_table(sc64)
     g64 = r2splat_amd.rasterize_bwd(sc64.ctx, v.double())
     kept = evaluate(table, sc64, "kept", v_image=v)
-    ref = evaluate(table, sc64, "ref", v_image=v)
+    ref = evaluate(table64, sc64, "ref", v_image=v)
     return dict(C=C, N=N, image=image, grads=grads, sc64=sc64, g64=g64, kept=kept, ref=ref, table=table,
-                coverage_diff=coverage_diff)
+                coverage_diff=coverage_diff(table, table64, C, N))
 
 
 def test_config1_fp64_pair_evaluation_reproduces_the_float64_operator_path(config1):
-    """The float64 pair evaluation with float64 decisions equals the float64
-    operator path (rasterize_fwd / rasterize_bwd on float64 inputs)."""
+    """The float64 pair evaluation over the float64 path's own lists with
+    float64 decisions equals the float64 operator path (rasterize_fwd /
+    rasterize_bwd on float64 inputs)."""
     ref, sc64, g64 = config1["ref"], config1["sc64"], config1["g64"]
+    C, N = config1["C"], config1["N"]
     print(f"config1 (camera, primitive) records whose float32 and float64 tile counts differ: "
-          f"{config1['coverage_diff']}", flush=True)
-    assert config1["coverage_diff"] == 0
+          f"{config1['coverage_diff']} of {C * N}", flush=True)
+    assert config1["coverage_diff"] <= COVERAGE_SHARE * C * N
     err = O.relative_error(ref.image, sc64.image)
     print(f"config1 fp64 pair evaluation vs fp64 operators: image {err:.2e}", flush=True)
     assert err <= 1e-12
@@ -75,8 +79,8 @@ def test_config1_float32_image_within_1e5_of_float64_on_its_kept_pairs(config1):
     flips = O.relative_error(kept.image, ref.image)
     total = O.relative_error(image, ref.image)
     print(f"config1 image: float32 vs float64 on kept pairs {arith:.2e}; cut-off flips {kept.flips_fwd} of "
-          f"{kept.pairs} pairs, their contribution {flips:.2e}; float32 vs float64 reference {total:.2e}",
-          flush=True)
+          f"{kept.pairs} pairs and {config1['coverage_diff']} coverage differences, their contribution "
+          f"{flips:.2e}; float32 vs float64 reference {total:.2e}", flush=True)
     assert arith <= 1e-5
 
 
@@ -122,11 +126,12 @@ def test_config2_beta_mean_variance_vs_float64_on_kept_pairs(config2_views, deta
     grads = r2splat_amd.rasterize_bwd(ctx, v, vv)
     sc64 = fp64_scene(*sc.params(), sc.viewmats, sc.Ks, sc.image_size, beta=sc.beta, uncertainty=sc.uncertainty,
                       detach=detach)
-    counts64 = torch.bincount(sc64.buffers["flatten_ids"].long(), minlength=C * N)
-    assert int((counts64 != _counts(C, N, table)).sum()) == 0
+    table64 = fp64_table(sc64)
+    cov = coverage_diff(table, table64, C, N)
+    assert cov <= COVERAGE_SHARE * C * N
     g64 = r2splat_amd.rasterize_bwd(sc64.ctx, v.double(), vv.double())
     kept = evaluate(table, sc64, "kept", v_image=v, v_variance=vv, detach=detach)
-    ref = evaluate(table, sc64, "ref", v_image=v, v_variance=vv, detach=detach)
+    ref = evaluate(table64, sc64, "ref", v_image=v, v_variance=vv, detach=detach)
     mean64, var64 = sc64.image
     # the pair evaluation with float64 decisions is the float64 operator path
     assert O.relative_error(ref.image, mean64) <= 1e-12 and O.relative_error(ref.variance, var64) <= 1e-12
@@ -135,7 +140,8 @@ def test_config2_beta_mean_variance_vs_float64_on_kept_pairs(config2_views, deta
     arith = (O.relative_error(mean, kept.image), O.relative_error(var, kept.variance))
     total = (O.relative_error(mean, mean64), O.relative_error(var, var64))
     print(f"config2 (32 views, detach={detach}): mean / variance vs float64 on kept pairs {arith[0]:.2e} / "
-          f"{arith[1]:.2e}; forward flips {kept.flips_fwd} of {kept.pairs} pairs; vs the float64 reference "
+          f"{arith[1]:.2e}; forward flips {kept.flips_fwd} of {kept.pairs} pairs, {cov} coverage differences; "
+          f"vs the float64 reference "
           f"{total[0]:.2e} / {total[1]:.2e}", flush=True)
     assert arith[0] <= 1e-5 and arith[1] <= 1e-5
     print(f"  backward flips {kept.flips_bwd}", flush=True)
@@ -144,3 +150,57 @@ def test_config2_beta_mean_variance_vs_float64_on_kept_pairs(config2_views, deta
         t = O.relative_error(getattr(grads, name), ref.grads[name])
         print(f"  {name}: float32 vs float64 on kept pairs {a:.2e}; vs the float64 reference {t:.2e}", flush=True)
         assert a <= 1e-4, name
+
+
+# --------------------------------------------------------------------------- voxels
+def _voxel_pair(sc, shape, *, beta=None, uncertainty=None, detach=False, adjoint_seed=3):
+    """float32 voxelize fwd + bwd (fused kernels) and the float64 operator path on the same inputs."""
+    kw = dict(volume_extent=(1.0, 1.0, 1.0), detach_variance=detach)
+    d = lambda t: None if t is None else t.double()  # noqa: E731
+    out32, c32 = r2splat_amd.voxelize_fwd(*sc.params(), shape, beta=beta, uncertainty=uncertainty, **kw)
+    out64, c64 = r2splat_amd.voxelize_fwd(*(d(t) for t in sc.params()), shape, beta=d(beta),
+                                          uncertainty=d(uncertainty), **kw)
+    g = torch.Generator(device=sc.means.device).manual_seed(adjoint_seed)
+    v = torch.randn(shape, generator=g, device=sc.means.device)
+    vv = torch.randn(shape, generator=g, device=sc.means.device) if uncertainty is not None else None
+    g32 = r2splat_amd.voxelize_bwd(c32, v, vv)
+    g64 = r2splat_amd.voxelize_bwd(c64, v.double(), None if vv is None else vv.double())
+    return out32, out64, g32, g64
+
+
+def test_config1_voxels_128_vs_float64(device):
+    """The config 1 Gaussians voxelized to 128^3 (extent 1, the benchmark's
+    voxel leg), fused float32 against the float64 operator path."""
+    sc = workloads.config1(device=device)
+    out32, out64, g32, g64 = _voxel_pair(sc, (128, 128, 128))
+    err = O.relative_error(out32, out64)
+    print(f"config1 voxels 128^3: volume vs float64 {err:.2e} (max {float(out64.max()):.3g})", flush=True)
+    assert err <= 1e-5
+    for name in FIELDS:
+        gerr = O.relative_error(getattr(g32, name), getattr(g64, name))
+        print(f"  {name}: {gerr:.2e}", flush=True)
+        assert gerr <= 1e-4, name
+
+
+@pytest.mark.parametrize("uncertainty", [False, True])
+def test_config2_beta_voxels_256_vs_float64(device, uncertainty):
+    """The config 2 Beta primitives voxelized to 256^3: the fused float32 volume
+    (and variance) and every gradient against the float64 operator path. Near
+    the edge of its support the profile (1 - d^2)^beta with beta < 1 is steep
+    enough that float32 rounding of d^2 (or of the voxel-frame mean or conic
+    it is formed from) moves a term by up to ~3e-4, so the fused kernels form
+    d^2 and 1 - d^2 in double from a double-precision frame."""
+    sc = workloads.config2(device=device)
+    out32, out64, g32, g64 = _voxel_pair(sc, (256, 256, 256), beta=sc.beta,
+                                         uncertainty=sc.uncertainty if uncertainty else None)
+    pairs = zip(out32, out64) if uncertainty else [(out32, out64)]
+    for name, (a, b) in zip(("volume", "variance"), pairs):
+        err = O.relative_error(a, b)
+        print(f"config2 voxels 256^3 {name}: vs float64 {err:.2e}", flush=True)
+        assert err <= 1e-5, name
+    for name in BETA_FIELDS:
+        if name == "uncertainty" and not uncertainty:
+            continue
+        gerr = O.relative_error(getattr(g32, name), getattr(g64, name))
+        print(f"  {name}: {gerr:.2e}", flush=True)
+        assert gerr <= 1e-4, name
