@@ -502,6 +502,16 @@ size_t r2s_mse_partials(uint64_t n);
 int r2s_mse_adjoint(const float* image, const float* target, uint64_t n, float* v_image,
                     float* partials, float* loss, r2s_stream_t stream);
 
+/* r2s_mse_adjoint with partials and loss, for callers that cannot allocate
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * device allocation): no counterpart in the reference. The loss partials live
+ * in a static device array of the library (256 KiB per GPU, held until the
+ * process ends) instead of a caller workspace; n is limited to 134,217,728
+ * pixels (larger n returns R2S_ERR_INVALID). Results are bitwise those of
+ * r2s_mse_adjoint. Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (concurrent streams or threads) race. */
+int r2s_mse_adjoint_noalloc(const float* image, const float* target, uint64_t n, float* v_image,
+                            float* loss, r2s_stream_t stream);
 
 /* Adam over a flat float32 buffer (torch.optim.Adam semantics, no weight
  * decay). state uint32 [2]: applied steps (bias correction uses state[0]+1)

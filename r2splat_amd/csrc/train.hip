@@ -7,6 +7,9 @@
 #include "common.hpp"
 #include "raster_common.hpp"
 
+#include <atomic>
+#include <mutex>
+
 namespace r2s {
 
 constexpr int kTrainThreads = 256;
@@ -119,6 +122,74 @@ extern "C" int r2s_mse_adjoint(const float* image, const float* target, uint64_t
         return check_launch("mse_sum");
     }
     return R2S_OK;
+}
+
+// Loss partials of r2s_mse_adjoint_noalloc: one file-scope array per GPU (the
+// code object's own device memory, held until the process ends) in place of
+// a caller workspace. 2^16 partials of 2048 pixels each cover images of up to
+// 134,217,728 pixels in 256 KiB.
+constexpr uint32_t kStaticMsePartials = 1u << 16;
+static __device__ __align__(256) float scratch[kStaticMsePartials];
+
+// Device address of `scratch` on each GPU. In host code the array's name is a
+// host-side stand-in, never a device address: kernels get these pointers.
+constexpr int kMaxDevices = 64;
+static std::atomic<float*> g_scratch[kMaxDevices];
+static std::mutex g_scratch_lock;
+
+// The device that kernels launched on `stream` run on.
+static int stream_device(r2s_stream_t stream, int* device) {
+    if (stream == nullptr) {
+        R2S_HIP(hipGetDevice(device));
+        return R2S_OK;
+    }
+    hipDevice_t d = 0;
+    R2S_HIP(hipStreamGetDevice(stream, &d));
+    *device = int(d);
+    return R2S_OK;
+}
+
+// Looks up (once per device, with that device current) the device address of
+// `scratch` on the device `stream` runs on.
+static int static_partials(r2s_stream_t stream, float** out) {
+    int device = 0;
+    if (int rc = stream_device(stream, &device)) return rc;
+    R2S_REQUIRE(device >= 0 && device < kMaxDevices, "device ordinal %d beyond the %d supported", device,
+                kMaxDevices);
+    float* p = g_scratch[device].load(std::memory_order_acquire);
+    if (p == nullptr) {
+        std::lock_guard<std::mutex> guard(g_scratch_lock);
+        p = g_scratch[device].load(std::memory_order_relaxed);
+        if (p == nullptr) {
+            int current = 0;
+            R2S_HIP(hipGetDevice(&current));
+            if (current != device) R2S_HIP(hipSetDevice(device));
+            void* addr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+            if (current != device) R2S_HIP(hipSetDevice(current));
+            if (e != hipSuccess || addr == nullptr) {
+                set_error("mse_adjoint_noalloc: hipGetSymbolAddress failed on device %d: %s", device,
+                          hipGetErrorString(e));
+                return R2S_ERR_HIP;
+            }
+            p = static_cast<float*>(addr);
+            g_scratch[device].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return R2S_OK;
+}
+
+extern "C" int r2s_mse_adjoint_noalloc(const float* image, const float* target, uint64_t n, float* v_image,
+                                       float* loss, r2s_stream_t stream) {
+    R2S_REQUIRE(loss != nullptr, "mse_adjoint_noalloc: loss is required");
+    const uint64_t limit = uint64_t(kStaticMsePartials) * kMseBlockElems;
+    R2S_REQUIRE(n <= limit, "mse_adjoint_noalloc: %llu pixels exceed the static partial buffer's %llu",
+                (unsigned long long)n, (unsigned long long)limit);
+    if (n == 0) return fill_zero<float>(loss, 1, stream);
+    float* partials = nullptr;
+    if (int rc = static_partials(stream, &partials)) return rc;
+    return r2s_mse_adjoint(image, target, n, v_image, partials, loss, stream);
 }
 
 extern "C" int r2s_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq, uint64_t n,

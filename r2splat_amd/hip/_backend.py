@@ -75,6 +75,7 @@ SIGNATURES = {
                                P, I32, P, SZ, P, P, P, P, P, P],
     "r2s_mse_partials": [U64],
     "r2s_mse_adjoint": [P, P, U64, P, P, P, P],
+    "r2s_mse_adjoint_noalloc": [P, P, U64, P, P, P],
     "r2s_adam_step": [P, P, P, P, U64, F, F, F, F, P, P, P],
     "r2s_clamp_min": [P, U64, F, P],
     "r2s_volume_tv_fwd": [I32, P, U32, U32, U32, U32, P, P, P],
