@@ -46,18 +46,20 @@ PLAIN_CHECK_STEPS = 5
 # --no-graph`; profiles/raster_100k_64x256_occupancy_valu.md). The kernels
 # evaluate only the rows (backward) and row groups (forward) each record can
 # reach, so they issue far fewer operations than pairs x FLOP_PER_PAIR.
-VALU_INSTS = {"rasterize_fwd": 7.370e8, "rasterize_bwd": 8.573e8}
+VALU_INSTS = {"rasterize_fwd": 7.370e8, "rasterize_bwd": 8.121e8}
 SIMDS = 256 * 4          # compute units x SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
-# HBM bytes per launch of the raster kernels on this workload, from two
-# rocprofv3 --pmc runs of `bench.py --steps 2 --warmup 1 --plain-steps 0
-# --no-cpu-baseline --no-voxel --no-graph --copy-calibration` (FETCH_SIZE,
-# then WRITE_SIZE; KiB per dispatch), each scaled by the ratio of the bytes a
-# plain 256 MiB device copy moves to what the same counter reads for it
-# (FETCH 131,089 KiB -> x2.0; WRITE 262,144 KiB -> x1.0). Summary and
-# commands: profiles/raster_100k_64x256_hbm_traffic.md.
-PMC_KIB = {"rasterize_fwd": (1_459_759, 16_441), "rasterize_bwd": (1_480_332, 862_928)}
-PMC_SCALE = (262_144 / 131_089, 262_144 / 262_144)
+# Bytes each launch of the raster kernels reads and writes beyond L2 on this
+# workload, from the L2-to-memory requests counted by size in rocprofv3 --pmc
+# runs of `bench.py --steps 2 --warmup 1 --plain-steps 0 --no-cpu-baseline
+# --no-voxel --no-graph --serve-window 0 --no-parity --copy-calibration`:
+# reads 128 TCC_EA0_RDREQ_128B + 64 TCC_EA0_RDREQ_64B + 32 TCC_EA0_RDREQ_32B,
+# writes 64 TCC_EA0_WRREQ_64B + 32 (TCC_EA0_WRREQ - TCC_EA0_WRREQ_64B)
+# (medians over the dispatches). The plain 256 MiB copy of the same run
+# counts exactly its 268,435,456 bytes each way this way; FETCH_SIZE counts
+# its 128-byte reads as 64 bytes (half). Summary and commands:
+# profiles/raster_100k_64x256_hbm_traffic.md.
+TRAFFIC_BYTES = {"rasterize_fwd": (2_061_623_776, 17_357_984), "rasterize_bwd": (2_386_446_336, 922_386_400)}
 
 
 def parse():
@@ -163,8 +165,7 @@ def main():
     kept = pc["kept_bwd"] if dominant == "rasterize_bwd" else pc["kept_fwd"]
     achieved = kept * FLOP_PER_PAIR[dominant] / (dom_ms * 1e-3) / 1e12
     issue = VALU_INSTS[dominant] / (dom_ms * 1e-3 * CLOCK_HZ * SIMDS)
-    fetch, write = PMC_KIB[dominant]
-    traffic = int((fetch * PMC_SCALE[0] + write * PMC_SCALE[1]) * 1024)
+    traffic = sum(TRAFFIC_BYTES[dominant])
     roofline = {"bound": "valu", "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "traffic_gbs": round(traffic / (dom_ms * 1e-3) / 1e9, 1),

@@ -352,9 +352,11 @@ __device__ __forceinline__ bool project_f64(const float* __restrict__ viewmat, c
 }
 
 // ---------------------------------------------------------------------------
-// Fused projection + tile count. grid (ceil(N/256), C). Culling, radii and the
-// tile rectangle follow the float32 projection (as the operators do); Beta
-// records take their mean, conic and mu from the projection in double.
+// Fused projection + tile count. grid (ceil(N/256), C). Each (camera,
+// primitive) is projected in double (project_f64: culling, radii, tile
+// rectangle, conic and mu as the operators compute them in float64) and its
+// record stores the results rounded to float32; Beta records also keep their
+// mean's float32 remainder.
 // ---------------------------------------------------------------------------
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(256)
@@ -370,48 +372,24 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
     const uint32_t c = blockIdx.y;
     if (n >= N) return;
     const size_t idx = size_t(c) * N + n;
-    Camera<float> cam;
-    cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
-    float q[4], mag, norm, R[9], cov[6];
-    unit_quat(quats + 4 * size_t(n), q, mag, norm);
-    quat_to_rot(q, R);
-    const float s[3] = {scales[3 * size_t(n)], scales[3 * size_t(n) + 1], scales[3 * size_t(n) + 2]};
-    covariance_from(R, s, cov);
-    const float p[3] = {means[3 * size_t(n)], means[3 * size_t(n) + 1], means[3 * size_t(n) + 2]};
     Rec<BETA, VAR> rec = {};
     rec.g = int32_t(idx);
     if constexpr (VAR) rec.unc = unc[n];
     uint2 rect = make_uint2(0u, 0u);
     float2 rr = make_float2(0.f, -1.f);
     uint32_t count = 0;
-    ProjState<float> st;
-    const float Wf = float(W), Hf = float(H);
-    if (project_state(cam, p, cov, Wf, Hf, st)) {
-        const float mx = cam.fx * st.x / st.z + cam.cx;
-        const float my = cam.fy * st.y / st.z + cam.cy;
-        float c2[6];
-        projected_cov(st, c2);
+    double m2[2], c2[6];
+    if (project_f64(viewmats + 16 * size_t(c), Ks + 9 * size_t(c), means + 3 * size_t(n), quats + 4 * size_t(n),
+                    scales + 3 * size_t(n), W, H, m2, c2)) {
         int32_t rx, ry;
-        footprint_radii(c2, mx, my, eps, clip, Wf, Hf, rx, ry);
-        double cd[6], P[3], mu;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
-        rec.mx = mx;
-        rec.my = my;
+        footprint_radii(c2, m2[0], m2[1], double(eps), double(clip), double(W), double(H), rx, ry);
+        double P[3], mu;
+        conic_mu_f64(c2, double(eps), P, mu);
+        rec.mx = float(m2[0]);
+        rec.my = float(m2[1]);
         if constexpr (BETA) {
-            double md[2], c2d[6];
-            if (project_f64(viewmats + 16 * size_t(c), Ks + 9 * size_t(c), p, quats + 4 * size_t(n),
-                            scales + 3 * size_t(n), W, H, md, c2d)) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) cd[k] = c2d[k];
-                rec.mx = float(md[0]);
-                rec.my = float(md[1]);
-                rec.lx = float(md[0] - double(rec.mx));
-                rec.ly = float(md[1] - double(rec.my));
-            }
-        }
-        conic_mu_f64(cd, double(eps), P, mu);
-        if constexpr (BETA) {
+            rec.lx = float(m2[0] - double(rec.mx));
+            rec.ly = float(m2[1] - double(rec.my));
             const float mus = float(mu) * ratio[n];  // mu_beta = mu ratio(beta), in float32
             const float w = dens[n] * mus;
             rec.A = float(P[0]);
@@ -429,8 +407,8 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
             rec.qlim = qlim_of(w);
         }
         uint32_t x0, x1, y0, y1;
-        if (rx > 0 && ry > 0 && tile_span(mx, rx, tile_size, ntx, x0, x1) &&
-            tile_span(my, ry, tile_size, nty, y0, y1)) {
+        if (rx > 0 && ry > 0 && tile_span(m2[0], rx, tile_size, ntx, x0, x1) &&
+            tile_span(m2[1], ry, tile_size, nty, y0, y1)) {
             count = (x1 - x0) * (y1 - y0);
             rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
             rr = make_float2(rec.my, row_reach(rec));

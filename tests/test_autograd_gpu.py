@@ -113,9 +113,10 @@ def _subset_oracle_vjp(sc, idx, v, device):
     from r2splat_amd import hip as ops
 
     sub = [t[idx].detach() for t in sc.params()]
-    covars, _ = ops.parameter_transform_fwd(sub[1].contiguous(), sub[2].contiguous())
-    m2, _, radii = ops.world_to_image_projection_fwd(sub[0].contiguous(), covars, sc.viewmats, sc.Ks, sc.width,
-                                                     sc.height, 1e-8, 3.0)
+    # tile coverage of the operators in float64 (the fused projection's own)
+    covars, _ = ops.parameter_transform_fwd(sub[1].double().contiguous(), sub[2].double().contiguous())
+    m2, _, radii = ops.world_to_image_projection_fwd(sub[0].double().contiguous(), covars, sc.viewmats.double(),
+                                                     sc.Ks.double(), sc.width, sc.height, 1e-8, 3.0)
     mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), 16, sc.width, sc.height)
     leaves = [t.cpu().double().requires_grad_() for t in sub]
     total = [torch.zeros_like(t) for t in leaves]

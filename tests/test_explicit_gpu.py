@@ -294,9 +294,12 @@ def test_large_float32_beta_centre_projection(device, row):
 
 # --------------------------------------------------------------------------- working-size parity
 def _gpu_tile_mask(scene, eps, clip, tile=16):
-    covars, _ = ops.parameter_transform_fwd(scene.quats, scene.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(scene.means, covars, scene.viewmats, scene.Ks, scene.width,
-                                                     scene.height, eps, clip)
+    """Tile coverage of the operators in float64 (which the fused float32
+    projection also computes in double)."""
+    d = lambda t: t.double()  # noqa: E731
+    covars, _ = ops.parameter_transform_fwd(d(scene.quats), d(scene.scales))
+    m2, _, radii = ops.world_to_image_projection_fwd(d(scene.means), covars, d(scene.viewmats), d(scene.Ks),
+                                                     scene.width, scene.height, eps, clip)
     return O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), tile, scene.width, scene.height)
 
 

@@ -27,10 +27,17 @@ def _run(sc, backend, v_fn=None):
         explicit.BACKEND = old
 
 
+def _proj64(sc, clip=3.0, W=None, H=None, eps=1e-8):
+    """means2d and radii of the operators in float64: the tile coverage the
+    fused projection (which projects in double) reproduces."""
+    covars, _ = ops.parameter_transform_fwd(sc.quats.double(), sc.scales.double())
+    m2, _, radii = ops.world_to_image_projection_fwd(sc.means.double(), covars, sc.viewmats.double(), sc.Ks.double(),
+                                                     W or sc.width, H or sc.height, eps, clip)
+    return m2, radii
+
+
 def _mask(sc, eps=1e-8, clip=3.0):
-    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
-                                                     eps, clip)
+    m2, radii = _proj64(sc, clip, eps=eps)
     return O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), 16, sc.width, sc.height)
 
 
@@ -51,12 +58,15 @@ def test_fused_matches_oracle(device, size):
 
 
 def test_fused_matches_staged_ops_path(device):
+    """Config 0: the fused pipeline (projection in double, records and
+    rasterization in float32) and the staged float32 operators agree to 1e-5
+    on the image and 1e-4 on every gradient."""
     sc = workloads.config0(device=device)
     a_img, a, _, _ = _run(sc, "fused")
     b_img, b, _, _ = _run(sc, "ops")
-    assert O.relative_error(a_img, b_img) <= 2e-6
+    assert O.relative_error(a_img, b_img) <= 1e-5
     for name in ("means", "quats", "scales", "densities"):
-        assert O.relative_error(getattr(a, name), getattr(b, name)) <= 2e-5, name
+        assert O.relative_error(getattr(a, name), getattr(b, name)) <= 1e-4, name
 
 
 def _count_grows(monkeypatch):
@@ -106,10 +116,7 @@ def test_sort_path_overflow_is_flagged_and_regrown(device, monkeypatch):
 
 
 def _proj(sc):
-    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
-                                                     1e-8, 3.0)
-    return m2, radii
+    return _proj64(sc, 3.0)
 
 
 def test_fused_tile_offsets_match_operator_binning(device):
@@ -192,19 +199,27 @@ def test_outstanding_contexts_are_independent(device):
         assert torch.equal(getattr(ga, name), getattr(ga2, name)), name
 
 
-def test_working_size_fused_agrees_with_staged_path(device):
-    """Config 1 (100k Gaussians, 64 views of 256^2): the fused pipeline and the
-    staged operator path (itself pinned to the oracle at config 0) agree to
-    1e-5 on the image and 1e-4 on every gradient; intersection counts match."""
+def test_working_size_fused_agrees_with_float64_operator_path(device):
+    """Config 1 (100k Gaussians, 64 views of 256^2): the fused float32 pipeline
+    against the staged operator path in float64 (itself pinned to the oracle
+    at config 0), everything included (pairs decided differently at the 1e-5
+    cut-off too): 1e-5 on the image and 1e-4 on every gradient. The fused
+    projection computes radii and tile rectangles in double, so the two paths
+    cover the same tiles."""
     sc = workloads.config1(device=device)
     tgt = workloads.config1(device=device, seed=1)
     target, _ = r2splat_amd.rasterize_fwd(*tgt.params(), tgt.viewmats, tgt.Ks, tgt.image_size)
     vfn = lambda img: ((2.0 / img.numel()) * (img - target)).contiguous()
-    a_img, a, _, _ = _run(sc, "fused", vfn)
-    b_img, b, _, _ = _run(sc, "ops", vfn)
-    assert O.relative_error(a_img, b_img) <= 1e-5
+    a_img, a, v, _ = _run(sc, "fused", vfn)
+    p64 = [t.double() for t in sc.params()]
+    img64, ctx64 = r2splat_amd.rasterize_fwd(*p64, sc.viewmats.double(), sc.Ks.double(), sc.image_size)
+    b = r2splat_amd.rasterize_bwd(ctx64, v.double())
+    err = O.relative_error(a_img, img64)
+    print(f"config1 fused float32 vs float64 operators: image {err:.2e}", flush=True)
+    assert err <= 1e-5
     for name in ("means", "quats", "scales", "densities"):
         err = O.relative_error(getattr(a, name), getattr(b, name))
+        print(f"  {name}: {err:.2e}", flush=True)
         assert err <= 1e-4, f"{name}: {err:.2e}"
     # additivity over a split of the primitive set (each primitive's cut-off depends only on
     # itself, so render(A u B) = render(A) + render(B) up to summation order)
@@ -232,8 +247,7 @@ def test_fused_tile_sizes_and_wave_counts_match_oracle(device, tile, waves, W, H
     assert f.n_isects == ids.numel()
     v = (2.0 * (img - 0.8 * img) / img.numel()).contiguous()
     grads = fused.backward(f, *sc.params(), sc.viewmats, sc.Ks, v, waves=waves)
-    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, W, H, 1e-8, 3.0)
+    m2, radii = _proj64(sc, 3.0, W, H)
     mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), tile, W, H)
     leaves = [t.detach().cpu().double().requires_grad_() for t in sc.params()]
     ref = O.rasterize_scene(*leaves, sc.viewmats.cpu().double(), sc.Ks.cpu().double(), (H, W), eps=1e-8, mask=mask)
@@ -256,9 +270,7 @@ def _beta_scene(device, n=700, c=3, H=56, W=72, seed=6):
 
 
 def _beta_oracle(sc, v, tile, clip=1.0):
-    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
-                                                     1e-8, clip)
+    m2, radii = _proj64(sc, clip)
     mask = O.tile_mask_2d(m2.cpu().numpy(), radii.cpu().numpy(), tile, sc.width, sc.height)
     leaves = [t.detach().cpu().double().requires_grad_() for t in sc.params() + (sc.beta,)]
     ref = O.rasterize_scene(*leaves[:4], sc.viewmats.cpu().double(), sc.Ks.cpu().double(), sc.image_size, eps=1e-8,
@@ -294,10 +306,7 @@ def test_fused_beta_matches_oracle(device, tile, waves, W, H):
 
 
 def _proj_clip(sc, clip):
-    covars, _ = ops.parameter_transform_fwd(sc.quats, sc.scales)
-    m2, _, radii = ops.world_to_image_projection_fwd(sc.means, covars, sc.viewmats, sc.Ks, sc.width, sc.height,
-                                                     1e-8, clip)
-    return m2, radii
+    return _proj64(sc, clip)
 
 
 def _run_beta(sc, backend, v_fn=None):
