@@ -321,7 +321,10 @@ int r2s_voxelize_to_voxels_bwd_per_gaussian(
  * decreasing height class, a deterministic order that only changes the
  * per-pixel summation order), then the C*N projected records of
  * r2s_fused_record_bytes(kind) each (offset: capacity * 8 rounded up to 256
- * bytes), which the raster kernels gather through the entries; tile_offsets int32
+ * bytes; Beta records are 96 bytes: the 48 the raster kernels gather through
+ * the entries, then the record's mean, conic and cut-off bound in double, which
+ * the backward reads to decide pairs within float32 rounding of the cut-off as
+ * float64 does); tile_offsets int32
  * [C, ceil(H/tile), ceil(W/tile)] (the reference's layout); tile_order
  * uint32 [same count]
  * (tiles by decreasing intersection count, the order the raster kernels

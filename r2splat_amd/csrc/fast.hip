@@ -29,7 +29,10 @@
 // qlim = 0 (nothing passes) when S < 0, w = 0 or S is NaN.
 // Beta primitives (d^2 = delta^T P delta, alpha = w (1 - d^2)^b2d kept for
 // 0 <= d^2 < 1 and |alpha| >= 1e-5) use the same single compare on d^2 with
-// the bound D = min(largest float below 1, 1 - (1e-5 / |w|)^(1 / b2d)).
+// the bound D = min(largest float below 1, 1 - (1e-5 / |w|)^(1 / b2d))
+// (computed in double); the backward decides the pairs in a narrow band
+// around D, where float32 d^2 may fall on the other side from float64 d^2, in
+// double from the record's double-precision form (BetaEdge, edge_keep).
 #include <algorithm>
 #include <type_traits>
 
@@ -66,6 +69,33 @@ static_assert(sizeof(BetaSplat) == 48, "wide record must be 48 bytes");
 template <bool BETA, bool VAR = false>
 using Rec = typename std::conditional<BETA || VAR, BetaSplat, Splat>::type;
 
+// Double-precision form of a Beta record: its mean, conic (d^2 = a dx^2 +
+// 2 b dx dy + c dy^2) and the largest d^2 whose term passes the cut-off
+// (|alpha| = |w| (1 - d^2)^b2d >= 1e-5; -1 when none does). The raster
+// backward reads it only for pairs whose float32 d^2 lies within rounding of
+// the record's float32 bound (edge_keep), so that each of its kept / skipped
+// decisions is the one float64 makes: at the cut-off a Beta term's gradient
+// jumps by about w b2d h^(b2d - 1) |d d^2|, not by 1e-5 as a Gaussian's does.
+// (The forward keeps the float32 decision: a pair decided the other way moves
+// the image by at most 1e-5.)
+struct __align__(16) BetaEdge {
+    double mx, my, a, b, c, dmax;
+};
+// Stored Beta record: the 48-byte record the raster kernels gather, then its
+// double-precision form.
+struct __align__(16) BetaRec {
+    BetaSplat s;
+    BetaEdge e;
+};
+static_assert(sizeof(BetaRec) == 96, "stored Beta record must be 96 bytes");
+
+template <bool BETA, bool VAR = false>
+using Stored = typename std::conditional<BETA, BetaRec, Rec<BETA, VAR>>::type;
+
+template <typename R>
+__device__ __forceinline__ const R& head(const R& r) { return r; }
+__device__ __forceinline__ const BetaSplat& head(const BetaRec& r) { return r.s; }
+
 // Pixel-minus-mean offsets of a record: Beta records carry their mean's
 // float32 remainder (the profile's slope at its support edge would pass the
 // rounding of a pixel coordinate to the image), the others do not.
@@ -99,14 +129,69 @@ __device__ __forceinline__ uint32_t qlim_of(float w) {
     return (w != 0.f && S >= 0.f) ? __float_as_uint(S) + 1u : 0u;
 }
 
-__device__ __forceinline__ uint32_t beta_qlim_of(float w, float b2d) {
-    const float S = __log2f(fabsf(w)) + kLog2Inv1e5;  // log2(|w| / 1e-5)
-    const float D = 1.f - __builtin_amdgcn_exp2f(-S / b2d);
-    if (!(w != 0.f && S >= 0.f && D >= 0.f)) return 0u;
-    return __float_as_uint(fminf(D, 0.99999994f)) + 1u;
+__device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
+
+// The Beta line-integral factor r(beta) = Gamma(beta + 1) / (sqrt(2)
+// Gamma(beta + 3/2)) in double, as the host computes it (lgamma below 20; from
+// 20 the Stirling series of log Gamma(z) - log Gamma(z + 1/2) + 1/2 log z, z =
+// beta + 1, which avoids the cancellation of two large lgamma values).
+__device__ __forceinline__ double beta_ratio_f64(double beta) {
+    constexpr double kHalfLn2 = 0.34657359027997264;
+    if (beta < 20.0) return exp(lgamma(beta + 1.0) - lgamma(beta + 1.5) - kHalfLn2);
+    const double z = beta + 1.0, w = 1.0 / z, w2 = w * w;
+    const double series = w * (1.0 / 8.0 + w2 * (-1.0 / 192.0 + w2 * (1.0 / 640.0 + w2 * (-17.0 / 14336.0 +
+                                                                                          w2 * (31.0 / 18432.0)))));
+    return exp(series - 0.5 * (2.0 * kHalfLn2 + log(z)));
 }
 
-__device__ __forceinline__ bool contributes(float q, uint32_t qlim) { return __float_as_uint(q) < qlim; }
+// The Beta cut-off in double: the largest d^2 with |w| (1 - d^2)^b2d >= 1e-5
+// (-1 when |w| < 1e-5), and the qlim of its float32 rounding.
+__device__ __forceinline__ double beta_dmax(double w, double b2d) {
+    const double aw = fabs(w);
+    if (!(aw >= 1e-5) || !(b2d > 0.0)) return -1.0;
+    return 1.0 - pow(1e-5 / aw, 1.0 / b2d);
+}
+__device__ __forceinline__ uint32_t beta_qlim_from(double dmax) {
+    if (!(dmax >= 0.0)) return 0u;
+    return __float_as_uint(fminf(float(dmax), 0.99999994f)) + 1u;
+}
+
+// Half-width of the band around a Beta record's float32 bound D in which its
+// float32 d^2 can fall on the other side of the cut-off from the float64
+// d^2: with rho = |b| / sqrt(a c) the terms of d^2 sum to at most (1 + rho) /
+// (1 - rho) d^2, each carrying a few roundings (conic, offsets, products);
+// 16 float32 ulps of that sum, plus the rounding of D itself. Degenerate
+// forms (rho near 1) get the whole range.
+__device__ __forceinline__ float edge_band(const BetaSplat& s) {
+    const float ac = s.A * s.C, b = 0.5f * s.B;
+    const float rho = (ac > 0.f) ? fabsf(b) * __builtin_amdgcn_rsqf(ac) : 1.f;
+    if (!(rho < 0.999f)) return 2.f;
+    return 9.5367431640625e-07f * (1.f + rho) / (1.f - rho) + 5.9604644775390625e-08f;  // 16 ulp(1) x ..., + ulp(1)
+}
+
+// The edge band as float32 bit limits: pairs with bits(d^2) < x pass the
+// cut-off whatever their float64 d^2, pairs with bits(d^2) >= y fail it, and
+// those in between are settled in double (edge_keep). Both 0 when the record
+// contributes nowhere (and for records without a Beta form).
+template <typename R>
+__device__ __forceinline__ uint2 edge_limits(const R& s) {
+    if constexpr (std::is_same<R, BetaSplat>::value) {
+        if (s.qlim == 0u) return make_uint2(0u, 0u);
+        const float bound = __uint_as_float(s.qlim - 1u), band = edge_band(s);
+        const float lo = bound - band, hi = bound + band;
+        return make_uint2(lo >= 0.f ? __float_as_uint(lo) + 1u : 0u, __float_as_uint(hi) + 1u);
+    } else {
+        return make_uint2(0u, 0u);
+    }
+}
+
+// Whether a Beta pair passes the cut-off in double: pixel centre (px, py),
+// the record's double-precision form e (0 <= d^2 < 1 and d^2 <= dmax).
+__device__ __forceinline__ bool edge_keep(const BetaEdge& e, float px, float py) {
+    const double dx = e.mx - double(px), dy = e.my - double(py);
+    const double q = e.a * dx * dx + 2.0 * e.b * dx * dy + e.c * dy * dy;
+    return q >= 0.0 && q < 1.0 && q <= e.dmax;
+}
 
 // LDS written by some lanes of a wave is visible to its other lanes.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -366,13 +451,14 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
                      const float* __restrict__ unc, const float* __restrict__ viewmats,
                      const float* __restrict__ Ks, uint32_t W, uint32_t H, float eps, float clip,
                      uint32_t tile_size, uint32_t ntx, uint32_t nty, uint32_t N,
-                     Rec<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects, float2* __restrict__ reach,
+                     Stored<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects, float2* __restrict__ reach,
                      uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = blockIdx.y;
     if (n >= N) return;
     const size_t idx = size_t(c) * N + n;
     Rec<BETA, VAR> rec = {};
+    BetaEdge edge = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0};
     rec.g = int32_t(idx);
     if constexpr (VAR) rec.unc = unc[n];
     uint2 rect = make_uint2(0u, 0u);
@@ -397,7 +483,11 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
             rec.C = float(P[2]);
             rec.w = w;
             rec.b2d = betas[n] + 0.5f;
-            rec.qlim = beta_qlim_of(w, rec.b2d);
+            // the cut-off in double (w = density mu r(beta) and b2d as float64 forms them)
+            const double dmax = beta_dmax(double(dens[n]) * mu * beta_ratio_f64(double(betas[n])),
+                                          double(betas[n]) + 0.5);
+            rec.qlim = beta_qlim_from(dmax);
+            edge = BetaEdge{m2[0], m2[1], P[0], P[1], P[2], dmax};
         } else {
             const float w = dens[n] * float(mu);
             rec.A = 0.5f * float(P[0]) * kLog2e;
@@ -414,7 +504,10 @@ fused_project_kernel(const float* __restrict__ means, const float* __restrict__ 
             rr = make_float2(rec.my, row_reach(rec));
         }
     }
-    recs[idx] = rec;
+    if constexpr (BETA)
+        recs[idx] = BetaRec{rec, edge};
+    else
+        recs[idx] = rec;
     rects[idx] = rect;
     if (reach) reach[idx] = rr;
     counts[idx] = count;
@@ -837,12 +930,12 @@ __device__ __forceinline__ uint2 load_entry(const uint2* __restrict__ entries, i
 }
 
 // The record an entry names, with g = the entry's emission index (g = -1: none).
-template <typename R>
-__device__ __forceinline__ R gather_splat(const R* __restrict__ recs, uint2 e) {
+template <typename R, typename S>
+__device__ __forceinline__ R gather_splat(const S* __restrict__ recs, uint2 e) {
     R s = {};
     s.g = -1;
     if (e.x != 0xffffffffu) {
-        s = recs[e.x];
+        s = head(recs[e.x]);
         s.g = int32_t(e.y);
     }
     return s;
@@ -859,7 +952,7 @@ __device__ __forceinline__ void settle(const R& s) {
 
 template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
                         const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
@@ -895,7 +988,7 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
     // two-stage prefetch: a batch's records are gathered one batch ahead, their
     // list entries two batches ahead
     uint2 ahead = load_entry(entries, lo + int32_t(lane), hi);
-    R next = gather_splat(recs, ahead);
+    R next = gather_splat<R>(recs, ahead);
     ahead = load_entry(entries, lo + kStride + int32_t(lane), hi);
     const float y_base = float(ty * T);
     for (int32_t base = lo; base < hi; base += kStride) {
@@ -909,7 +1002,7 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
         wave_lds_sync();  // every lane is done reading the previous batch
         slot[lane] = next;
         wave_lds_sync();
-        next = gather_splat(recs, ahead);
+        next = gather_splat<R>(recs, ahead);
         ahead = load_entry(entries, base + 2 * kStride + int32_t(lane), hi);
         const int32_t n = min(int32_t(kWave), hi - base);
         // LDS reads pipelined one record ahead in two registers that alternate
@@ -984,7 +1077,7 @@ constexpr int bwd_waves_per_eu() { return (T == 16 && !BETA && !VAR) ? 5 : 1; }
 
 template <int T, int WAVES, bool BETA, bool VAR>
 __global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
-fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
                         const int32_t* __restrict__ offsets,
                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
                         uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
@@ -1016,14 +1109,17 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
     int32_t chunk = start + int32_t(wave) * kWave;
     // records gathered a chunk ahead, list entries two chunks ahead
     uint2 ahead = load_entry(entries, chunk + int32_t(lane), end);
-    R next = gather_splat(recs, ahead);
+    R next = gather_splat<R>(recs, ahead);
+    uint32_t next_rec = ahead.x;
     ahead = load_entry(entries, chunk + kStride + int32_t(lane), end);
     for (; chunk < end; chunk += kStride) {
         const int32_t i = chunk + int32_t(lane);
         const bool active = i < end;
         const R s = next;
+        const uint32_t rec = next_rec;  // Beta: the record whose double-precision form settles edge pairs
         settle(s);
-        next = gather_splat(recs, ahead);
+        next = gather_splat<R>(recs, ahead);
+        next_rec = ahead.x;
         ahead = load_entry(entries, chunk + 2 * kStride + int32_t(lane), end);
         const float dx0 = offset_x<BETA>(s, x0);
         float m0 = 0.f, mx = 0.f, mxx = 0.f, my = 0.f, mxy = 0.f, myy = 0.f, se = 0.f, sl = 0.f, vq = 0.f;
@@ -1034,6 +1130,8 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
         // the wave runs as many rows as its tallest record needs
         int r_lo = 0, r_hi = 0;
         if (active) row_span(s, y0 - 0.5f, T, r_lo, r_hi);
+        // Beta: pairs below lim.x pass; those in [lim.x, lim.y) are settled in double
+        const uint2 lim = (BETA && active) ? edge_limits(s) : make_uint2(0u, 0u);
         // the columns' offsets and their squares are the same in every row
         float dxc[T], dxsq[T];
 #pragma unroll
@@ -1066,32 +1164,41 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
                 }
             }
             float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+            // one Beta pair's terms (in: it passes the cut-off; h <= 0 outside the
+            // support makes log2 h -inf or NaN there, so it is selected away, not
+            // multiplied by a zero weight; kept pairs have h >= the smallest normal float)
+            auto beta_pair = [&](float q, float dx, float dxsq_j, float vi, float vvi, bool in) {
+                const float h = fmaxf(1.f - q, 1.17549435e-38f), lg = __log2f(h);
+                const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(b2d - 1)
+                float g, ve;
+                if constexpr (VAR) {
+                    const float he = hb * h;  // h^b2d = alpha / w
+                    const float vj = fmaf(kvar * he, vvi, vi);
+                    g = vj * hb;
+                    ve = vj * he;
+                    vq = fmaf(vvi * he, he, vq);
+                } else {
+                    g = vi * hb;
+                    ve = g * h;
+                }
+                se += ve;
+                sl = fmaf(ve, in ? lg : 0.f, sl);
+                r0 += g;
+                r1 = fmaf(g, dx, r1);
+                r2 = fmaf(g, dxsq_j, r2);
+            };
+            bool any_near = false;  // Beta: a pair of this row lies in the edge band
 #pragma unroll
             for (int j = 0; j < T; ++j) {
                 const float dx = dxc[j];
                 const float q = form_by_rows(s.A, dx, bdy, cdy2);
-                float g;
                 if constexpr (BETA) {
-                    // h <= 0 outside the support: log2 h is -inf or NaN there, so it is
-                    // selected away (not multiplied by a zero weight)
-                    const bool in = contributes(q, s.qlim);
-                    const float h = 1.f - q, lg = __log2f(h);
-                    const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(b2d - 1)
-                    float ve;
-                    if constexpr (VAR) {
-                        const float he = hb * h;  // h^b2d = alpha / w
-                        const float vj = fmaf(kvar * he, vv[j], v[j]);
-                        g = vj * hb;
-                        ve = vj * he;
-                        vq = fmaf(vv[j] * he, he, vq);
-                    } else {
-                        g = v[j] * hb;
-                        ve = g * h;
-                    }
-                    se += ve;
-                    sl = fmaf(ve, in ? lg : 0.f, sl);
+                    const bool in = contributes(q, lim.x);
+                    any_near = any_near || (!in && contributes(q, lim.y));
+                    beta_pair(q, dx, dxsq[j], v[j], VAR ? vv[VAR ? j : 0] : 0.f, in);
                 } else {
                     const float e = contributes(q, s.qlim) ? __builtin_amdgcn_exp2f(-q) : 0.f;
+                    float g;
                     if constexpr (VAR) {
                         const float vj = fmaf(kvar * e, vv[j], v[j]);
                         g = vj * e;
@@ -1099,10 +1206,23 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
                     } else {
                         g = e * v[j];
                     }
+                    r0 += g;
+                    r1 = fmaf(g, dx, r1);
+                    r2 = fmaf(g, dxsq[j], r2);
                 }
-                r0 += g;
-                r1 = fmaf(g, dx, r1);
-                r2 = fmaf(g, dxsq[j], r2);
+            }
+            if constexpr (BETA) {
+                // the pairs of this row in the edge band, settled in double (rare)
+                if (__builtin_expect(__ballot(any_near) != 0ull, 0) && any_near) {
+#pragma unroll 1
+                    for (int j = 0; j < T; ++j) {
+                        const float dx = dx0 + float(j);
+                        const float q = form_by_rows(s.A, dx, bdy, cdy2);
+                        if (!contributes(q, lim.x) && contributes(q, lim.y) &&
+                            edge_keep(recs[rec].e, x0 + float(j), y0 + float(r)))
+                            beta_pair(q, dx, dx * dx, sv[0][r * T + j], VAR ? sv[VAR ? 1 : 0][r * T + j] : 0.f, true);
+                    }
+                }
             }
             m0 += r0;
             mx += r1;
@@ -1144,7 +1264,7 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>*
 // ---------------------------------------------------------------------------
 template <int T, bool BETA, bool VAR>
 __global__ void __launch_bounds__(256)
-fused_keep_mask_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* __restrict__ recs,
+fused_keep_mask_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
                        const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n, uint32_t capacity,
                        uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t* __restrict__ keep_fwd,
                        uint32_t* __restrict__ keep_bwd) {
@@ -1159,24 +1279,36 @@ fused_keep_mask_kernel(const uint2* __restrict__ entries, const Rec<BETA, VAR>* 
     const float x0 = float(tx * T) + 0.5f, y0 = float(ty * T) + 0.5f;
     for (int64_t i = int64_t(start) * WORDS + threadIdx.x; i < int64_t(end) * WORDS; i += blockDim.x) {
         const int32_t entry = int32_t(i / WORDS), word = int32_t(i % WORDS);
-        const Rec<BETA, VAR> s = recs[entries[entry].x];
+        const uint32_t rec = entries[entry].x;
+        const Rec<BETA, VAR> s = head(recs[rec]);
         int r_lo, r_hi;
         row_span(s, y_base, T, r_lo, r_hi);
         const int k_lo = r_lo / STEP, k_hi = (r_hi + STEP - 1) / STEP;  // the forward's row groups
+        // the backward's decision: the float32 cut-off; Beta: below the edge band
+        // pass, in it settled in double (the forward takes the float32 cut-off)
+        const uint2 lim = BETA ? edge_limits(s) : make_uint2(0u, 0u);
+        auto keeps_bwd = [&](float q, float px, float py) {
+            if constexpr (BETA) {
+                if (contributes(q, lim.x)) return true;
+                return contributes(q, lim.y) && edge_keep(recs[rec].e, px, py);
+            }
+            return contributes(q, s.qlim);
+        };
         uint32_t bits_fwd = 0u, bits_bwd = 0u;
         for (int b = 0; b < 32; ++b) {
             const int pix = word * 32 + b, r = pix / T, col = pix % T;
+            const float px = float(tx * T + uint32_t(col)) + 0.5f, py = float(ty * T + uint32_t(r)) + 0.5f;
             if (r_lo < r_hi && r / STEP >= k_lo && r / STEP < k_hi) {
-                const float dx = offset_x<BETA>(s, float(tx * T + uint32_t(col)) + 0.5f);
+                const float dx = offset_x<BETA>(s, px);
                 const float adx2 = s.A * dx * dx, bdx = s.B * dx;
-                const float dy = offset_y<BETA>(s, float(ty * T + uint32_t(r)) + 0.5f);
+                const float dy = offset_y<BETA>(s, py);
                 if (contributes(form_by_columns(adx2, bdx, s.C, dy), s.qlim)) bits_fwd |= 1u << b;
             }
             if (r >= r_lo && r < r_hi) {
                 const float dx = offset_x<BETA>(s, x0) + float(col);
                 const float dy = offset_y<BETA>(s, y0 + float(r));
                 const float bdy = s.B * dy, cdy2 = s.C * dy * dy;
-                if (contributes(form_by_rows(s.A, dx, bdy, cdy2), s.qlim)) bits_bwd |= 1u << b;
+                if (keeps_bwd(form_by_rows(s.A, dx, bdy, cdy2), px, py)) bits_bwd |= 1u << b;
             }
         }
         keep_fwd[i] = bits_fwd;
@@ -1368,7 +1500,9 @@ static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 // kind: bit 0 = Beta primitives, bit 1 = primitives with an uncertainty
 // (variance output); either selects the 48-byte record.
 constexpr int kKindBeta = 1, kKindVar = 2;
-static size_t record_bytes(int kind) { return (kind & (kKindBeta | kKindVar)) ? sizeof(BetaSplat) : sizeof(Splat); }
+static size_t record_bytes(int kind) {
+    return (kind & kKindBeta) ? sizeof(BetaRec) : (kind & kKindVar) ? sizeof(BetaSplat) : sizeof(Splat);
+}
 
 // The packed buffer the forward leaves for the raster kernels and the
 // backward: `capacity` list entries {record index, emission index} (uint2),
@@ -1490,7 +1624,7 @@ struct LaunchFwd {
                     uint32_t W, uint32_t H, float* image, float* image_var) {
         fused_raster_fwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
             static_cast<const uint2*>(packed),
-            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
             offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, image, image_var);
     }
 };
@@ -1503,7 +1637,7 @@ struct LaunchBwd {
                     float* isect_grads) {
         fused_raster_bwd_kernel<T, WAVES, BETA, VAR><<<n_tiles, WAVES * kWave, 0, stream>>>(
             static_cast<const uint2*>(packed),
-            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
             offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, v_image, v_image_var, detach, isect_grads);
     }
 };
@@ -1515,7 +1649,7 @@ struct LaunchKeep {
                     uint32_t* keep_bwd) {
         fused_keep_mask_kernel<T, BETA, VAR><<<n_tiles, 256, 0, stream>>>(
             static_cast<const uint2*>(packed),
-            reinterpret_cast<const Rec<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
+            reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
             offsets, counters, capacity, n_tiles, ntx, nty, keep_fwd, keep_bwd);
     }
 };
@@ -1527,7 +1661,7 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
                         uint32_t height, uint32_t tile_size, float eps, float radius_clip, uint32_t capacity,
                         void* scratch, size_t scratch_bytes, void* packed, int32_t* tile_offsets,
                         uint32_t* tile_order, uint32_t* record_offsets, uint32_t* counters, hipStream_t stream) {
-    using R = Rec<BETA, VAR>;
+    using R = Stored<BETA, VAR>;
     const uint32_t ntx = (width + tile_size - 1) / tile_size, nty = (height + tile_size - 1) / tile_size;
     if (int rc = fused_check(C, N, ntx, nty, capacity)) return rc;
     const FusedScratch L = fused_scratch(C, N, C * ntx * nty, capacity, (BETA ? kKindBeta : 0) | (VAR ? kKindVar : 0));

@@ -10,6 +10,7 @@ import r2splat_amd
 from oracle import splat as O
 from r2splat_amd import explicit, fused, workloads
 from r2splat_amd import hip as ops
+from r2splat_amd.hip import _backend as be
 
 from r2splat_amd.parity import evaluate, fp64_scene, pair_table
 
@@ -159,7 +160,9 @@ def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
     clip = 1.0 if use_beta else 3.0
     _, f = fused.forward(*sc.params(), sc.viewmats, sc.Ks, 80, 96, 1e-8, clip, beta=sc.beta)
     n, C, N, ntx, nty = f.n_isects, 5, 3000, 5, 6
-    width = 12 if use_beta else 8
+    # stored record width in floats (Beta records carry their double-precision form after the 12 floats)
+    width = be.lib().r2s_fused_record_bytes(f.kind) // 4
+    assert width == (24 if use_beta else 8)
     # packed = list entries {record index, emission index}, then the C*N records
     entries = f.packed[: n * 8].view(torch.int32).reshape(n, 2).cpu().numpy()
     rec0 = -(-f.capacity * 8 // 256) * 256
