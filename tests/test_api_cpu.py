@@ -83,11 +83,12 @@ def test_library_version_and_workspace_queries_need_no_gpu():
 
 def test_fused_size_queries_need_no_gpu():
     """The packed buffer: `capacity` 8-byte list entries (rounded up to 256
-    bytes), then C*N records of 32 bytes (Gaussian) or 48 bytes (Beta or
-    uncertainty); the scratch grows with the intersection capacity only on
-    the sort path (tile grids too large for direct placement)."""
+    bytes), then C*N records of 32 bytes (Gaussian), 48 bytes (Gaussian with
+    uncertainty) or 96 bytes (Beta: the 48-byte record and its double-precision
+    form); the scratch grows with the intersection capacity only on the sort
+    path (tile grids too large for direct placement)."""
     lib = _backend.lib(require_gpu=False)
-    assert [lib.r2s_fused_record_bytes(k) for k in range(4)] == [32, 48, 48, 48]
+    assert [lib.r2s_fused_record_bytes(k) for k in range(4)] == [32, 96, 48, 96]
     for C, N, cap, kind in ((4, 1000, 1000, 0), (64, 100_000, 30_000_000, 0), (3, 7, 5, 3)):
         entries = -(-cap * 8 // 256) * 256
         records = -(-C * N * lib.r2s_fused_record_bytes(kind) // 256) * 256
