@@ -88,6 +88,9 @@ def parse():
     p.add_argument("--plain-jobs", type=int, default=3, help="serving: requests re-fitted on the plain path")
     p.add_argument("--jobs", type=int, default=32, help="jobs of the batched workload")
     p.add_argument("--streams", type=int, default=4, help="streams of the batched workload")
+    p.add_argument("--no-packed", action="store_true",
+                   help="batched: run each job's raster stages as its own launches on its stream instead of "
+                        "packed launches over all jobs' tiles")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run one plain 256 MiB device copy (reference kernel for HBM byte counters)")
     return p.parse_args()
@@ -243,7 +246,8 @@ def batched_main(args):
     t_gen = time.perf_counter() - t0
     tuner = Autotuner(repeats=2)
     t0 = time.perf_counter()
-    batch = BatchedFit(jobs, adam=AdamConfig(lr=1e-2, min_scale=1e-6), n_streams=args.streams, tuner=tuner)
+    batch = BatchedFit(jobs, adam=AdamConfig(lr=1e-2, min_scale=1e-6), n_streams=args.streams, tuner=tuner,
+                       packed=not args.no_packed)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t0
     batch.step(1)
@@ -311,6 +315,7 @@ def batched_main(args):
         "data": "synthetic (per-job seeded Gaussian sets and orbit cameras; targets from second seeded sets)",
         "config": {"workload": f"batched: {len(jobs)} jobs, N log-uniform [10k, 400k], C in {{16,32,64,128}}, "
                                f"side in {{128,256,512}}, Adam lr 1e-2", "streams": args.streams,
+                   "packed_raster_launches": batch.packed,
                    "jobs": [list(sp[:3]) for sp in specs]},
         "job_steps_per_s": round(len(jobs) * 1e3 / ms, 2),
         "applied_steps": applied,

@@ -420,6 +420,42 @@ int r2s_fused_keep_masks(const void* packed, const int32_t* tile_offsets, const 
  * 0 and the workspace arguments may be NULL / 0 (kept for callers that pass
  * a buffer). */
 size_t r2s_fused_param_workspace_bytes(uint32_t C, uint32_t N);
+
+/* Packed launches, no counterpart in the reference (which fits one scene per
+ * call): the render or the raster VJP of several independent jobs (prepared
+ * fused forwards of any C, N and image size, with one tile edge, wave count
+ * and record kind) as one launch over the concatenation of their tiles, in
+ * job order (job j's C * ceil(H/tile) * ceil(W/tile) tiles from first_tile
+ * on: first_tile is the exclusive prefix of those counts). Each job's results
+ * are bitwise those of its own r2s_fused_render / r2s_fused_raster_vjp.
+ * Each of the 8 XCDs takes the band of every job's tile order that the job's
+ * own launch would give it, so jobs of very different per-tile cost spread
+ * evenly. jobs_host and jobs_device hold the same table (n_jobs entries, 1 to 1024):
+ * the host copy sizes and checks the launch, the kernels read the device
+ * copy, which must stay valid while the launch (or a graph that captured
+ * it) runs. Render reads packed, tile_offsets, tile_order, counters,
+ * capacity, C, width, height and writes image (and image_var with the
+ * uncertainty kind); the VJP also reads v_image (v_image_var, detach) and
+ * writes isect_grads. Unused fields may be NULL / 0. */
+typedef struct r2s_raster_job {
+    const void* packed;
+    const int32_t* tile_offsets;
+    const uint32_t* tile_order;
+    const uint32_t* counters;
+    float* image;
+    float* image_var;
+    const float* v_image;
+    const float* v_image_var;
+    float* isect_grads;
+    uint32_t capacity, C, width, height;
+    int32_t detach;
+    uint32_t first_tile;
+    uint32_t reserved[2];
+} r2s_raster_job;
+int r2s_fused_render_packed(const r2s_raster_job* jobs_host, const r2s_raster_job* jobs_device, uint32_t n_jobs,
+                            uint32_t tile_size, uint32_t waves, int kind, r2s_stream_t stream);
+int r2s_fused_raster_vjp_packed(const r2s_raster_job* jobs_host, const r2s_raster_job* jobs_device,
+                                uint32_t n_jobs, uint32_t tile_size, uint32_t waves, int kind, r2s_stream_t stream);
 int r2s_fused_param_vjp(const float* means, const float* quats, const float* scales,
                         const float* densities, const float* betas, const float* beta_ratio,
                         const float* beta_log_grad, const float* viewmats, const float* Ks, uint32_t C,

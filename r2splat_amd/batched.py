@@ -6,11 +6,19 @@ autotuner.
 image size and Adam state) on one of a few streams, longest jobs first onto
 the least-loaded stream, and captures one step of every job (fused forward,
 MSE, backward, Adam) into a single graph whose streams fork from and join
-one capture stream, so jobs on different streams overlap on the device and a
-whole batched step costs one graph launch. Jobs on the same stream run one
-after another and share one ``Workspace`` (the per-intersection buffers),
-sized for the largest of them: HBM holds every job's parameters and
-optimizer state plus one workspace per stream, not one per job.
+one capture stream, so a whole batched step costs one graph launch.
+
+With ``packed`` (the default) the two raster stages of all jobs run as packed
+launches: one render and one raster-VJP launch per group of jobs that share a
+tile edge, wave count and record kind, over the concatenation of those jobs'
+tiles (``r2s_fused_render_packed`` / ``r2s_fused_raster_vjp_packed``), so
+small jobs fill the GPU together instead of one after another. The graph then
+runs every job's prepare on its stream, the packed renders, every job's loss,
+the packed raster VJPs, and every job's parameter VJP and Adam update. Each job
+keeps its own ``Workspace`` (the packed launches read all jobs' intersection
+lists at once). Without ``packed`` the jobs run their whole steps on their
+streams and jobs on the same stream share one workspace, sized for the
+largest of them.
 
 ``Autotuner`` picks the tile edge (8, 16, 32) and the waves per tile of the
 forward and backward raster kernels (4, 8, 16) for a job by timing the
@@ -30,6 +38,7 @@ the batched-fit capability built on the same operators.
 
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass
 from typing import Optional, Sequence
@@ -120,6 +129,44 @@ class Autotuner:
         return best
 
 
+class _PackedGroup:
+    """Jobs that share a tile edge, wave count and record kind: the job table
+    of their packed raster launch (host copy and device copy)."""
+
+    def __init__(self, fits, tile_size: int, waves: int, kind: int, device):
+        self.fits = list(fits)
+        self.tile_size, self.waves, self.kind = tile_size, waves, kind
+        table = (be.RasterJob * len(self.fits))()
+        first = 0
+        for entry, fit in zip(table, self.fits):
+            for name, value in fit.raster_job().items():
+                setattr(entry, name, value)
+            entry.first_tile = first
+            first += fit.C * (-(-fit.H // tile_size)) * (-(-fit.W // tile_size))
+        self.tiles = first
+        self.host = table
+        raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table))),
+                               dtype=torch.uint8)
+        self.table = raw.to(device)  # the copy the kernels read (kept alive with the graph)
+
+    def issue(self, name: str, stream):
+        call(name, ctypes.addressof(self.host), ptr(self.table), len(self.fits), self.tile_size, self.waves,
+             self.kind, stream)
+
+
+def _groups(fits, waves_of, device):
+    """Packed groups by (tile edge, waves, kind); within a group the jobs with
+    the most primitives per tile of a view come first, so each XCD (which takes
+    its band of every job in table order) ends on the cheapest tiles."""
+    groups = {}
+    for fit in fits:
+        key = (fit.tile_size, waves_of(fit) or 8, fit.kind)
+        groups.setdefault(key, []).append(fit)
+    per_tile = lambda f: f.N / ((-(-f.H // f.tile_size)) * (-(-f.W // f.tile_size)))  # noqa: E731
+    return [_PackedGroup(sorted(members, key=per_tile, reverse=True), *key, device)
+            for key, members in sorted(groups.items())]
+
+
 def _clip(job, radius_clip):
     """The explicit API's default footprint clip: 3 sigma (Gaussian), 1 (Beta support)."""
     if radius_clip is not None:
@@ -137,12 +184,15 @@ class BatchedFit:
     """
 
     def __init__(self, jobs, *, adam: Optional[AdamConfig] = AdamConfig(), n_streams: int = 4,
-                 tuner: Optional[Autotuner] = None, eps: float = 1e-8, radius_clip: Optional[float] = None):
+                 tuner: Optional[Autotuner] = None, eps: float = 1e-8, radius_clip: Optional[float] = None,
+                 packed: bool = True):
         if not jobs:
             raise ValueError("no jobs")
         self.device = jobs[0]["means"].device
+        self.packed = bool(packed)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, n_streams))]
-        self.workspaces = [Workspace(self.device) for _ in self.streams]
+        stream_ws = [Workspace(self.device) for _ in self.streams]
+        self.workspaces = [] if self.packed else stream_ws
         self.tune_records = []
         load = [0.0] * len(self.streams)
         order = sorted(range(len(jobs)), key=lambda i: -self._cost(jobs[i]))
@@ -159,11 +209,15 @@ class BatchedFit:
             cfg = cfg or KernelConfig()
             s = min(range(len(self.streams)), key=lambda k: load[k])
             load[s] += self._cost(job)
+            ws = Workspace(self.device) if self.packed else stream_ws[s]
+            if self.packed:
+                self.workspaces.append(ws)
             self.fits[i] = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"],
                                        job["Ks"], job["target"], eps=eps, radius_clip=_clip(job, radius_clip),
                                        adam=adam, capacity=job.get("capacity"), stream=self.streams[s],
                                        tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves,
-                                       workspace=self.workspaces[s], beta=job.get("beta"))
+                                       workspace=ws, beta=job.get("beta"))
+        self._render_groups = self._vjp_groups = None
         self.graph = None
         self._versions = None
         self.steps_issued = 0
@@ -173,28 +227,61 @@ class BatchedFit:
         C, H, W = job["target"].shape
         return float(job["means"].shape[0]) * C * math.sqrt(H * W)
 
+    def _state(self) -> list:
+        """What a captured graph bakes in: the workspace buffers (their versions)
+        and each job's capacity (a job can grow its capacity inside a shared
+        workspace that is already large enough)."""
+        return [ws.version for ws in self.workspaces] + [f.capacity for f in self.fits]
+
     def _stale(self) -> bool:
-        return self.graph is None or self._versions != [ws.version for ws in self.workspaces]
+        return self.graph is None or self._versions != self._state()
+
+    def _per_job(self, main, stage):
+        """Fork to the job streams, run `stage(fit)` for each job on its stream, join."""
+        for s in self.streams:
+            s.wait_stream(main)
+        for fit in self.fits:
+            with torch.cuda.stream(fit.stream):
+                stage(fit)
+        for s in self.streams:
+            main.wait_stream(s)
+
+    def _per_group(self, main, groups, name):
+        """Fork, one packed launch per group (groups dealt over the streams, so
+        one group's tail overlaps the next group), join."""
+        for s in self.streams:
+            s.wait_stream(main)
+        for i, group in enumerate(groups):
+            s = self.streams[i % len(self.streams)]
+            group.issue(name, s.cuda_stream)
+        for s in self.streams:
+            main.wait_stream(s)
 
     def capture(self):
-        """One graph: fork to the job streams, each job's step in order, join."""
+        """One graph: fork to the job streams, each job's step in order, join
+        (packed: each job's prepare, the packed renders, each job's loss, the
+        packed raster VJPs, each job's update)."""
         main = torch.cuda.Stream(self.device)
         main.wait_stream(torch.cuda.current_stream(self.device))
         for s in self.streams:  # buffers written before capture are visible to every job stream
             s.wait_stream(torch.cuda.current_stream(self.device))
+        if self.packed:  # the job tables hold the current buffers
+            self._render_groups = _groups(self.fits, lambda f: f.fwd_waves, self.device)
+            self._vjp_groups = _groups(self.fits, lambda f: f.bwd_waves, self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(main):
             with torch.cuda.graph(g, stream=main):
-                for s in self.streams:
-                    s.wait_stream(main)
-                for fit in self.fits:
-                    with torch.cuda.stream(fit.stream):
-                        fit._issue()
-                for s in self.streams:
-                    main.wait_stream(s)
+                if not self.packed:
+                    self._per_job(main, lambda fit: fit._issue())
+                else:
+                    self._per_job(main, lambda fit: fit._issue_prepare())
+                    self._per_group(main, self._render_groups, "r2s_fused_render_packed")
+                    self._per_job(main, lambda fit: fit._issue_loss())
+                    self._per_group(main, self._vjp_groups, "r2s_fused_raster_vjp_packed")
+                    self._per_job(main, lambda fit: fit._issue_update())
         torch.cuda.current_stream(self.device).wait_stream(main)
         self.graph = g
-        self._versions = [ws.version for ws in self.workspaces]
+        self._versions = self._state()
         return g
 
     def step(self, n: int = 1):

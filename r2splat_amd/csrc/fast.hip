@@ -92,6 +92,13 @@ static_assert(sizeof(BetaRec) == 96, "stored Beta record must be 96 bytes");
 template <bool BETA, bool VAR = false>
 using Stored = typename std::conditional<BETA, BetaRec, Rec<BETA, VAR>>::type;
 
+// The packed buffer the forward leaves for the raster kernels and the
+// backward: `capacity` list entries {record index, emission index} (uint2),
+// then (from a 256-byte boundary) the C*N projected records.
+__host__ __device__ constexpr size_t entries_bytes(uint32_t capacity) {
+    return (size_t(capacity) * sizeof(uint2) + 255) / 256 * 256;
+}
+
 template <typename R>
 __device__ __forceinline__ const R& head(const R& r) { return r; }
 __device__ __forceinline__ const BetaSplat& head(const BetaRec& r) { return r.s; }
@@ -950,20 +957,23 @@ __device__ __forceinline__ void settle(const R& s) {
     if constexpr (std::is_same<R, BetaSplat>::value) asm volatile("" ::"v"(s.b2d), "v"(s.unc), "v"(s.lx), "v"(s.ly));
 }
 
+// One tile of the forward: the workgroup renders the tile at position `pos`
+// of the tile order (or tile `pos` without one).
 template <int T, int WAVES, bool BETA, bool VAR>
-__global__ void __launch_bounds__(WAVES * kWave)
-fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
-                        const int32_t* __restrict__ offsets,
-                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
-                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
-                        uint32_t H, float* __restrict__ out, float* __restrict__ out_var) {
+__device__ __forceinline__ void raster_fwd_tile(const uint2* __restrict__ entries,
+                                                const Stored<BETA, VAR>* __restrict__ recs,
+                                                const int32_t* __restrict__ offsets,
+                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                                                uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx,
+                                                uint32_t nty, uint32_t W, uint32_t H, float* __restrict__ out,
+                                                float* __restrict__ out_var, uint32_t pos) {
     constexpr int PX = T * T / kWave, STEP = kWave / T;
     static_assert(PX * kWave == T * T && STEP * T == kWave, "tile must be 8, 16 or 32");
     __shared__ float part[WAVES][T * T];  // the slices' partial images (mean, then variance)
     using R = Rec<BETA, VAR>;
     __shared__ R stage[WAVES][kWave];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
+    const uint32_t tile = order ? order[pos] : pos;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -1075,17 +1085,19 @@ fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VA
 template <int T, bool BETA, bool VAR>
 constexpr int bwd_waves_per_eu() { return (T == 16 && !BETA && !VAR) ? 5 : 1; }
 
+// One tile of the backward (the tile at position `pos` of the tile order).
 template <int T, int WAVES, bool BETA, bool VAR>
-__global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
-fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
-                        const int32_t* __restrict__ offsets,
-                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
-                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
-                        uint32_t H, const float* __restrict__ v_img, const float* __restrict__ v_var, int detach,
-                        float* __restrict__ isect_grads) {
+__device__ __forceinline__ void raster_bwd_tile(const uint2* __restrict__ entries,
+                                                const Stored<BETA, VAR>* __restrict__ recs,
+                                                const int32_t* __restrict__ offsets,
+                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                                                uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx,
+                                                uint32_t nty, uint32_t W, uint32_t H, const float* __restrict__ v_img,
+                                                const float* __restrict__ v_var, int detach,
+                                                float* __restrict__ isect_grads, uint32_t pos) {
     __shared__ __align__(16) float sv[VAR ? 2 : 1][T * T];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t tile = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
+    const uint32_t tile = order ? order[pos] : pos;
     const uint32_t n_live = live(d_n, capacity);
     int32_t start, end;
     fused_tile(offsets, tile, n_tiles_total, n_live, start, end);
@@ -1251,6 +1263,102 @@ fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VA
             }
         }
     }
+}
+
+// One job: tile at the XCD-banded position of this workgroup.
+template <int T, int WAVES, bool BETA, bool VAR>
+__global__ void __launch_bounds__(WAVES * kWave)
+fused_raster_fwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
+                        const int32_t* __restrict__ offsets,
+                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
+                        uint32_t H, float* __restrict__ out, float* __restrict__ out_var) {
+    raster_fwd_tile<T, WAVES, BETA, VAR>(entries, recs, offsets, order, d_n, capacity, n_tiles_total, ntx, nty, W, H,
+                                         out, out_var, order ? xcd_slot(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+template <int T, int WAVES, bool BETA, bool VAR>
+__global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
+fused_raster_bwd_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR>* __restrict__ recs,
+                        const int32_t* __restrict__ offsets,
+                        const uint32_t* __restrict__ order, const uint32_t* __restrict__ d_n,
+                        uint32_t capacity, uint32_t n_tiles_total, uint32_t ntx, uint32_t nty, uint32_t W,
+                        uint32_t H, const float* __restrict__ v_img, const float* __restrict__ v_var, int detach,
+                        float* __restrict__ isect_grads) {
+    raster_bwd_tile<T, WAVES, BETA, VAR>(entries, recs, offsets, order, d_n, capacity, n_tiles_total, ntx, nty, W, H,
+                                         v_img, v_var, detach, isect_grads,
+                                         order ? xcd_slot(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Packed launches: the tiles of several jobs (independent reconstructions of
+// any size with the same tile edge, waves and record kind) in one launch.
+// Job j's n_j tiles are jobs[j].first_tile .. first_tile + n_j - 1 of the
+// concatenation. Each XCD x takes band x of every job (the band its own launch
+// would give XCD x: per_j + (x < rem_j) positions from x per_j + min(x,
+// rem_j), per_j = n_j / 8, rem_j = n_j % 8), jobs in table order, so the
+// per-tile cost of different jobs (which differs by orders of magnitude)
+// spreads evenly over the XCDs. Workgroup b (XCD b % 8) takes item b / 8 of
+// its XCD's sequence; the grid is 8 x the longest sequence and workgroups past
+// the end of theirs exit. The job is found by a scan of the table
+// (wave-uniform: scalar loads).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t packed_band(uint32_t n, uint32_t x) { return n / 8u + (x < n % 8u ? 1u : 0u); }
+
+// The job of workgroup b (n_jobs: none) and its position in that job's tile order.
+__device__ __forceinline__ uint32_t packed_job(const r2s_raster_job* __restrict__ jobs, uint32_t n_jobs,
+                                               uint32_t total, uint32_t b, uint32_t& pos) {
+    const uint32_t x = b % 8u;
+    uint32_t k = b / 8u;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const uint32_t n = (j + 1 < n_jobs ? jobs[j + 1].first_tile : total) - jobs[j].first_tile;
+        const uint32_t size = packed_band(n, x);
+        if (k < size) {
+            pos = x * (n / 8u) + min(x, n % 8u) + k;
+            return j;
+        }
+        k -= size;
+    }
+    return n_jobs;
+}
+
+template <int T>
+__device__ __forceinline__ void packed_grid(const r2s_raster_job& J, uint32_t& ntx, uint32_t& nty, uint32_t& nt) {
+    ntx = (J.width + T - 1) / T;
+    nty = (J.height + T - 1) / T;
+    nt = J.C * ntx * nty;
+}
+
+template <int T, int WAVES, bool BETA, bool VAR>
+__global__ void __launch_bounds__(WAVES * kWave)
+fused_raster_fwd_packed_kernel(const r2s_raster_job* __restrict__ jobs, uint32_t n_jobs, uint32_t total) {
+    uint32_t pos = 0;
+    const uint32_t j = packed_job(jobs, n_jobs, total, blockIdx.x, pos);
+    if (j == n_jobs) return;  // past the end of this XCD's sequence (uniform over the workgroup)
+    const r2s_raster_job& J = jobs[j];
+    uint32_t ntx, nty, nt;
+    packed_grid<T>(J, ntx, nty, nt);
+    const auto* recs = reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(J.packed) +
+                                                                   entries_bytes(J.capacity));
+    raster_fwd_tile<T, WAVES, BETA, VAR>(static_cast<const uint2*>(J.packed), recs, J.tile_offsets, J.tile_order,
+                                         J.counters, J.capacity, nt, ntx, nty, J.width, J.height, J.image,
+                                         J.image_var, pos);
+}
+
+template <int T, int WAVES, bool BETA, bool VAR>
+__global__ void __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(bwd_waves_per_eu<T, BETA, VAR>())))
+fused_raster_bwd_packed_kernel(const r2s_raster_job* __restrict__ jobs, uint32_t n_jobs, uint32_t total) {
+    uint32_t pos = 0;
+    const uint32_t j = packed_job(jobs, n_jobs, total, blockIdx.x, pos);
+    if (j == n_jobs) return;
+    const r2s_raster_job& J = jobs[j];
+    uint32_t ntx, nty, nt;
+    packed_grid<T>(J, ntx, nty, nt);
+    const auto* recs = reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(J.packed) +
+                                                                   entries_bytes(J.capacity));
+    raster_bwd_tile<T, WAVES, BETA, VAR>(static_cast<const uint2*>(J.packed), recs, J.tile_offsets, J.tile_order,
+                                         J.counters, J.capacity, nt, ntx, nty, J.width, J.height, J.v_image,
+                                         J.v_image_var, J.detach, J.isect_grads, pos);
 }
 
 // ---------------------------------------------------------------------------
@@ -1498,16 +1606,12 @@ constexpr size_t kPermutedFloats = 14;
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 // kind: bit 0 = Beta primitives, bit 1 = primitives with an uncertainty
-// (variance output); either selects the 48-byte record.
+// (variance output): 32-byte records, 48 with an uncertainty, 96 for Beta.
 constexpr int kKindBeta = 1, kKindVar = 2;
 static size_t record_bytes(int kind) {
     return (kind & kKindBeta) ? sizeof(BetaRec) : (kind & kKindVar) ? sizeof(BetaSplat) : sizeof(Splat);
 }
 
-// The packed buffer the forward leaves for the raster kernels and the
-// backward: `capacity` list entries {record index, emission index} (uint2),
-// then the C*N projected records.
-static size_t entries_bytes(uint32_t capacity) { return al(size_t(capacity) * sizeof(uint2)); }
 static size_t packed_bytes(uint32_t C, uint32_t N, uint32_t capacity, int kind) {
     return entries_bytes(capacity) + al(size_t(C) * N * record_bytes(kind));
 }
@@ -1639,6 +1743,24 @@ struct LaunchBwd {
             static_cast<const uint2*>(packed),
             reinterpret_cast<const Stored<BETA, VAR>*>(static_cast<const char*>(packed) + entries_bytes(capacity)),
             offsets, order, counters, capacity, n_tiles, ntx, nty, W, H, v_image, v_image_var, detach, isect_grads);
+    }
+};
+
+template <int T, int WAVES, bool BETA, bool VAR>
+struct LaunchFwdPacked {
+    static void run(uint32_t n_groups, hipStream_t stream, const r2s_raster_job* jobs, uint32_t n_jobs,
+                    uint32_t total) {
+        fused_raster_fwd_packed_kernel<T, WAVES, BETA, VAR><<<n_groups, WAVES * kWave, 0, stream>>>(jobs, n_jobs,
+                                                                                                      total);
+    }
+};
+
+template <int T, int WAVES, bool BETA, bool VAR>
+struct LaunchBwdPacked {
+    static void run(uint32_t n_groups, hipStream_t stream, const r2s_raster_job* jobs, uint32_t n_jobs,
+                    uint32_t total) {
+        fused_raster_bwd_packed_kernel<T, WAVES, BETA, VAR><<<n_groups, WAVES * kWave, 0, stream>>>(jobs, n_jobs,
+                                                                                                      total);
     }
 };
 
@@ -1827,6 +1949,57 @@ extern "C" int r2s_fused_raster_vjp(const void* packed, const int32_t* tile_offs
     launch_tiled<LaunchBwd>(tile_size, waves, kind, uint32_t(n_tiles), stream, packed, tile_offsets, tile_order,
                             counters, capacity, ntx, nty, width, height, v_image, v_image_var, detach, isect_grads);
     return check_launch("fused_raster_bwd");
+}
+
+// Checks a packed job table (host copy); returns the total tile count and the
+// grid (8 x the longest per-XCD sequence of job bands).
+static int check_packed(const r2s_raster_job* jobs, const r2s_raster_job* jobs_device, uint32_t n_jobs,
+                        uint32_t tile_size, uint32_t waves, int kind, bool vjp, uint32_t* total,
+                        uint32_t* groups) {
+    if (int rc = check_tile(tile_size, waves)) return rc;
+    R2S_REQUIRE(jobs && jobs_device, "packed: both job tables are required");
+    R2S_REQUIRE(n_jobs >= 1 && n_jobs <= 1024, "packed: 1 to 1024 jobs (got %u)", n_jobs);
+    uint64_t t = 0, seq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const r2s_raster_job& J = jobs[j];
+        R2S_REQUIRE(J.first_tile == t, "packed: job %u starts at tile %u, expected %llu", j, J.first_tile,
+                    (unsigned long long)t);
+        R2S_REQUIRE(J.packed && J.tile_offsets && J.counters && J.capacity > 0, "packed: job %u lacks its buffers", j);
+        if (vjp)
+            R2S_REQUIRE(J.v_image && J.isect_grads, "packed: job %u lacks v_image or isect_grads", j);
+        else
+            R2S_REQUIRE(J.image && (!(kind & kKindVar) || J.image_var), "packed: job %u lacks its image", j);
+        const uint64_t ntx = (J.width + tile_size - 1) / tile_size, nty = (J.height + tile_size - 1) / tile_size;
+        const uint64_t n = uint64_t(J.C) * ntx * nty;
+        for (uint32_t x = 0; x < 8; ++x) seq[x] += n / 8 + (x < n % 8 ? 1 : 0);
+        t += n;
+    }
+    R2S_REQUIRE(t + 8ull * n_jobs < (1ull << 31), "packed: too many tiles for one launch");
+    *total = uint32_t(t);
+    *groups = 8u * uint32_t(*std::max_element(seq, seq + 8));
+    return R2S_OK;
+}
+
+extern "C" int r2s_fused_render_packed(const r2s_raster_job* jobs_host, const r2s_raster_job* jobs_device,
+                                       uint32_t n_jobs, uint32_t tile_size, uint32_t waves, int kind,
+                                       r2s_stream_t stream) {
+    uint32_t total = 0, groups = 0;
+    if (int rc = check_packed(jobs_host, jobs_device, n_jobs, tile_size, waves, kind, false, &total, &groups))
+        return rc;
+    if (total == 0) return R2S_OK;
+    launch_tiled<LaunchFwdPacked>(tile_size, waves, kind, groups, stream, jobs_device, n_jobs, total);
+    return check_launch("fused_raster_fwd_packed");
+}
+
+extern "C" int r2s_fused_raster_vjp_packed(const r2s_raster_job* jobs_host, const r2s_raster_job* jobs_device,
+                                           uint32_t n_jobs, uint32_t tile_size, uint32_t waves, int kind,
+                                           r2s_stream_t stream) {
+    uint32_t total = 0, groups = 0;
+    if (int rc = check_packed(jobs_host, jobs_device, n_jobs, tile_size, waves, kind, true, &total, &groups))
+        return rc;
+    if (total == 0) return R2S_OK;
+    launch_tiled<LaunchBwdPacked>(tile_size, waves, kind, groups, stream, jobs_device, n_jobs, total);
+    return check_launch("fused_raster_bwd_packed");
 }
 
 extern "C" int r2s_fused_keep_masks(const void* packed, const int32_t* tile_offsets, const uint32_t* counters,

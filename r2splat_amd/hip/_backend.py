@@ -25,6 +25,17 @@ F = ctypes.c_float
 D = ctypes.c_double
 SZ = ctypes.c_size_t
 
+
+
+class RasterJob(ctypes.Structure):
+    """One entry of a packed render / raster-VJP table (r2s_raster_job)."""
+
+    _fields_ = [("packed", P), ("tile_offsets", P), ("tile_order", P), ("counters", P), ("image", P),
+                ("image_var", P), ("v_image", P), ("v_image_var", P), ("isect_grads", P), ("capacity", U32),
+                ("C", U32), ("width", U32), ("height", U32), ("detach", ctypes.c_int32), ("first_tile", U32),
+                ("reserved", U32 * 2)]
+
+
 # name -> argument types (return type is int status unless listed in _RESTYPES)
 SIGNATURES = {
     "r2s_abi_version": [],
@@ -64,6 +75,8 @@ SIGNATURES = {
     "r2s_fused_prepare": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, U32, F, F, U32, P, SZ, P, P, P, P, P, P],
     "r2s_fused_render": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P],
     "r2s_fused_raster_vjp": [P, P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, I32, P, P],
+    "r2s_fused_render_packed": [P, P, U32, U32, U32, I32, P],
+    "r2s_fused_raster_vjp_packed": [P, P, U32, U32, U32, I32, P],
     "r2s_fused_keep_masks": [P, P, P, U32, U32, U32, U32, U32, I32, P, P, P],
     "r2s_fused_param_workspace_bytes": [U32, U32],
     "r2s_fused_param_vjp": [P, P, P, P, P, P, P, P, P, U32, U32, U32, U32, F, P, P, U32, P, P, SZ, P, P, P, P, P, P,

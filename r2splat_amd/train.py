@@ -189,27 +189,56 @@ class GaussianFit:
         return self.resident_bytes() + self.ws.nbytes()
 
     # -- one step -------------------------------------------------------------
-    def _issue(self, with_loss: bool = True):
-        """Enqueues one step on the current stream (no host synchronisation)."""
+    # The step in stages (each enqueued on the current stream): prepare, render,
+    # loss, raster VJP, update (parameter VJP, Adam, domain clamp). A batch of
+    # jobs may run the render and the raster VJP of many jobs as one packed
+    # launch (``raster_job``) between the other stages of each job.
+    def _issue_prepare(self):
         s = be.stream_of(self.device)
-        C, N, W, H = self.C, self.N, self.W, self.H
         ws = self.ws
         if self.kind:  # line-integral factors of the current betas (float64 math, as the explicit path)
             self.beta_ratio.copy_(beta_path_ratio(self.beta))
             self.beta_log_grad.copy_(beta_path_log_grad(self.beta))
         call("r2s_fused_prepare", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
-             ptr(self.beta), ptr(self.beta_ratio), None, ptr(self.viewmats), ptr(self.Ks), C, N, W, H, self.tile_size,
-             self.eps, self.radius_clip, self.capacity,
+             ptr(self.beta), ptr(self.beta_ratio), None, ptr(self.viewmats), ptr(self.Ks), self.C, self.N, self.W,
+             self.H, self.tile_size, self.eps, self.radius_clip, self.capacity,
              ptr(ws.scratch), ws.scratch.numel(), ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
              ptr(self.record_offsets), ptr(self.counters), s)
-        call("r2s_fused_render", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order), ptr(self.counters),
-             self.capacity, C, W, H, self.tile_size, self.fwd_waves, self.kind, ptr(self.image), None, s)
+
+    def _issue_render(self):
+        call("r2s_fused_render", ptr(self.ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
+             ptr(self.counters), self.capacity, self.C, self.W, self.H, self.tile_size, self.fwd_waves, self.kind,
+             ptr(self.image), None, be.stream_of(self.device))
+
+    def _issue_loss(self, with_loss: bool = True):
         call("r2s_mse_adjoint", ptr(self.image), ptr(self.target), self.image.numel(), ptr(self.v_image),
-             ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None, s)
-        call("r2s_fused_raster_vjp", ptr(ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
-             ptr(self.counters), self.capacity, C, W, H, self.tile_size, self.bwd_waves, self.kind, ptr(self.v_image),
-             None, 0,
-             ptr(ws.isect_grads), s)
+             ptr(self.loss_partials) if with_loss else None, ptr(self.loss) if with_loss else None,
+             be.stream_of(self.device))
+
+    def _issue_raster_vjp(self):
+        call("r2s_fused_raster_vjp", ptr(self.ws.packed), ptr(self.tile_offsets), ptr(self.tile_order),
+             ptr(self.counters), self.capacity, self.C, self.W, self.H, self.tile_size, self.bwd_waves, self.kind,
+             ptr(self.v_image), None, 0, ptr(self.ws.isect_grads), be.stream_of(self.device))
+
+    def raster_job(self) -> dict:
+        """This job's entry of a packed render / raster-VJP table (r2s_raster_job fields)."""
+        return dict(packed=ptr(self.ws.packed), tile_offsets=ptr(self.tile_offsets), tile_order=ptr(self.tile_order),
+                    counters=ptr(self.counters), image=ptr(self.image), v_image=ptr(self.v_image),
+                    isect_grads=ptr(self.ws.isect_grads), capacity=self.capacity, C=self.C, width=self.W,
+                    height=self.H)
+
+    def _issue(self, with_loss: bool = True):
+        """Enqueues one step on the current stream (no host synchronisation)."""
+        self._issue_prepare()
+        self._issue_render()
+        self._issue_loss(with_loss)
+        self._issue_raster_vjp()
+        self._issue_update()
+
+    def _issue_update(self):
+        s = be.stream_of(self.device)
+        C, N, W, H = self.C, self.N, self.W, self.H
+        ws = self.ws
         call("r2s_fused_param_vjp", ptr(self.means), ptr(self.quats), ptr(self.scales), ptr(self.densities),
              ptr(self.beta), ptr(self.beta_ratio), ptr(self.beta_log_grad), ptr(self.viewmats), ptr(self.Ks), C, N,
              W, H, self.eps, ptr(self.record_offsets), ptr(self.counters), self.capacity, ptr(ws.isect_grads),

@@ -1,14 +1,19 @@
 """Batched graph-captured fits (r2splat_amd.batched): every job's result in the
-batch equals the job fitted alone (bitwise), whatever stream and shared
-workspace it landed on; per-job kernel configurations; the autotuner; a job
-overflowing its capacity inside the batch is repaired without disturbing
-the others."""
+batch equals the job fitted alone (bitwise), with the raster stages packed
+into shared launches or each job's step on its stream (sharing a workspace);
+per-job kernel configurations; the autotuner; a job overflowing its capacity
+inside the batch is repaired without disturbing the others; packed raster
+launches equal each job's own launches bitwise."""
+
+import ctypes
 
 import pytest
 import torch
 
-from r2splat_amd import workloads
+from r2splat_amd import fused, workloads
 from r2splat_amd.batched import Autotuner, BatchedFit, KernelConfig
+from r2splat_amd.hip import _backend as be
+from r2splat_amd.hip._backend import call, ptr
 from r2splat_amd.train import AdamConfig, GaussianFit
 
 pytestmark = pytest.mark.gpu
@@ -39,10 +44,11 @@ def _alone(job, adam, steps):
     return fit
 
 
-def test_batched_jobs_equal_jobs_fitted_alone(device):
+@pytest.mark.parametrize("packed", [True, False])
+def test_batched_jobs_equal_jobs_fitted_alone(device, packed):
     adam = AdamConfig(lr=1e-3)
     jobs = _jobs(device)
-    batch = BatchedFit(jobs, adam=adam, n_streams=2)
+    batch = BatchedFit(jobs, adam=adam, n_streams=2, packed=packed)
     batch.step(3)
     assert batch.sync() == 3
     for job, fit in zip(jobs, batch.fits):
@@ -52,10 +58,11 @@ def test_batched_jobs_equal_jobs_fitted_alone(device):
         assert torch.equal(fit.loss, ref.loss)
 
 
-def test_batched_overflow_in_one_job_is_repaired(device):
+@pytest.mark.parametrize("packed", [True, False])
+def test_batched_overflow_in_one_job_is_repaired(device, packed):
     adam = AdamConfig(lr=1e-3)
     jobs = _jobs(device, capacities=[None, 1024, None, None])
-    batch = BatchedFit(jobs, adam=adam, n_streams=2)
+    batch = BatchedFit(jobs, adam=adam, n_streams=2, packed=packed)
     batch.step(2)
     assert batch.sync() == 2
     assert batch.fits[1].capacity > 1024
@@ -103,3 +110,55 @@ def test_mixed_gaussian_and_beta_batch_equals_jobs_alone(device):
         ref.step(3)
         assert ref.sync() == 3
         assert torch.equal(fit.params, ref.params)
+
+
+@pytest.mark.parametrize("use_beta", [False, True])
+def test_packed_raster_launches_equal_each_jobs_own(device, use_beta):
+    """Four jobs of different views, sizes and image shapes (one of them
+    overflowed, rendering nothing) in one packed render and one packed raster
+    VJP: every image and every intersection partial equals the job's own
+    launch bitwise."""
+    specs = [(900, 3, 48, 40), (400, 2, 64, 64), (1500, 4, 40, 72), (250, 5, 56, 24)]
+    T, waves, kind = 16, 8, int(use_beta)
+    stream = be.stream_of(device)
+    preps, table = [], (be.RasterJob * len(specs))()
+    first = 0
+    for k, (n, c, h, w) in enumerate(specs):
+        sc = workloads.scene(n, c, h, w, scale_range=(0.01, 0.08), seed=30 + k, device=device,
+                             beta_range=(0.5, 4.0) if use_beta else None)
+        f = fused.prepare(*sc.params(), sc.viewmats, sc.Ks, w, h, 1e-8, 1.0 if use_beta else 3.0, tile_size=T,
+                          beta=sc.beta, capacity=64 if k == 2 else None, check=k != 2)
+        g = torch.Generator(device=device).manual_seed(k)
+        v = torch.randn((c, h, w), generator=g, device=device)
+        mine = (torch.full((c, h, w), float("nan"), device=device), torch.full((f.capacity * 8,), float("nan"),
+                                                                            device=device))
+        packed_out = (torch.full_like(mine[0], float("nan")), torch.full_like(mine[1], float("nan")))
+        call("r2s_fused_render", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters), f.capacity, c,
+             w, h, T, waves, kind, ptr(mine[0]), None, stream)
+        call("r2s_fused_raster_vjp", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters),
+             f.capacity, c, w, h, T, waves, kind, ptr(v), None, 0, ptr(mine[1]), stream)
+        e = table[k]
+        e.packed, e.tile_offsets, e.tile_order, e.counters = ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), \
+            ptr(f.counters)
+        e.image, e.v_image, e.isect_grads = ptr(packed_out[0]), ptr(v), ptr(packed_out[1])
+        e.capacity, e.C, e.width, e.height, e.first_tile = f.capacity, c, w, h, first
+        first += c * (-(-h // T)) * (-(-w // T))
+        preps.append((f, mine, packed_out, v))  # the table points into every job's buffers: keep them
+    assert int(preps[2][0].counters[1].item()) == 1  # the overflowed job
+    dev_table = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table))),
+                                 dtype=torch.uint8).to(device)
+    call("r2s_fused_render_packed", ctypes.addressof(table), ptr(dev_table), len(specs), T, waves, kind, stream)
+    call("r2s_fused_raster_vjp_packed", ctypes.addressof(table), ptr(dev_table), len(specs), T, waves, kind, stream)
+    torch.cuda.synchronize(device)
+    bits = lambda t: t.view(torch.int32)  # noqa: E731 (bitwise, NaN fill included)
+    for k, (f, mine, packed_out, _) in enumerate(preps):
+        assert torch.equal(bits(packed_out[0]), bits(mine[0])), k
+        # slots of footprint-rectangle tiles the ellipse does not reach are never
+        # written: both launches leave the same NaN fill there
+        n = int(f.counters[0].item()) if k != 2 else 0
+        assert torch.equal(bits(packed_out[1][: 8 * n]), bits(mine[1][: 8 * n])), k
+    assert not bool(preps[2][2][0].any())  # an overflowed job renders zeros
+    # a table whose first_tile prefix is wrong is refused before any launch
+    table[1].first_tile += 1
+    with pytest.raises(RuntimeError, match="starts at tile"):
+        call("r2s_fused_render_packed", ctypes.addressof(table), ptr(dev_table), len(specs), T, waves, kind, stream)
