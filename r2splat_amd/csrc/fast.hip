@@ -153,10 +153,13 @@ __device__ __forceinline__ double beta_ratio_f64(double beta) {
 
 // The Beta cut-off in double: the largest d^2 with |w| (1 - d^2)^b2d >= 1e-5
 // (-1 when |w| < 1e-5), and the qlim of its float32 rounding.
-__device__ __forceinline__ double beta_dmax(double w, double b2d) {
+// Takes inv_b2d = 1 / b2d (NaN when b2d <= 0) and forms the bound as
+// 1 - 2^((log2 1e-5 - log2 |w|) inv_b2d).
+__device__ __forceinline__ double beta_dmax_inv(double w, double inv_b2d) {
+    constexpr double kLog2Thr = -16.609640474436812;  // log2(1e-5)
     const double aw = fabs(w);
-    if (!(aw >= 1e-5) || !(b2d > 0.0)) return -1.0;
-    return 1.0 - pow(1e-5 / aw, 1.0 / b2d);
+    if (!(aw >= 1e-5) || !(inv_b2d > 0.0)) return -1.0;
+    return 1.0 - exp2((kLog2Thr - log2(aw)) * inv_b2d);
 }
 __device__ __forceinline__ uint32_t beta_qlim_from(double dmax) {
     if (!(dmax >= 0.0)) return 0u;
@@ -392,51 +395,80 @@ morton_keys_kernel(const float* __restrict__ means, uint32_t N, const uint32_t* 
     vals[n] = n;
 }
 
-// Copies of the primitive parameters in the sorted order (perm[p] = input index).
+// The camera-independent double-precision terms of one primitive's
+// projection: its 3D covariance (00, 01, 02, 11, 12, 22) from the normalised
+// quaternion and the scales, as the float64 parameter transform computes and
+// stores it, and (Beta) the line-integral ratio B(1/2, beta + 1) / sqrt(2 pi)
+// and 1 / b2d (b2d = beta + 1/2; NaN when b2d <= 0) in double. 64 bytes.
+struct PrimFrame {
+    double cov[6];
+    double ratio;
+    double inv_b2d;
+};
+
+// One camera in double (exact from its float32 viewmat and K) with its
+// field-of-view limits, computed once per camera for the projection.
+struct CamFrame {
+    Camera<double> cam;
+    FovLimits<double> fov;
+};
+
+__global__ void __launch_bounds__(64)
+camera_frames_kernel(const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t C, uint32_t W,
+                     uint32_t H, CamFrame* __restrict__ out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    CamFrame f;
+    f.cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
+    f.fov = fov_limits(f.cam, double(W), double(H));
+    out[c] = f;
+}
+
+// Copies of the primitive parameters in the sorted order (perm[p] = input
+// index), and each primitive's PrimFrame.
 __global__ void __launch_bounds__(256)
 permute_params_kernel(const uint32_t* __restrict__ perm, uint32_t N, const float* __restrict__ means,
                       const float* __restrict__ quats, const float* __restrict__ scales,
                       const float* __restrict__ dens, const float* __restrict__ betas,
                       const float* __restrict__ ratio, const float* __restrict__ unc, float* __restrict__ o_means,
-                      float* __restrict__ o_quats, float* __restrict__ o_scales, float* __restrict__ o_dens,
-                      float* __restrict__ o_betas, float* __restrict__ o_ratio, float* __restrict__ o_unc) {
+                      float* __restrict__ o_dens, float* __restrict__ o_betas, float* __restrict__ o_ratio,
+                      float* __restrict__ o_unc, PrimFrame* __restrict__ o_frames) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const size_t n = perm[p];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        o_means[3 * size_t(p) + k] = means[3 * n + k];
-        o_scales[3 * size_t(p) + k] = scales[3 * n + k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o_quats[4 * size_t(p) + k] = quats[4 * n + k];
+    for (int k = 0; k < 3; ++k) o_means[3 * size_t(p) + k] = means[3 * n + k];
     o_dens[p] = dens[n];
     if (betas) {
         o_betas[p] = betas[n];
         o_ratio[p] = ratio[n];
     }
     if (unc) o_unc[p] = unc[n];
+    const double raw[4] = {double(quats[4 * n]), double(quats[4 * n + 1]), double(quats[4 * n + 2]),
+                           double(quats[4 * n + 3])};
+    double q[4], mag, norm, R[9];
+    unit_quat(raw, q, mag, norm);
+    quat_to_rot(q, R);
+    const double s[3] = {double(scales[3 * n]), double(scales[3 * n + 1]), double(scales[3 * n + 2])};
+    PrimFrame f;
+    covariance_from(R, s, f.cov);
+    const double b2d = betas ? double(betas[n]) + 0.5 : 0.0;
+    f.ratio = betas ? beta_ratio_f64(double(betas[n])) : 0.0;
+    f.inv_b2d = b2d > 0.0 ? 1.0 / b2d : __builtin_nan("");
+    o_frames[p] = f;
 }
 
 // Projected mean and 2D covariance of one primitive for one camera in double
-// precision (its float32 parameters are exact in double): the operator chain
-// parameter transform -> world-to-image projection evaluated in double.
-// Returns false when the primitive is culled (depth <= 0.01).
-__device__ __forceinline__ bool project_f64(const float* __restrict__ viewmat, const float* __restrict__ K,
-                                         const float* __restrict__ mean, const float* __restrict__ quat,
-                                         const float* __restrict__ scale, uint32_t W, uint32_t H, double m2[2],
-                                         double c2[6]) {
-    Camera<double> cam;
-    cam.load(viewmat, K);
-    const double raw[4] = {double(quat[0]), double(quat[1]), double(quat[2]), double(quat[3])};
-    double q[4], mag, norm, R[9], cov[6];
-    unit_quat(raw, q, mag, norm);
-    quat_to_rot(q, R);
-    const double s[3] = {double(scale[0]), double(scale[1]), double(scale[2])};
-    covariance_from(R, s, cov);
+// precision (its float32 mean is exact in double; cov is its PrimFrame
+// covariance): the operator chain parameter transform -> world-to-image
+// projection evaluated in double. Returns false when the primitive is culled
+// (depth <= 0.01).
+__device__ __forceinline__ bool project_f64(const CamFrame& cf, const float* __restrict__ mean, const double cov[6],
+                                         double m2[2], double c2[6]) {
+    const Camera<double>& cam = cf.cam;
     const double p[3] = {double(mean[0]), double(mean[1]), double(mean[2])};
     ProjState<double> st;
-    if (!project_state(cam, p, cov, double(W), double(H), st)) return false;
+    if (!project_state(cam, cf.fov, p, cov, st)) return false;
     m2[0] = cam.fx * st.x / st.z + cam.cx;
     m2[1] = cam.fy * st.y / st.z + cam.cy;
     projected_cov(st, c2);
@@ -444,80 +476,88 @@ __device__ __forceinline__ bool project_f64(const float* __restrict__ viewmat, c
 }
 
 // ---------------------------------------------------------------------------
-// Fused projection + tile count. grid (ceil(N/256), C). Each (camera,
-// primitive) is projected in double (project_f64: culling, radii, tile
-// rectangle, conic and mu as the operators compute them in float64) and its
-// record stores the results rounded to float32; Beta records also keep their
-// mean's float32 remainder.
+// Fused projection + tile count. grid (ceil(N/256), ceil(C/kProjectCams)):
+// each thread takes one primitive (its PrimFrame and parameters loaded once)
+// for kProjectCams consecutive cameras. Each (camera, primitive) is projected
+// in double (project_f64: culling, radii, tile rectangle, conic and mu as the
+// operators compute them in float64) and its record stores the results
+// rounded to float32; Beta records also keep their mean's float32 remainder
+// and the cut-off in double.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kProjectCams = 4;
+
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(256)
-fused_project_kernel(const float* __restrict__ means, const float* __restrict__ quats,
-                     const float* __restrict__ scales, const float* __restrict__ dens,
-                     const float* __restrict__ betas, const float* __restrict__ ratio,
-                     const float* __restrict__ unc, const float* __restrict__ viewmats,
-                     const float* __restrict__ Ks, uint32_t W, uint32_t H, float eps, float clip,
-                     uint32_t tile_size, uint32_t ntx, uint32_t nty, uint32_t N,
+fused_project_kernel(const float* __restrict__ means, const PrimFrame* __restrict__ frames,
+                     const float* __restrict__ dens, const float* __restrict__ betas,
+                     const float* __restrict__ ratio, const float* __restrict__ unc,
+                     const CamFrame* __restrict__ cams, uint32_t C, uint32_t W,
+                     uint32_t H, float eps, float clip, uint32_t tile_size, uint32_t ntx, uint32_t nty, uint32_t N,
                      Stored<BETA, VAR>* __restrict__ recs, uint2* __restrict__ rects, float2* __restrict__ reach,
                      uint32_t* __restrict__ counts) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = blockIdx.y;
     if (n >= N) return;
-    const size_t idx = size_t(c) * N + n;
-    Rec<BETA, VAR> rec = {};
-    BetaEdge edge = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0};
-    rec.g = int32_t(idx);
-    if constexpr (VAR) rec.unc = unc[n];
-    uint2 rect = make_uint2(0u, 0u);
-    float2 rr = make_float2(0.f, -1.f);
-    uint32_t count = 0;
-    double m2[2], c2[6];
-    if (project_f64(viewmats + 16 * size_t(c), Ks + 9 * size_t(c), means + 3 * size_t(n), quats + 4 * size_t(n),
-                    scales + 3 * size_t(n), W, H, m2, c2)) {
-        int32_t rx, ry;
-        footprint_radii(c2, m2[0], m2[1], double(eps), double(clip), double(W), double(H), rx, ry);
-        double P[3], mu;
-        conic_mu_f64(c2, double(eps), P, mu);
-        rec.mx = float(m2[0]);
-        rec.my = float(m2[1]);
-        if constexpr (BETA) {
-            rec.lx = float(m2[0] - double(rec.mx));
-            rec.ly = float(m2[1] - double(rec.my));
-            const float mus = float(mu) * ratio[n];  // mu_beta = mu ratio(beta), in float32
-            const float w = dens[n] * mus;
-            rec.A = float(P[0]);
-            rec.B = 2.f * float(P[1]);
-            rec.C = float(P[2]);
-            rec.w = w;
-            rec.b2d = betas[n] + 0.5f;
-            // the cut-off in double (w = density mu r(beta) and b2d as float64 forms them)
-            const double dmax = beta_dmax(double(dens[n]) * mu * beta_ratio_f64(double(betas[n])),
-                                          double(betas[n]) + 0.5);
-            rec.qlim = beta_qlim_from(dmax);
-            edge = BetaEdge{m2[0], m2[1], P[0], P[1], P[2], dmax};
-        } else {
-            const float w = dens[n] * float(mu);
-            rec.A = 0.5f * float(P[0]) * kLog2e;
-            rec.B = float(P[1]) * kLog2e;
-            rec.C = 0.5f * float(P[2]) * kLog2e;
-            rec.w = w;
-            rec.qlim = qlim_of(w);
+    const PrimFrame f = frames[n];
+    const float density = dens[n];
+    const float beta = BETA ? betas[n] : 0.f, beta_ratio = BETA ? ratio[n] : 0.f;
+    const float uncertainty = VAR ? unc[n] : 0.f;
+    const uint32_t c_end = min(C, (blockIdx.y + 1u) * kProjectCams);
+#pragma unroll 1
+    for (uint32_t c = blockIdx.y * kProjectCams; c < c_end; ++c) {
+        const size_t idx = size_t(c) * N + n;
+        Rec<BETA, VAR> rec = {};
+        BetaEdge edge = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0};
+        rec.g = int32_t(idx);
+        if constexpr (VAR) rec.unc = uncertainty;
+        uint2 rect = make_uint2(0u, 0u);
+        float2 rr = make_float2(0.f, -1.f);
+        uint32_t count = 0;
+        double m2[2], c2[6];
+        if (project_f64(cams[c], means + 3 * size_t(n), f.cov, m2, c2)) {
+            int32_t rx, ry;
+            footprint_radii(c2, m2[0], m2[1], double(eps), double(clip), double(W), double(H), rx, ry);
+            double P[3], mu;
+            conic_mu_fast_f64(c2, double(eps), P, mu);
+            rec.mx = float(m2[0]);
+            rec.my = float(m2[1]);
+            if constexpr (BETA) {
+                rec.lx = float(m2[0] - double(rec.mx));
+                rec.ly = float(m2[1] - double(rec.my));
+                const float mus = float(mu) * beta_ratio;  // mu_beta = mu ratio(beta), in float32
+                const float w = density * mus;
+                rec.A = float(P[0]);
+                rec.B = 2.f * float(P[1]);
+                rec.C = float(P[2]);
+                rec.w = w;
+                rec.b2d = beta + 0.5f;
+                // the cut-off in double (w = density mu r(beta) and b2d as float64 forms them)
+                const double dmax = beta_dmax_inv(double(density) * mu * f.ratio, f.inv_b2d);
+                rec.qlim = beta_qlim_from(dmax);
+                edge = BetaEdge{m2[0], m2[1], P[0], P[1], P[2], dmax};
+            } else {
+                const float w = density * float(mu);
+                rec.A = 0.5f * float(P[0]) * kLog2e;
+                rec.B = float(P[1]) * kLog2e;
+                rec.C = 0.5f * float(P[2]) * kLog2e;
+                rec.w = w;
+                rec.qlim = qlim_of(w);
+            }
+            uint32_t x0, x1, y0, y1;
+            if (rx > 0 && ry > 0 && tile_span(m2[0], rx, tile_size, ntx, x0, x1) &&
+                tile_span(m2[1], ry, tile_size, nty, y0, y1)) {
+                count = (x1 - x0) * (y1 - y0);
+                rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
+                rr = make_float2(rec.my, row_reach(rec));
+            }
         }
-        uint32_t x0, x1, y0, y1;
-        if (rx > 0 && ry > 0 && tile_span(m2[0], rx, tile_size, ntx, x0, x1) &&
-            tile_span(m2[1], ry, tile_size, nty, y0, y1)) {
-            count = (x1 - x0) * (y1 - y0);
-            rect = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
-            rr = make_float2(rec.my, row_reach(rec));
-        }
+        if constexpr (BETA)
+            recs[idx] = BetaRec{rec, edge};
+        else
+            recs[idx] = rec;
+        rects[idx] = rect;
+        if (reach) reach[idx] = rr;
+        counts[idx] = count;
     }
-    if constexpr (BETA)
-        recs[idx] = BetaRec{rec, edge};
-    else
-        recs[idx] = rec;
-    rects[idx] = rect;
-    if (reach) reach[idx] = rr;
-    counts[idx] = count;
 }
 
 // Records (key = tile within the camera, ty * ntx + tx; value = cam * N + n)
@@ -1529,7 +1569,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
             double cd[6], P[3], mu;
 #pragma unroll
             for (int k = 0; k < 6; ++k) cd[k] = double(c2[k]);
-            conic_mu_f64(cd, double(eps), P, mu);
+            conic_mu_fast_f64(cd, double(eps), P, mu);
             // the forward stored the conic and mu in float32
             const double Pf[3] = {double(float(P[0])), double(float(P[1])), double(float(P[2]))};
             const double muf = double(float(mu));
@@ -1596,12 +1636,12 @@ struct FusedScratch {
     uint32_t range_len, n_ranges, n_bins, bin_bits;
     size_t reach, range_counts, bin_totals, bin_scan;
     // primitive order: bounds, key sort, and the permuted parameter copies
-    size_t order_bounds, order_sort_n, order_sort_n_bytes, order_sorted_keys, params;
+    size_t order_bounds, order_sort_n, order_sort_n_bytes, order_sorted_keys, params, frames, cams;
 };
 
-// floats per primitive of the permuted copies: mean 3, quaternion 4, scale 3,
-// density, beta, ratio, uncertainty
-constexpr size_t kPermutedFloats = 14;
+// floats per primitive of the permuted copies: mean 3, density, beta, ratio,
+// uncertainty (then a PrimFrame per primitive)
+constexpr size_t kPermutedFloats = 7;
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -1630,6 +1670,8 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
     L.order_sort_n = o; o += al(L.order_sort_n_bytes);
     L.order_sorted_keys = o; o += al(size_t(N) * 4 + 4);
     L.params = o; o += al(size_t(N) * kPermutedFloats * sizeof(float));
+    L.frames = o; o += al(size_t(N) * sizeof(PrimFrame));
+    L.cams = o; o += al(size_t(C) * sizeof(CamFrame));
     L.rects = o; o += al(CN * sizeof(uint2));
     L.reach = o; o += al(CN * sizeof(float2));
     L.scan = o; o += al(scan_workspace_bytes(CN));
@@ -1795,8 +1837,9 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     uint32_t* cum = record_offsets;
     uint32_t* perm = record_offsets + size_t(CN) + 1;  // the primitive order (input index per position)
     float* pp = at<float>(scratch, L.params);
-    float *p_means = pp, *p_quats = pp + 3 * size_t(N), *p_scales = pp + 7 * size_t(N), *p_dens = pp + 10 * size_t(N);
-    float *p_betas = pp + 11 * size_t(N), *p_ratio = pp + 12 * size_t(N), *p_unc = pp + 13 * size_t(N);
+    float *p_means = pp, *p_dens = pp + 3 * size_t(N), *p_betas = pp + 4 * size_t(N), *p_ratio = pp + 5 * size_t(N);
+    float* p_unc = pp + 6 * size_t(N);
+    PrimFrame* p_frames = at<PrimFrame>(scratch, L.frames);
     if (N > 0) {
         void* ows = at<void>(scratch, L.order_sort_n);
         uint32_t* bounds = at<uint32_t>(scratch, L.order_bounds);
@@ -1818,15 +1861,17 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
             return rc;
         permute_params_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
             perm, N, means, quats, scales, densities, BETA ? betas : nullptr, BETA ? beta_ratio : nullptr,
-            VAR ? uncertainties : nullptr, p_means, p_quats, p_scales, p_dens, p_betas, p_ratio, p_unc);
+            VAR ? uncertainties : nullptr, p_means, p_dens, p_betas, p_ratio, p_unc, p_frames);
         if (int rc = check_launch("permute_params")) return rc;
     }
     if (CN > 0) {
-        dim3 grid(ceil_div(N, 256), C);
+        CamFrame* cams = at<CamFrame>(scratch, L.cams);
+        camera_frames_kernel<<<ceil_div(C, 64), 64, 0, stream>>>(viewmats, Ks, C, width, height, cams);
+        if (int rc = check_launch("camera_frames")) return rc;
+        dim3 grid(ceil_div(N, 256), ceil_div(C, kProjectCams));
         fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
-            p_means, p_quats, p_scales, p_dens, p_betas, p_ratio, p_unc, viewmats, Ks, width, height, eps,
-            radius_clip, tile_size, ntx, nty, N, recs, at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach),
-            cum);
+            p_means, p_frames, p_dens, p_betas, p_ratio, p_unc, cams, C, width, height, eps, radius_clip,
+            tile_size, ntx, nty, N, recs, at<uint2>(scratch, L.rects), at<float2>(scratch, L.reach), cum);
         if (int rc = check_launch("fused_project")) return rc;
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;

@@ -102,21 +102,36 @@ struct ProjState {
     S Sc[6];            // camera-frame covariance, packed
 };
 
-// Returns false when the primitive lies at depth z <= 0.01 (culled).
+// Field-of-view clamp limits of a camera (15% image margin on each side), in
+// normalised image coordinates: x in [-xn, xp], y in [-yn, yp].
 template <typename S>
-__device__ __forceinline__ bool project_state(const Camera<S>& cam, const S p[3], const S cov[6],
-                                              S width, S height, ProjState<S>& st) {
+struct FovLimits {
+    S xp, xn, yp, yn;
+};
+
+template <typename S>
+__device__ __forceinline__ FovLimits<S> fov_limits(const Camera<S>& cam, S width, S height) {
+    FovLimits<S> lim;
+    lim.xp = (S(1.15) * width - cam.cx) / cam.fx;
+    lim.xn = (cam.cx + S(0.15) * width) / cam.fx;
+    lim.yp = (S(1.15) * height - cam.cy) / cam.fy;
+    lim.yn = (cam.cy + S(0.15) * height) / cam.fy;
+    return lim;
+}
+
+// Returns false when the primitive lies at depth z <= 0.01 (culled). `fov` is
+// fov_limits(cam, width, height) (computed once per camera by callers that
+// project many primitives per camera).
+template <typename S>
+__device__ __forceinline__ bool project_state(const Camera<S>& cam, const FovLimits<S>& fov, const S p[3],
+                                              const S cov[6], ProjState<S>& st) {
     const S* R = cam.R;
     st.x = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + cam.t[0];
     st.y = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + cam.t[1];
     st.z = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + cam.t[2];
     if (!(st.z > S(0.01))) return false;
     const S rz = S(1) / st.z;
-    // Field-of-view clamp with a 15% image margin on each side.
-    const S lim_xp = (S(1.15) * width - cam.cx) / cam.fx;
-    const S lim_xn = (cam.cx + S(0.15) * width) / cam.fx;
-    const S lim_yp = (S(1.15) * height - cam.cy) / cam.fy;
-    const S lim_yn = (cam.cy + S(0.15) * height) / cam.fy;
+    const S lim_xp = fov.xp, lim_xn = fov.xn, lim_yp = fov.yp, lim_yn = fov.yn;
     const S u = st.x * rz, v = st.y * rz;
     st.clip_x = (u > lim_xp) || (u < -lim_xn);
     st.clip_y = (v > lim_yp) || (v < -lim_yn);
@@ -148,6 +163,12 @@ __device__ __forceinline__ bool project_state(const Camera<S>& cam, const S p[3]
         for (int j = i; j < 3; ++j)
             st.Sc[k++] = RS[3 * i + 0] * R[3 * j + 0] + RS[3 * i + 1] * R[3 * j + 1] + RS[3 * i + 2] * R[3 * j + 2];
     return true;
+}
+
+template <typename S>
+__device__ __forceinline__ bool project_state(const Camera<S>& cam, const S p[3], const S cov[6],
+                                              S width, S height, ProjState<S>& st) {
+    return project_state(cam, fov_limits(cam, width, height), p, cov, st);
 }
 
 // cov2d = J Sc J^T (packed upper triangle).
@@ -280,6 +301,31 @@ __device__ __forceinline__ bool conic_mu_f64(const double c[6], double eps, doub
     P[0] = (dn / det) / scale;
     P[1] = (-bn / det) / scale;
     P[2] = (an / det) / scale;
+    if (!(a > 0.0 && d > 0.0 && det > 0.0) || !isfinite(P[0]) || !isfinite(P[1]) || !isfinite(P[2])) {
+        P[0] = P[1] = P[2] = 0.0;
+        mu = 0.0;
+        return false;
+    }
+    const double w0 = P[0] * c[2] + P[1] * c[4];
+    const double w1 = P[1] * c[2] + P[2] * c[4];
+    const double s = c[5] + eps - c[2] * w0 - c[4] * w1;
+    mu = (s > 0.0 && isfinite(s)) ? ::sqrt(2.0 * M_PI * s) : 0.0;
+    return true;
+}
+
+// conic_mu_f64 with two divisions instead of nine (the reciprocal of the
+// scale and of det x scale, then products): the same values to within a few
+// double ulps, for the fused kernels that evaluate it per (camera, primitive).
+__device__ __forceinline__ bool conic_mu_fast_f64(const double c[6], double eps, double P[3], double& mu) {
+    const double a = c[0] + eps, b = c[1], d = c[3] + eps;
+    const double scale = fmax(fmax(fabs(a), fabs(d)), fabs(b));
+    const double rs = 1.0 / scale;
+    const double an = a * rs, bn = b * rs, dn = d * rs;
+    const double det = an * dn - bn * bn;
+    const double k = rs / det;
+    P[0] = dn * k;
+    P[1] = -bn * k;
+    P[2] = an * k;
     if (!(a > 0.0 && d > 0.0 && det > 0.0) || !isfinite(P[0]) || !isfinite(P[1]) || !isfinite(P[2])) {
         P[0] = P[1] = P[2] = 0.0;
         mu = 0.0;
