@@ -112,38 +112,43 @@ def test_mixed_gaussian_and_beta_batch_equals_jobs_alone(device):
         assert torch.equal(fit.params, ref.params)
 
 
-@pytest.mark.parametrize("use_beta", [False, True])
-def test_packed_raster_launches_equal_each_jobs_own(device, use_beta):
+@pytest.mark.parametrize("use_beta,use_var,detach", [(False, False, 0), (True, False, 0), (False, True, 0),
+                                                     (True, True, 1)])
+def test_packed_raster_launches_equal_each_jobs_own(device, use_beta, use_var, detach):
     """Four jobs of different views, sizes and image shapes (one of them
     overflowed, rendering nothing) in one packed render and one packed raster
-    VJP: every image and every intersection partial equals the job's own
-    launch bitwise."""
+    VJP, for each record kind (with uncertainty: the variance image, its
+    adjoint and detach): every image and every intersection partial equals
+    the job's own launch bitwise."""
     specs = [(900, 3, 48, 40), (400, 2, 64, 64), (1500, 4, 40, 72), (250, 5, 56, 24)]
-    T, waves, kind = 16, 8, int(use_beta)
+    T, waves = 16, 8
     stream = be.stream_of(device)
     preps, table = [], (be.RasterJob * len(specs))()
     first = 0
     for k, (n, c, h, w) in enumerate(specs):
         sc = workloads.scene(n, c, h, w, scale_range=(0.01, 0.08), seed=30 + k, device=device,
-                             beta_range=(0.5, 4.0) if use_beta else None)
+                             beta_range=(0.5, 4.0) if use_beta else None, uncertainty_max=0.1 if use_var else None)
         f = fused.prepare(*sc.params(), sc.viewmats, sc.Ks, w, h, 1e-8, 1.0 if use_beta else 3.0, tile_size=T,
-                          beta=sc.beta, capacity=64 if k == 2 else None, check=k != 2)
+                          beta=sc.beta, uncertainty=sc.uncertainty, capacity=64 if k == 2 else None, check=k != 2)
+        kind = f.kind
         g = torch.Generator(device=device).manual_seed(k)
         v = torch.randn((c, h, w), generator=g, device=device)
-        mine = (torch.full((c, h, w), float("nan"), device=device), torch.full((f.capacity * 8,), float("nan"),
-                                                                            device=device))
-        packed_out = (torch.full_like(mine[0], float("nan")), torch.full_like(mine[1], float("nan")))
+        vv = torch.randn((c, h, w), generator=g, device=device) if use_var else None
+        nan = lambda shape: torch.full(shape, float("nan"), device=device)  # noqa: E731
+        mine = (nan((c, h, w)), nan((f.capacity * 8,)), nan((c, h, w)) if use_var else None)
+        packed_out = (nan((c, h, w)), nan((f.capacity * 8,)), nan((c, h, w)) if use_var else None)
         call("r2s_fused_render", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters), f.capacity, c,
-             w, h, T, waves, kind, ptr(mine[0]), None, stream)
+             w, h, T, waves, kind, ptr(mine[0]), ptr(mine[2]), stream)
         call("r2s_fused_raster_vjp", ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), ptr(f.counters),
-             f.capacity, c, w, h, T, waves, kind, ptr(v), None, 0, ptr(mine[1]), stream)
+             f.capacity, c, w, h, T, waves, kind, ptr(v), ptr(vv), detach, ptr(mine[1]), stream)
         e = table[k]
         e.packed, e.tile_offsets, e.tile_order, e.counters = ptr(f.packed), ptr(f.offsets), ptr(f.tile_order), \
             ptr(f.counters)
-        e.image, e.v_image, e.isect_grads = ptr(packed_out[0]), ptr(v), ptr(packed_out[1])
+        e.image, e.image_var, e.v_image, e.v_image_var = ptr(packed_out[0]), ptr(packed_out[2]), ptr(v), ptr(vv)
+        e.isect_grads, e.detach = ptr(packed_out[1]), detach
         e.capacity, e.C, e.width, e.height, e.first_tile = f.capacity, c, w, h, first
         first += c * (-(-h // T)) * (-(-w // T))
-        preps.append((f, mine, packed_out, v))  # the table points into every job's buffers: keep them
+        preps.append((f, mine, packed_out, (v, vv)))  # the table points into every job's buffers: keep them
     assert int(preps[2][0].counters[1].item()) == 1  # the overflowed job
     dev_table = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table))),
                                  dtype=torch.uint8).to(device)
@@ -153,11 +158,15 @@ def test_packed_raster_launches_equal_each_jobs_own(device, use_beta):
     bits = lambda t: t.view(torch.int32)  # noqa: E731 (bitwise, NaN fill included)
     for k, (f, mine, packed_out, _) in enumerate(preps):
         assert torch.equal(bits(packed_out[0]), bits(mine[0])), k
+        if use_var:
+            assert torch.equal(bits(packed_out[2]), bits(mine[2])), k
         # slots of footprint-rectangle tiles the ellipse does not reach are never
         # written: both launches leave the same NaN fill there
         n = int(f.counters[0].item()) if k != 2 else 0
         assert torch.equal(bits(packed_out[1][: 8 * n]), bits(mine[1][: 8 * n])), k
     assert not bool(preps[2][2][0].any())  # an overflowed job renders zeros
+    if use_var:
+        assert not bool(preps[2][2][2].any())
     # a table whose first_tile prefix is wrong is refused before any launch
     table[1].first_tile += 1
     with pytest.raises(RuntimeError, match="starts at tile"):
