@@ -62,6 +62,40 @@ CLOCK_HZ = 2.4e9         # peak engine clock
 TRAFFIC_BYTES = {"rasterize_fwd": (2_061_623_776, 17_357_984), "rasterize_bwd": (2_386_446_336, 922_386_400)}
 
 
+def stage_rooflines(stages_ms, N, CN, I, kept_fwd, kept_bwd):
+    """Per-stage roofline of the step against the compulsory work of the
+    reference's stage decomposition (FP32): HBM bytes for the stages that
+    stream memory, FLOP on the pairs that pass the cut-off for the raster
+    stages; and the iteration floor (the stages at their peaks, one after the
+    other) against the measured stage sum. Bytes per stage: projection 36N +
+    40CN, conics 40CN, tile count 20CN, scan 12CN, generate 24CN + 12I, sort
+    2 x 32I, offsets 8I (binning: 36N + 136CN + 84I); conics VJP 80CN,
+    projection VJP 32CN + 72N, parameter VJP 80N, density reduce 4CN
+    (parameter VJP: 152N + 116CN)."""
+    work = {"project_bin": ("hbm", 36 * N + 136 * CN + 84 * I),
+            "rasterize_fwd": ("valu", kept_fwd * FLOP_PER_PAIR["rasterize_fwd"]),
+            "rasterize_bwd": ("valu", kept_bwd * FLOP_PER_PAIR["rasterize_bwd"]),
+            "param_bwd": ("hbm", 152 * N + 116 * CN)}
+    out, floor_ms, total_ms = {}, 0.0, 0.0
+    for name, (bound, amount) in work.items():
+        if name not in stages_ms:
+            continue
+        ms = stages_ms[name][0]
+        if bound == "hbm":
+            achieved, peak, unit = amount / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+            floor = amount / (HBM_PEAK_GBS * 1e9) * 1e3
+        else:
+            achieved, peak, unit = amount / (ms * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s"
+            floor = amount / (FP32_PEAK_TFLOPS * 1e12) * 1e3
+        out[name] = {"bound": bound, "achieved": round(achieved, 1), "peak": peak, "unit": unit,
+                     "frac": round(achieved / peak, 3), "ms": round(ms, 4)}
+        floor_ms += floor
+        total_ms += ms
+    out["iteration"] = {"floor_ms": round(floor_ms, 3), "stage_ms": round(total_ms, 3),
+                        "frac": round(floor_ms / total_ms, 3) if total_ms else None}
+    return out
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -176,6 +210,8 @@ def main():
                                     f"{FLOP_PER_PAIR[dominant]:.0f} FLOP (of {pc['list_pairs']:.4e} pairs in the "
                                     f"intersected tiles)",
                 "avg_launch_ms": round(dom_ms, 4),
+                "stages": stage_rooflines(stages, sc.means.shape[0], sc.viewmats.shape[0] * sc.means.shape[0],
+                                          n_isects, pc["kept_fwd"], pc["kept_bwd"]),
                 "issued": {"valu_insts": VALU_INSTS[dominant], "per_simd_cycle": round(issue, 4),
                            "note": "vector instructions the kernel issues per SIMD per cycle (measured count / "
                                    "(launch time x 2.4 GHz x 1,024 SIMDs)); a 16-lane SIMD completes one "
