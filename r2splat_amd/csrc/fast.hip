@@ -960,9 +960,9 @@ __device__ __forceinline__ void splat_accumulate(const R& s, uint32_t rows, floa
             e = __builtin_amdgcn_exp2f(-q);
         const float ek = contributes(q, s.qlim) ? e : 0.f;
         if constexpr (VAR) {
-            const float alpha = ek * s.w;  // variance = sum q alpha^2
-            acc[k] += alpha;
-            acc_var[k] = fmaf(s.unc * alpha, alpha, acc_var[k]);
+            // s.unc is the staged variance weight q w^2: variance = sum q alpha^2
+            acc[k] = fmaf(ek, s.w, acc[k]);
+            acc_var[k] = fmaf(s.unc * ek, ek, acc_var[k]);
         } else {
             acc[k] = fmaf(ek, s.w, acc[k]);
         }
@@ -1050,7 +1050,15 @@ __device__ __forceinline__ void raster_fwd_tile(const uint2* __restrict__ entrie
         const uint32_t spans = r_lo < r_hi ? ((k_hi >= 32 ? 0xffffffffu : (1u << k_hi) - 1u) & ~((1u << k_lo) - 1u))
                                            : 0u;
         wave_lds_sync();  // every lane is done reading the previous batch
-        slot[lane] = next;
+        if constexpr (VAR) {
+            // the staged copy's uncertainty field carries the record's variance
+            // weight q w^2 (variance = sum q alpha^2 = sum (q w^2) e^2)
+            R staged = next;
+            staged.unc = next.unc * next.w * next.w;
+            slot[lane] = staged;
+        } else {
+            slot[lane] = next;
+        }
         wave_lds_sync();
         next = gather_splat<R>(recs, ahead);
         ahead = load_entry(entries, base + 2 * kStride + int32_t(lane), hi);
@@ -1216,25 +1224,27 @@ __device__ __forceinline__ void raster_bwd_tile(const uint2* __restrict__ entrie
                 }
             }
             float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-            // one Beta pair's terms (in: it passes the cut-off; h <= 0 outside the
-            // support makes log2 h -inf or NaN there, so it is selected away, not
-            // multiplied by a zero weight; kept pairs have h >= the smallest normal float)
+            // one Beta pair's terms (in: it passes the cut-off). h is clamped to
+            // the smallest normal float (outside the support 1 - q <= 0), so log2 h
+            // is finite and a pair that fails the cut-off (hb = 0, so ve = 0) adds
+            // exact zeros to every sum
             auto beta_pair = [&](float q, float dx, float dxsq_j, float vi, float vvi, bool in) {
                 const float h = fmaxf(1.f - q, 1.17549435e-38f), lg = __log2f(h);
                 const float hb = in ? __builtin_amdgcn_exp2f(bm1 * lg) : 0.f;  // h^(b2d - 1)
                 float g, ve;
                 if constexpr (VAR) {
                     const float he = hb * h;  // h^b2d = alpha / w
-                    const float vj = fmaf(kvar * he, vvi, vi);
+                    const float t = vvi * he;
+                    const float vj = fmaf(kvar, t, vi);
                     g = vj * hb;
                     ve = vj * he;
-                    vq = fmaf(vvi * he, he, vq);
+                    vq = fmaf(t, he, vq);
                 } else {
                     g = vi * hb;
                     ve = g * h;
                 }
                 se += ve;
-                sl = fmaf(ve, in ? lg : 0.f, sl);
+                sl = fmaf(ve, lg, sl);
                 r0 += g;
                 r1 = fmaf(g, dx, r1);
                 r2 = fmaf(g, dxsq_j, r2);
@@ -1252,9 +1262,10 @@ __device__ __forceinline__ void raster_bwd_tile(const uint2* __restrict__ entrie
                     const float e = contributes(q, s.qlim) ? __builtin_amdgcn_exp2f(-q) : 0.f;
                     float g;
                     if constexpr (VAR) {
-                        const float vj = fmaf(kvar * e, vv[j], v[j]);
+                        const float t = vv[j] * e;
+                        const float vj = fmaf(kvar, t, v[j]);
                         g = vj * e;
-                        vq = fmaf(vv[j] * e, e, vq);
+                        vq = fmaf(t, e, vq);
                     } else {
                         g = e * v[j];
                     }
