@@ -336,6 +336,11 @@ def batched_main(args):
     for f in batch.fits:
         key = f"tile{f.tile_size}/fwd{f.fwd_waves or 8}/bwd{f.bwd_waves or 8}"
         configs[key] = configs.get(key, 0) + 1
+    # prepare + render + raster VJP of every job at its tuned configuration
+    # against the default one (tile 16, 8 waves each), from the tuner's timings
+    tuned_ms = sum(r.ms for r in batch.tune_records)
+    default_ms = sum(r.timings[(16, "prepare", 0)] + r.timings[(16, "render", 8)] + r.timings[(16, "raster_vjp", 8)]
+                     for r in batch.tune_records)
     result = {
         "metric": "batched fwd+MSE+bwd+Adam steps/s (32 uneven jobs, one hipGraph per step)",
         "value": round(1e3 / ms, 3),
@@ -358,6 +363,11 @@ def batched_main(args):
         "intersections_per_step": int(sum(n_isects)),
         "resident_gb": round(batch.resident_bytes() / 1e9, 2),
         "tuned_configs": configs,
+        "job_configs": [[int(f.N), int(f.C), int(f.H), f.tile_size, f.fwd_waves or 8, f.bwd_waves or 8]
+                        for f in batch.fits],
+        "tuning": {"tuned_ms": round(tuned_ms, 2), "default_ms": round(default_ms, 2),
+                   "description": "prepare + render + raster VJP of every job once, at its tuned configuration and "
+                                  "at tile 16 / 8 waves (the tuner's timings)"},
         "setup_s": {"generate": round(t_gen, 2), "tune_and_build": round(t_tune, 2)},
         "mse_first_last": [round(sum(losses0) / len(losses0), 6), round(sum(losses) / len(losses), 6)],
         "plain_path": None if plain_ms is None else {
@@ -454,7 +464,7 @@ def serving_main(args):
     import torch
 
     from r2splat_amd import explicit
-    from r2splat_amd.serving import ExplicitFit, FitServer, build_job, poisson_requests
+    from r2splat_amd.serving import ExplicitFit, FitServer, build_job, job_config, poisson_requests
     from r2splat_amd.train import AdamConfig, GaussianFit
 
     device = torch.device("cuda:0")
@@ -486,8 +496,10 @@ def serving_main(args):
         torch.cuda.synchronize()
         plain_ms = (time.perf_counter() - s0) * 1e3 / PLAIN_CHECK_STEPS
         explicit.BACKEND = "fused"
+        cfg = job_config(req)
         alone = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
-                            job["target"], adam=adam, beta=job["beta"])
+                            job["target"], adam=adam, beta=job["beta"], tile_size=cfg.tile_size,
+                            fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves)
         alone.capture()
         alone.step(PLAIN_CHECK_STEPS)
         alone.sync()

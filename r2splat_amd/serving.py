@@ -32,6 +32,7 @@ from typing import Callable, List, Optional
 import torch
 
 from . import explicit, workloads
+from .batched import KernelConfig, heuristic_config
 from .train import AdamConfig, GaussianFit
 
 
@@ -65,6 +66,12 @@ def poisson_requests(rate: float = 4.0, duration: float = 120.0, *, seed: int = 
         beta = bool(torch.rand((), generator=g) < p_beta)
         out.append(FitRequest(rid, t, n, c, side, 7000 + 2 * rid, beta, steps))
         rid += 1
+
+
+def job_config(req: FitRequest) -> KernelConfig:
+    """The kernel configuration the server gives a request's job (tile edge,
+    waves per tile of the raster forward and backward), from its shape."""
+    return heuristic_config(req.n, req.views, req.side, req.side, req.beta)
 
 
 def build_job(req: FitRequest, device) -> dict:
@@ -156,8 +163,11 @@ class ServeStats:
 class FitServer:
     def __init__(self, device="cuda", *, hbm_budget: Optional[int] = None, n_streams: int = 4,
                  adam: AdamConfig = AdamConfig(lr=1e-2, min_scale=1e-6), headroom: float = 1.15,
-                 sync_every: int = 8):
+                 sync_every: int = 8, config: Optional[Callable[[FitRequest], KernelConfig]] = None):
+        """``config`` maps a request to its job's kernel configuration
+        (default ``job_config``: from the job's shape, no tuning runs)."""
         self.device = torch.device(device)
+        self.config = config or job_config
         if hbm_budget is None:
             hbm_budget = int(torch.cuda.get_device_properties(self.device).total_memory * 0.85)
         self.budget = int(hbm_budget)
@@ -190,8 +200,10 @@ class FitServer:
         job = build_job(req, self.device)
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
+        cfg = self.config(req)
         fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
-                          job["target"], adam=self.adam, stream=stream, headroom=self.headroom, beta=job["beta"])
+                          job["target"], adam=self.adam, stream=stream, headroom=self.headroom, beta=job["beta"],
+                          tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves)
         fit.capture()
         self.live.append(_Live(req, fit, now, 0, fit.memory_bytes()))
         return True
@@ -291,4 +303,4 @@ class FitServer:
         return stats
 
 
-__all__ = ["ExplicitFit", "FitRequest", "FitServer", "ServeStats", "build_job", "poisson_requests"]
+__all__ = ["ExplicitFit", "FitRequest", "FitServer", "ServeStats", "build_job", "job_config", "poisson_requests"]

@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle import splat as O
-from r2splat_amd.serving import ExplicitFit, FitRequest, FitServer, build_job
+from r2splat_amd.serving import ExplicitFit, FitRequest, FitServer, build_job, job_config
 from r2splat_amd.train import AdamConfig, GaussianFit
 
 pytestmark = pytest.mark.gpu
@@ -21,9 +21,12 @@ def _requests():
 
 
 def _alone(req, device):
+    """The request's job fitted alone, with the kernel configuration the server gives it."""
     job = build_job(req, device)
+    cfg = job_config(req)
     fit = GaussianFit(job["means"], job["quats"], job["scales"], job["densities"], job["viewmats"], job["Ks"],
-                      job["target"], adam=ADAM, beta=job["beta"])
+                      job["target"], adam=ADAM, beta=job["beta"], tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves,
+                      bwd_waves=cfg.bwd_waves)
     fit.capture()
     fit.step(req.steps)
     fit.sync()
@@ -37,6 +40,24 @@ def test_each_result_is_the_same_alone_and_in_the_mix(device):
     assert max(f[3] - f[1] for f in stats.finished) > 0
     for req in _requests():
         assert torch.equal(stats.results[req.rid], _alone(req, device)), req.rid
+
+
+def test_jobs_run_with_the_configuration_of_their_shape(device):
+    seen = {}
+    server = FitServer(device, n_streams=2, adam=ADAM)
+    admit = server._admit
+
+    def record(req, now):
+        ok = admit(req, now)
+        if ok:
+            fit = server.live[-1].fit
+            seen[req.rid] = (fit.tile_size, fit.fwd_waves, fit.bwd_waves)
+        return ok
+
+    server._admit = record
+    server.run(_requests(), clock="ticks")
+    assert seen == {r.rid: (job_config(r).tile_size, job_config(r).fwd_waves, job_config(r).bwd_waves)
+                    for r in _requests()}
 
 
 @pytest.mark.parametrize("beta", [False, True])

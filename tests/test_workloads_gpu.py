@@ -23,7 +23,7 @@ import torch
 from r2splat_amd import autograd as ag
 from r2splat_amd import explicit, workloads
 from r2splat_amd.batched import Autotuner, BatchedFit
-from r2splat_amd.serving import FitServer, build_job, poisson_requests
+from r2splat_amd.serving import FitServer, build_job, job_config, poisson_requests
 from r2splat_amd.train import AdamConfig, GaussianFit
 
 pytestmark = pytest.mark.gpu
@@ -96,4 +96,6 @@ def test_config4_serving_window_latency_and_alone_vs_mix(device):
     del server
     sample = reqs[:2] + [r for r in reqs if r.beta][:1]
     for req in sample:
-        assert torch.equal(stats.results[req.rid], _alone(build_job(req, device), ADAM, req.steps)), req.rid
+        cfg = job_config(req)
+        config = dict(tile_size=cfg.tile_size, fwd_waves=cfg.fwd_waves, bwd_waves=cfg.bwd_waves)
+        assert torch.equal(stats.results[req.rid], _alone(build_job(req, device), ADAM, req.steps, **config)), req.rid
