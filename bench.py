@@ -122,6 +122,11 @@ def parse():
     p.add_argument("--plain-jobs", type=int, default=3, help="serving: requests re-fitted on the plain path")
     p.add_argument("--jobs", type=int, default=32, help="jobs of the batched workload")
     p.add_argument("--streams", type=int, default=4, help="streams of the batched workload")
+    p.add_argument("--max-active", type=int, default=0,
+                   help="serving: step only the earliest-arrived N live jobs per tick (0: every live job)")
+    p.add_argument("--active-ms", type=float, default=0.0,
+                   help="serving: step the earliest-arrived live jobs whose estimated iteration times add up to "
+                        "this many ms per tick (0: every live job)")
     p.add_argument("--no-packed", action="store_true",
                    help="batched: run each job's raster stages as its own launches on its stream instead of "
                         "packed launches over all jobs' tiles")
@@ -219,8 +224,8 @@ def main():
 
     parity = None if args.no_parity else parity_vs_fp64(sc, target)
     voxel = None if args.no_voxel else voxel_timing(sc, args.steps)
-    latency = fit_latency(args.rate, args.serve_window, args.serve_steps, args.streams) if args.serve_window > 0 \
-        else None
+    latency = fit_latency(args.rate, args.serve_window, args.serve_steps, args.streams, args.max_active or None,
+                          args.active_ms or None) if args.serve_window > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -470,7 +475,8 @@ def serving_main(args):
     device = torch.device("cuda:0")
     adam = AdamConfig(lr=1e-2, min_scale=1e-6)
     reqs = poisson_requests(args.rate, args.duration, seed=0, steps=args.serve_steps)
-    server = FitServer(device, n_streams=args.streams, adam=adam)
+    server = FitServer(device, n_streams=args.streams, adam=adam, max_active=args.max_active or None,
+                       active_ms=args.active_ms or None)
 
     def progress(stats, live, queued):
         print(f"[serving] {time.perf_counter() - t0:.0f}s finished {len(stats.finished)}/{len(reqs)} live {live} "
@@ -530,6 +536,7 @@ def serving_main(args):
         "config": {"workload": f"serving: Poisson {args.rate} req/s for {args.duration} s, N log-uniform [10k, 400k], "
                                f"C in {{16,32,64,128}}, side in {{128,256,512}}, Gaussian or Beta (p=0.5), "
                                f"{args.serve_steps} Adam iterations (lr 1e-2) each", "streams": args.streams,
+                   "max_active": args.max_active or None, "active_ms": args.active_ms or None,
                    "hbm_budget_gb": round(server.budget / 1e9, 1)},
         "requests": len(reqs),
         "finished": len(stats.finished),
@@ -587,7 +594,7 @@ def parity_vs_fp64(sc, target):
     return out
 
 
-def fit_latency(rate, duration, steps, streams):
+def fit_latency(rate, duration, steps, streams, max_active=None, active_ms=None):
     """p50 / p99 time-to-finished-fit of Poisson fit requests (the serving
     workload's request mix) over a short arrival window, served by the
     continuous-batching server on this GPU."""
@@ -597,7 +604,8 @@ def fit_latency(rate, duration, steps, streams):
     from r2splat_amd.train import AdamConfig
 
     reqs = poisson_requests(rate, duration, seed=0, steps=steps)
-    server = FitServer(torch.device("cuda:0"), n_streams=streams, adam=AdamConfig(lr=1e-2, min_scale=1e-6))
+    server = FitServer(torch.device("cuda:0"), n_streams=streams, adam=AdamConfig(lr=1e-2, min_scale=1e-6),
+                       max_active=max_active, active_ms=active_ms)
     stats = server.run(reqs)
     del server
     torch.cuda.empty_cache()

@@ -163,11 +163,19 @@ class ServeStats:
 class FitServer:
     def __init__(self, device="cuda", *, hbm_budget: Optional[int] = None, n_streams: int = 4,
                  adam: AdamConfig = AdamConfig(lr=1e-2, min_scale=1e-6), headroom: float = 1.15,
-                 sync_every: int = 8, config: Optional[Callable[[FitRequest], KernelConfig]] = None):
+                 sync_every: int = 8, config: Optional[Callable[[FitRequest], KernelConfig]] = None,
+                 max_active: Optional[int] = None, active_ms: Optional[float] = None):
         """``config`` maps a request to its job's kernel configuration
-        (default ``job_config``: from the job's shape, no tuning runs)."""
+        (default ``job_config``: from the job's shape, no tuning runs).
+        ``max_active`` caps the jobs that step on a tick to the earliest
+        arrived ones; ``active_ms`` steps the earliest arrived jobs until their
+        estimated iteration times (``estimate_ms``) add up to that many
+        milliseconds (enough work to keep the GPU busy); None for both: every
+        live job steps on every tick."""
         self.device = torch.device(device)
         self.config = config or job_config
+        self.max_active = None if max_active is None else max(1, int(max_active))
+        self.active_ms = None if active_ms is None else float(active_ms)
         if hbm_budget is None:
             hbm_budget = int(torch.cuda.get_device_properties(self.device).total_memory * 0.85)
         self.budget = int(hbm_budget)
@@ -186,6 +194,14 @@ class FitServer:
         per = {128: 2.0, 256: 5.0, 512: 15.0}.get(req.side, 5.0 * (req.side / 256.0) ** 2)
         isects = req.n * req.views * per
         return int(16 * 11 * req.n + 12 * req.views * req.side * req.side + 100 * isects + 48 * req.views * req.n)
+
+    @staticmethod
+    def estimate_ms(req: FitRequest) -> float:
+        """Rough GPU time of one iteration of a request's job on MI355X: a
+        fixed launch cost, then per expected intersection and per pixel (the
+        config 1 shape, 27.5M intersections and 4.2M pixels, takes ~3.5 ms)."""
+        per = {128: 2.0, 256: 5.0, 512: 15.0}.get(req.side, 5.0 * (req.side / 256.0) ** 2)
+        return 0.15 + 1.0e-7 * req.n * req.views * per + 1.5e-7 * req.views * req.side * req.side
 
     def used_bytes(self) -> int:
         """Current footprint of the live jobs (their workspaces grow when their
@@ -270,13 +286,23 @@ class FitServer:
             stats.peak_bytes = max(stats.peak_bytes, self.used_bytes())
             stats.peak_live = max(stats.peak_live, len(self.live))
             running = [l for l in self.live if not l.stalled]
+            if self.max_active is not None or self.active_ms is not None:  # the earliest arrivals first
+                running = sorted(running, key=lambda l: (l.req.arrival, l.req.rid))
+                if self.max_active is not None:
+                    running = running[:self.max_active]
+                if self.active_ms is not None:
+                    total, k = 0.0, 0
+                    while k < len(running) and total < self.active_ms:
+                        total += self.estimate_ms(running[k].req)
+                        k += 1
+                    running = running[:k]
             for l in running:   # one iteration of every live job that can run
                 l.fit.step(1)
                 l.issued += 1
             stats.ticks += 1
             stats.job_steps += len(running)
             finishing = any(l.issued >= l.req.steps for l in running)
-            if not finishing and stats.ticks % self.sync_every and len(running) == len(self.live):
+            if not finishing and stats.ticks % self.sync_every and not any(l.stalled for l in self.live):
                 continue
             for l in self.live:
                 self._sync(l)

@@ -33,11 +33,15 @@ def _alone(req, device):
     return fit.params.clone()
 
 
-def test_each_result_is_the_same_alone_and_in_the_mix(device):
-    server = FitServer(device, n_streams=2, adam=ADAM)
+@pytest.mark.parametrize("max_active", [None, 2])
+def test_each_result_is_the_same_alone_and_in_the_mix(device, max_active):
+    server = FitServer(device, n_streams=2, adam=ADAM, max_active=max_active)
     stats = server.run(_requests(), clock="ticks", keep_results=True)
     assert sorted(f[0] for f in stats.finished) == [0, 1, 2, 3]
     assert max(f[3] - f[1] for f in stats.finished) > 0
+    if max_active is not None:  # the earliest arrivals step first: they finish in arrival order
+        order = [f[0] for f in sorted(stats.finished, key=lambda f: f[3])]
+        assert order.index(0) < order.index(3) and order.index(1) < order.index(3)
     for req in _requests():
         assert torch.equal(stats.results[req.rid], _alone(req, device)), req.rid
 
