@@ -33,13 +33,17 @@ def _alone(req, device):
     return fit.params.clone()
 
 
-@pytest.mark.parametrize("max_active", [None, 2])
-def test_each_result_is_the_same_alone_and_in_the_mix(device, max_active):
-    server = FitServer(device, n_streams=2, adam=ADAM, max_active=max_active)
+@pytest.mark.parametrize("policy", [{}, {"max_active": 2}, {"active_ms": 0.2}])
+def test_each_result_is_the_same_alone_and_in_the_mix(device, policy):
+    """Every job equals the job fitted alone whatever steps with it; with an
+    earliest-arrival active set (2 jobs, or ~0.2 ms of estimated work: the
+    test jobs estimate 0.15-0.16 ms each, so 2 jobs) the early jobs finish first."""
+    assert all(0.15 <= FitServer.estimate_ms(r) < 0.2 for r in _requests())
+    server = FitServer(device, n_streams=2, adam=ADAM, **policy)
     stats = server.run(_requests(), clock="ticks", keep_results=True)
     assert sorted(f[0] for f in stats.finished) == [0, 1, 2, 3]
     assert max(f[3] - f[1] for f in stats.finished) > 0
-    if max_active is not None:  # the earliest arrivals step first: they finish in arrival order
+    if policy:  # the earliest arrivals step first: they finish before the last arrival
         order = [f[0] for f in sorted(stats.finished, key=lambda f: f[3])]
         assert order.index(0) < order.index(3) and order.index(1) < order.index(3)
     for req in _requests():
