@@ -171,7 +171,7 @@ def main():
 
     def step():
         img = ag.rasterize(*params, sc.viewmats, sc.Ks, sc.image_size, recompute=False)
-        loss = ((img - target) ** 2).mean()
+        loss = ag.mse_loss(img, target)
         return torch.autograd.grad(loss, params)
 
     def timed(backend, steps, warmup):

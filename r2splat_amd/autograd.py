@@ -12,6 +12,11 @@ forward to the backward (False) and rebuilding them in the backward (True).
 exceeds ``recompute_budget`` bytes (default: a quarter of the device's
 memory), which is the regime of many views times many primitives.
 
+``mse_loss`` is the mean squared error of a float32 image against a target
+in one kernel that also writes its adjoint 2 (image - target) / n (plus a
+fixed-order reduction of per-block partial sums), so an MSE step adds two
+launches and an elementwise scale in the backward.
+
 The reference exposes only the explicit (output, ctx) pair
 (r2splat/explicit.py:102-351); these wrappers compose it with autograd.
 """
@@ -146,4 +151,39 @@ def voxelize(
     return _Voxelize.apply(options, means, quats, scales, densities, beta, uncertainty)
 
 
-__all__ = ["rasterize", "voxelize"]
+class _MSE(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, image, target):
+        from .hip import _backend as be
+
+        if image.dtype != torch.float32 or target.dtype != torch.float32 or not image.is_cuda:
+            raise TypeError("mse_loss takes float32 CUDA tensors")
+        if image.shape != target.shape or image.device != target.device:
+            raise ValueError(f"mse_loss: image {tuple(image.shape)} and target {tuple(target.shape)} differ")
+        image, target = image.contiguous(), target.contiguous()
+        n = image.numel()
+        v = torch.empty_like(image)
+        partials = torch.empty((max(1, int(be.lib().r2s_mse_partials(n))),), dtype=torch.float32,
+                               device=image.device)
+        loss = torch.empty((), dtype=torch.float32, device=image.device)
+        with torch.cuda.device(image.device):
+            be.call("r2s_mse_adjoint", be.ptr(image), be.ptr(target), n, be.ptr(v), be.ptr(partials), be.ptr(loss),
+                    be.stream_of(image.device))
+        ctx.save_for_backward(v)
+        ctx.target_grad = target.requires_grad
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad):
+        (v,) = ctx.saved_tensors
+        g = v * grad
+        return g, (-g if ctx.target_grad else None)
+
+
+def mse_loss(image: Tensor, target: Tensor) -> Tensor:
+    """mean((image - target)^2) of float32 CUDA tensors of one shape, with
+    d/d image = 2 (image - target) / n from the same kernel."""
+    return _MSE.apply(image, target)
+
+
+__all__ = ["mse_loss", "rasterize", "voxelize"]

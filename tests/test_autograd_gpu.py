@@ -254,3 +254,50 @@ def test_recompute_context_holds_no_intersection_lists(device):
     assert set(ctx.buffers) == {"viewmats", "Ks", "recompute"}
     assert ag._should_recompute(None, 1024, 1000, 4, sc.means.device)
     assert not ag._should_recompute(None, 1 << 40, 1000, 4, sc.means.device)
+
+
+def test_mse_loss_value_and_gradient(device):
+    """mse_loss against the float64 mean of squared differences, its
+    gradient 2 (image - target) / n scaled by the upstream gradient, the
+    target's gradient, and the error cases."""
+    g = torch.Generator(device=device).manual_seed(0)
+    a = torch.randn((3, 37, 41), generator=g, device=device).requires_grad_()
+    b = torch.randn((3, 37, 41), generator=g, device=device).requires_grad_()
+    loss = ag.mse_loss(a, b)
+    ref = ((a.detach().double() - b.detach().double()) ** 2).mean()
+    assert abs(loss.item() - ref.item()) <= 2e-7 * ref.item()
+    ga, gb = torch.autograd.grad(loss * 3.0, (a, b))
+    exp = 3.0 * 2.0 * (a.detach().double() - b.detach().double()) / a.numel()
+    assert (ga.double() - exp).abs().max() <= 1e-6 * exp.abs().max()
+    assert torch.equal(gb, -ga)
+    with pytest.raises(ValueError):
+        ag.mse_loss(a, b[:, :, :40])
+    with pytest.raises(TypeError):
+        ag.mse_loss(a.double(), b.double())
+
+
+def test_mse_step_in_a_graph_equals_eager(device):
+    """The rasterize + mse_loss step captured into a hipGraph: every replay
+    gives the eager step's gradients bitwise."""
+    sc = workloads.config0(device=device)
+    params = [t.detach().requires_grad_() for t in sc.params()]
+    target = torch.rand((sc.viewmats.shape[0],) + tuple(sc.image_size), device=device,
+                        generator=torch.Generator(device=device).manual_seed(1))
+
+    def step():
+        img = ag.rasterize(*params, sc.viewmats, sc.Ks, sc.image_size, recompute=False)
+        return torch.autograd.grad(ag.mse_loss(img, target), params)
+
+    eager = [t.clone() for t in step()]
+    side = torch.cuda.Stream(device)
+    side.wait_stream(torch.cuda.current_stream(device))
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream(device).wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        out = step()
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize(device)
+        assert all(torch.equal(o, e) for o, e in zip(out, eager))
