@@ -619,35 +619,69 @@ __device__ __forceinline__ void fused_tile(const int32_t* __restrict__ offsets, 
 // The raster kernels deal their n tiles to the 8 XCDs (workgroup b runs on
 // XCD b % 8) as contiguous bands of the camera-major tile index: band x holds
 // per + (x < rem) tiles (per = n / 8, rem = n % 8), so the tiles that gather
-// the same records (neighbours in one view) share one XCD's L2. xcd_band is a
-// tile's band; xcd_slot the position of workgroup b in the band-major order.
-__device__ __forceinline__ uint32_t xcd_band(uint32_t t, uint32_t n) {
-    const uint32_t per = n / 8u, rem = n % 8u, big = rem * (per + 1u);
-    return t < big ? t / (per + 1u) : rem + (t - big) / per;  // per >= 1 whenever t >= big
-}
+// the same records (neighbours in one view) share one XCD's L2. xcd_slot is
+// the position of workgroup b in the band-major order.
 __device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
     const uint32_t per = n / 8u, rem = n % 8u, x = b % 8u;
     return x * per + min(x, rem) + b / 8u;
 }
 
-// Sort keys that order tiles by XCD band, then by decreasing intersection
-// count (in size classes of 2^(1/8)): key = band << 8 | (255 - floor(8
-// log2(count + 1))). Each XCD walks its band's tiles longest first, so the
+// The tile order: each XCD band's tiles (the band the raster launches give
+// XCD x: per + (x < rem) tiles of the camera-major index from x per +
+// min(x, rem)) in decreasing size class of 2^(1/8) (class 255 - floor(8
+// log2(count + 1))), so that each XCD walks its tiles longest first and the
 // few heavy tiles of a view do not all land on the same compute units at the
-// end of the launch.
-constexpr uint32_t kTileOrderBits = 11;
-__global__ void __launch_bounds__(256)
-fused_tile_order_keys_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n,
-                             uint32_t capacity, uint32_t C, uint32_t nt, uint32_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= C * nt) return;
-    const uint32_t n_live = live(d_n, capacity);
+// end of the launch. One workgroup per band deals its tiles into the classes
+// with an LDS histogram, one wave's exclusive scan of it and atomic slots.
+// The order inside a class is whatever the atomics give: it decides only
+// which workgroup runs a tile when, never a result.
+constexpr uint32_t kTileClasses = 256, kTileOrderThreads = 1024;
+
+__device__ __forceinline__ uint32_t tile_class(const int32_t* __restrict__ offsets, uint32_t t, uint32_t n_tiles,
+                                               uint32_t n_live) {
     int32_t start, end;
-    fused_tile(offsets, t, C * nt, n_live, start, end);
+    fused_tile(offsets, t, n_tiles, n_live, start, end);
     const float cls = floorf(8.f * __log2f(float(end - start) + 1.f));
-    keys[t] = (xcd_band(t, C * nt) << 8) | (255u - min(255u, uint32_t(cls)));
-    vals[t] = t;
+    return 255u - min(255u, uint32_t(cls));
+}
+
+__global__ void __launch_bounds__(kTileOrderThreads)
+fused_tile_order_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n, uint32_t capacity,
+                        uint32_t n_tiles, uint32_t* __restrict__ order) {
+    __shared__ uint32_t slot[kTileClasses];
+    const uint32_t x = blockIdx.x, per = n_tiles / 8u, rem = n_tiles % 8u;
+    const uint32_t lo = x * per + min(x, rem), hi = lo + per + (x < rem ? 1u : 0u);
+    const uint32_t n_live = live(d_n, capacity);
+    for (uint32_t c = threadIdx.x; c < kTileClasses; c += blockDim.x) slot[c] = 0u;
+    __syncthreads();
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x)
+        atomicAdd(&slot[tile_class(offsets, t, n_tiles, n_live)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // exclusive scan of the class counts, 4 per lane
+        constexpr uint32_t kPer = kTileClasses / kWave;
+        const uint32_t lane = threadIdx.x;
+        uint32_t c[kPer], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            c[j] = slot[lane * kPer + j];
+            sum += c[j];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (uint32_t off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            slot[lane * kPer + j] = run;
+            run += c[j];
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x)
+        order[lo + atomicAdd(&slot[tile_class(offsets, t, n_tiles, n_live)], 1u)] = t;
 }
 
 // Workgroup b of a grid of 8k groups, renumbered so that the k consecutive
@@ -1640,7 +1674,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
 // Forward scratch layout (released by the caller after the forward)
 // ---------------------------------------------------------------------------
 struct FusedScratch {
-    size_t rects, scan, sort, sort_bytes, keys, vals, order_sort, order_sort_bytes, order_keys, tm, lengths, cm,
+    size_t rects, scan, sort, sort_bytes, keys, vals, tm, lengths, cm,
         tile_scan, bytes;
     // direct placement (place = true) or sort + pack
     bool place;
@@ -1697,9 +1731,6 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
         L.range_counts = o; o += al(size_t(C) * L.n_ranges * L.n_bins * 4 + 4);
         L.bin_totals = o; o += al(size_t(C) * L.n_bins * 4 + 4);
         L.bin_scan = o; o += al(scan_workspace_bytes(uint64_t(C) * L.n_bins));
-        L.order_sort_bytes = sort_workspace_bytes(n_tiles);
-        L.order_sort = o; o += al(L.order_sort_bytes);
-        L.order_keys = o; o += al(size_t(n_tiles) * 4 + 4);
         L.bytes = o;
         return L;
     }
@@ -1707,9 +1738,6 @@ static FusedScratch fused_scratch(uint32_t C, uint32_t N, uint32_t n_tiles, uint
     L.sort = o; o += al(L.sort_bytes);
     L.keys = o; o += al(size_t(capacity) * 4 + 4);
     L.vals = o; o += al(size_t(capacity) * 4 + 4);
-    L.order_sort_bytes = sort_workspace_bytes(n_tiles);
-    L.order_sort = o; o += al(L.order_sort_bytes);
-    L.order_keys = o; o += al(size_t(n_tiles) * 4 + 4);
     L.tm = o; o += al(size_t(n_tiles) * 4 + 4);
     L.lengths = o; o += al(size_t(n_tiles) * 4 + 4);
     L.cm = o; o += al(size_t(n_tiles) * 4 + 8);
@@ -1943,16 +1971,10 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         cm, counters, capacity, entries, tile_offsets);
     if (int rc = check_launch("fused_pack")) return rc;
     }
-    if (tile_order) {
-        void* order_ws = at<void>(scratch, L.order_sort);
-        fused_tile_order_keys_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(
-            tile_offsets, counters, capacity, C, ntx * nty, sort_keys_buffer(order_ws),
-            sort_vals_buffer(order_ws, n_tiles));
+    if (tile_order && n_tiles > 0) {
+        fused_tile_order_kernel<<<8, kTileOrderThreads, 0, stream>>>(tile_offsets, counters, capacity, n_tiles,
+                                                                     tile_order);
         if (int rc = check_launch("fused_tile_order")) return rc;
-        if (int rc = sort_pairs_u32(order_ws, L.order_sort_bytes, n_tiles, nullptr, kTileOrderBits,
-                                    at<uint32_t>(scratch, L.order_keys),
-                                    tile_order, stream))
-            return rc;
     }
     return R2S_OK;
 }
