@@ -105,6 +105,11 @@ emit_3d_kernel(const S* __restrict__ means3d, const int32_t* __restrict__ radii,
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+// arrays up to 16384 elements (and the partials of arrays up to 2^25) are
+// scanned by one workgroup of 1024 threads in a single launch
+constexpr int kScanBigThreads = 1024;
+constexpr int kScanBigItems = 16;
+constexpr uint32_t kScanBigTile = kScanBigThreads * kScanBigItems;
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
     const int lane = lane_id();
@@ -154,17 +159,20 @@ scan_reduce_kernel(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __rest
     }
 }
 
-// Scans one tile per block; block b adds base[b] (if given). Writes out[n] =
-// total when the block holds the last element and `write_total` is set.
-__global__ void __launch_bounds__(kScanThreads)
+// Scans one tile of THREADS x ITEMS elements per block; block b adds base[b]
+// (if given). The block holding the last element writes out[n] = total (and
+// *d_total when given).
+template <int THREADS, int ITEMS>
+__global__ void __launch_bounds__(THREADS)
 scan_tile_kernel(const uint32_t* __restrict__ in, uint32_t n, const uint32_t* __restrict__ base,
                  uint32_t* __restrict__ out, uint32_t* __restrict__ d_total) {
-    __shared__ uint32_t lds[kScanThreads / kWave];
-    const size_t tile0 = size_t(blockIdx.x) * kScanTile;
-    const size_t t0 = tile0 + size_t(threadIdx.x) * kScanItems;
-    uint32_t v[kScanItems], s = 0;
+    constexpr uint32_t kTile = uint32_t(THREADS) * ITEMS;
+    __shared__ uint32_t lds[THREADS / kWave];
+    const size_t tile0 = size_t(blockIdx.x) * kTile;
+    const size_t t0 = tile0 + size_t(threadIdx.x) * ITEMS;
+    uint32_t v[ITEMS], s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         v[k] = (t0 + k < n) ? in[t0 + k] : 0u;
         s += v[k];
     }
@@ -173,11 +181,11 @@ scan_tile_kernel(const uint32_t* __restrict__ in, uint32_t n, const uint32_t* __
     const uint32_t b = base ? base[blockIdx.x] : 0u;
     uint32_t run = b + excl;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         if (t0 + k < n) out[t0 + k] = run;
         run += v[k];
     }
-    const bool last_block = (tile0 + kScanTile >= n);
+    const bool last_block = (tile0 + kTile >= n);
     if (last_block && threadIdx.x == 0) {
         out[n] = b + total;
         if (d_total) *d_total = b + total;
@@ -197,8 +205,13 @@ static size_t scan_levels_bytes(uint64_t n) {
 int scan_exclusive(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* d_total, void* ws,
                    hipStream_t stream) {
     if (n <= kScanTile) {
-        scan_tile_kernel<<<1, kScanThreads, 0, stream>>>(in, n, nullptr, out, d_total);
+        scan_tile_kernel<kScanThreads, kScanItems><<<1, kScanThreads, 0, stream>>>(in, n, nullptr, out, d_total);
         return check_launch("scan_tile");
+    }
+    if (n <= kScanBigTile) {
+        scan_tile_kernel<kScanBigThreads, kScanBigItems><<<1, kScanBigThreads, 0, stream>>>(in, n, nullptr, out,
+                                                                                            d_total);
+        return check_launch("scan_block");
     }
     const uint32_t nb = ceil_div(n, kScanTile);
     uint32_t* partial = static_cast<uint32_t*>(ws);
@@ -206,7 +219,7 @@ int scan_exclusive(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* d_to
     scan_reduce_kernel<<<nb, kScanThreads, 0, stream>>>(in, n, partial);
     if (int rc = check_launch("scan_reduce")) return rc;
     if (int rc = scan_exclusive(partial, nb, partial, nullptr, static_cast<char*>(ws) + used, stream)) return rc;
-    scan_tile_kernel<<<nb, kScanThreads, 0, stream>>>(in, n, partial, out, d_total);
+    scan_tile_kernel<kScanThreads, kScanItems><<<nb, kScanThreads, 0, stream>>>(in, n, partial, out, d_total);
     return check_launch("scan_apply");
 }
 

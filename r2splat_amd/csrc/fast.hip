@@ -413,11 +413,8 @@ struct CamFrame {
     FovLimits<double> fov;
 };
 
-__global__ void __launch_bounds__(64)
-camera_frames_kernel(const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t C, uint32_t W,
-                     uint32_t H, CamFrame* __restrict__ out) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
+__device__ __forceinline__ void camera_frame(const float* __restrict__ viewmats, const float* __restrict__ Ks,
+                                             uint32_t c, uint32_t W, uint32_t H, CamFrame* __restrict__ out) {
     CamFrame f;
     f.cam.load(viewmats + 16 * size_t(c), Ks + 9 * size_t(c));
     f.fov = fov_limits(f.cam, double(W), double(H));
@@ -425,14 +422,19 @@ camera_frames_kernel(const float* __restrict__ viewmats, const float* __restrict
 }
 
 // Copies of the primitive parameters in the sorted order (perm[p] = input
-// index), and each primitive's PrimFrame.
+// index), and each primitive's PrimFrame; workgroup 0 also writes the C
+// camera frames.
 __global__ void __launch_bounds__(256)
 permute_params_kernel(const uint32_t* __restrict__ perm, uint32_t N, const float* __restrict__ means,
                       const float* __restrict__ quats, const float* __restrict__ scales,
                       const float* __restrict__ dens, const float* __restrict__ betas,
                       const float* __restrict__ ratio, const float* __restrict__ unc, float* __restrict__ o_means,
                       float* __restrict__ o_dens, float* __restrict__ o_betas, float* __restrict__ o_ratio,
-                      float* __restrict__ o_unc, PrimFrame* __restrict__ o_frames) {
+                      float* __restrict__ o_unc, PrimFrame* __restrict__ o_frames,
+                      const float* __restrict__ viewmats, const float* __restrict__ Ks, uint32_t C, uint32_t W,
+                      uint32_t H, CamFrame* __restrict__ cams) {
+    if (blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < C; c += blockDim.x) camera_frame(viewmats, Ks, c, W, H, cams);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const size_t n = perm[p];
@@ -637,25 +639,32 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
 // which workgroup runs a tile when, never a result.
 constexpr uint32_t kTileClasses = 256, kTileOrderThreads = 1024;
 
-__device__ __forceinline__ uint32_t tile_class(const int32_t* __restrict__ offsets, uint32_t t, uint32_t n_tiles,
-                                               uint32_t n_live) {
-    int32_t start, end;
-    fused_tile(offsets, t, n_tiles, n_live, start, end);
+// Tile t's list starts at starts[t * stride] (stride 1: the tile offsets;
+// kBinClasses: the direct placement's scanned bin starts, a tile's first bin)
+// and ends where tile t + 1's starts (the last at the live count).
+__device__ __forceinline__ uint32_t tile_class(const uint32_t* __restrict__ starts, uint32_t stride, uint32_t t,
+                                               uint32_t n_tiles, uint32_t n_live) {
+    const uint32_t start = min(starts[size_t(t) * stride], n_live);
+    const uint32_t end = max(start, min(t + 1 == n_tiles ? n_live : starts[size_t(t + 1) * stride], n_live));
     const float cls = floorf(8.f * __log2f(float(end - start) + 1.f));
     return 255u - min(255u, uint32_t(cls));
 }
 
+// offsets_out (when given) also receives the tile offsets, starts[t * stride].
 __global__ void __launch_bounds__(kTileOrderThreads)
-fused_tile_order_kernel(const int32_t* __restrict__ offsets, const uint32_t* __restrict__ d_n, uint32_t capacity,
-                        uint32_t n_tiles, uint32_t* __restrict__ order) {
+fused_tile_order_kernel(const uint32_t* __restrict__ starts, uint32_t stride, const uint32_t* __restrict__ d_n,
+                        uint32_t capacity, uint32_t n_tiles, uint32_t* __restrict__ order,
+                        int32_t* __restrict__ offsets_out) {
     __shared__ uint32_t slot[kTileClasses];
     const uint32_t x = blockIdx.x, per = n_tiles / 8u, rem = n_tiles % 8u;
     const uint32_t lo = x * per + min(x, rem), hi = lo + per + (x < rem ? 1u : 0u);
     const uint32_t n_live = live(d_n, capacity);
     for (uint32_t c = threadIdx.x; c < kTileClasses; c += blockDim.x) slot[c] = 0u;
     __syncthreads();
-    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x)
-        atomicAdd(&slot[tile_class(offsets, t, n_tiles, n_live)], 1u);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        if (offsets_out) offsets_out[t] = int32_t(starts[size_t(t) * stride]);
+        atomicAdd(&slot[tile_class(starts, stride, t, n_tiles, n_live)], 1u);
+    }
     __syncthreads();
     if (threadIdx.x < kWave) {  // exclusive scan of the class counts, 4 per lane
         constexpr uint32_t kPer = kTileClasses / kWave;
@@ -681,7 +690,7 @@ fused_tile_order_kernel(const int32_t* __restrict__ offsets, const uint32_t* __r
     }
     __syncthreads();
     for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x)
-        order[lo + atomicAdd(&slot[tile_class(offsets, t, n_tiles, n_live)], 1u)] = t;
+        order[lo + atomicAdd(&slot[tile_class(starts, stride, t, n_tiles, n_live)], 1u)] = t;
 }
 
 // Workgroup b of a grid of 8k groups, renumbered so that the k consecutive
@@ -864,9 +873,12 @@ struct RectWalk {
 __global__ void __launch_bounds__(kWave)
 fused_bin_count_kernel(const uint2* __restrict__ rects, const float2* __restrict__ reach, uint32_t N,
                        uint32_t range_len, uint32_t n_ranges, uint32_t ntx, uint32_t T, uint32_t n_bins,
-                       uint32_t* __restrict__ range_counts) {
+                       uint32_t* __restrict__ range_counts, uint32_t* __restrict__ counters, uint32_t capacity) {
     extern __shared__ uint32_t hist[];
     const uint32_t range = blockIdx.x, cam = blockIdx.y, lane = threadIdx.x;
+    // counters = {intersections (the scan's total), overflow}: the flag is
+    // raised when the count exceeds the capacity
+    if (range == 0 && cam == 0 && lane == 0) counters[1] = counters[0] > capacity ? 1u : 0u;
     for (uint32_t b = lane; b < n_bins; b += kWave) hist[b] = 0u;
     __syncthreads();
     const uint32_t n0 = range * range_len, n1 = min(N, n0 + range_len);
@@ -952,12 +964,6 @@ fused_bin_tile_offsets_kernel(const uint32_t* __restrict__ bin_starts, uint32_t 
                               int32_t* __restrict__ tile_offsets) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n_tiles) tile_offsets[t] = int32_t(bin_starts[size_t(t) * kBinClasses]);
-}
-
-// counters = {intersections, overflow}: the flag is raised when the count
-// exceeds the capacity.
-__global__ void fused_overflow_kernel(uint32_t* __restrict__ counters, uint32_t capacity) {
-    if (threadIdx.x == 0) counters[1] = counters[0] > capacity ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1900,13 +1906,12 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
             return rc;
         permute_params_kernel<<<ceil_div(N, 256), 256, 0, stream>>>(
             perm, N, means, quats, scales, densities, BETA ? betas : nullptr, BETA ? beta_ratio : nullptr,
-            VAR ? uncertainties : nullptr, p_means, p_dens, p_betas, p_ratio, p_unc, p_frames);
+            VAR ? uncertainties : nullptr, p_means, p_dens, p_betas, p_ratio, p_unc, p_frames, viewmats, Ks, C,
+            width, height, at<CamFrame>(scratch, L.cams));
         if (int rc = check_launch("permute_params")) return rc;
     }
-    if (CN > 0) {
-        CamFrame* cams = at<CamFrame>(scratch, L.cams);
-        camera_frames_kernel<<<ceil_div(C, 64), 64, 0, stream>>>(viewmats, Ks, C, width, height, cams);
-        if (int rc = check_launch("camera_frames")) return rc;
+    if (CN > 0) {  // (N > 0: the permute pass wrote the camera frames)
+        const CamFrame* cams = at<CamFrame>(scratch, L.cams);
         dim3 grid(ceil_div(N, 256), ceil_div(C, kProjectCams));
         fused_project_kernel<BETA, VAR><<<grid, 256, 0, stream>>>(
             p_means, p_frames, p_dens, p_betas, p_ratio, p_unc, cams, C, width, height, eps, radius_clip,
@@ -1915,8 +1920,7 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     }
     if (int rc = scan_exclusive(cum, CN, cum, counters, at<void>(scratch, L.scan), stream)) return rc;
     if (L.place) {
-        fused_overflow_kernel<<<1, kWave, 0, stream>>>(counters, capacity);
-        if (int rc = check_launch("fused_overflow")) return rc;
+        // (with CN = 0 the count is 0 and the zeroed overflow flag stays 0)
         uint32_t* range_counts = at<uint32_t>(scratch, L.range_counts);
         uint32_t* totals = at<uint32_t>(scratch, L.bin_totals);
         const size_t lds = size_t(L.n_bins) * sizeof(uint32_t);
@@ -1924,7 +1928,8 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
         if (CN > 0) {
             fused_bin_count_kernel<<<grid, kWave, lds, stream>>>(at<uint2>(scratch, L.rects),
                                                                  at<float2>(scratch, L.reach), N, L.range_len,
-                                                                 L.n_ranges, ntx, tile_size, L.n_bins, range_counts);
+                                                                 L.n_ranges, ntx, tile_size, L.n_bins, range_counts,
+                                                                 counters, capacity);
             if (int rc = check_launch("fused_bin_count")) return rc;
         }
         const uint32_t n_cols = C * L.n_bins;
@@ -1938,6 +1943,11 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
                 L.range_len, L.n_ranges, ntx, tile_size, L.n_bins, L.bin_bits, range_counts, totals, capacity,
                 entries);
             if (int rc = check_launch("fused_bin_place")) return rc;
+        }
+        if (tile_order && n_tiles > 0) {  // the tile offsets come with the tile order
+            fused_tile_order_kernel<<<8, kTileOrderThreads, 0, stream>>>(totals, kBinClasses, counters, capacity,
+                                                                         n_tiles, tile_order, tile_offsets);
+            return check_launch("fused_tile_order");
         }
         fused_bin_tile_offsets_kernel<<<ceil_div(n_tiles, 256), 256, 0, stream>>>(totals, n_tiles, tile_offsets);
         if (int rc = check_launch("fused_bin_tile_offsets")) return rc;
@@ -1972,8 +1982,9 @@ static int prepare_impl(const float* means, const float* quats, const float* sca
     if (int rc = check_launch("fused_pack")) return rc;
     }
     if (tile_order && n_tiles > 0) {
-        fused_tile_order_kernel<<<8, kTileOrderThreads, 0, stream>>>(tile_offsets, counters, capacity, n_tiles,
-                                                                     tile_order);
+        fused_tile_order_kernel<<<8, kTileOrderThreads, 0, stream>>>(reinterpret_cast<const uint32_t*>(tile_offsets),
+                                                                     1u, counters, capacity, n_tiles, tile_order,
+                                                                     nullptr);
         if (int rc = check_launch("fused_tile_order")) return rc;
     }
     return R2S_OK;
