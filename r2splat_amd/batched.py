@@ -59,22 +59,8 @@ class KernelConfig:
 
 
 def heuristic_config(n: int, views: int, height: int, width: int, beta: bool = False) -> KernelConfig:
-    """A kernel configuration from a job's shape alone: the one ``Autotuner``
-    picks most often for jobs of that shape. Measured on MI355X over 56 jobs
-    of the serving mix (N 10k-400k, 16-128 views of 128^2-512^2, Gaussian and
-    Beta), its prepare + render + raster VJP time is within 1% of tuning
-    every job and 5.5% below tile 16 with 8 waves. Gaussians on 512^2 images
-    take 32-pixel tiles once views x N reaches 500k; Beta primitives on
-    512^2 images measured fastest with 16-pixel tiles; small images take 16
-    waves per tile."""
-    side = min(height, width)
-    if side >= 512:
-        if beta:
-            return KernelConfig(16, 4, 4)
-        return KernelConfig(32 if views * n >= 500_000 else 16, 8 if views <= 32 and n >= 60_000 else 4, 4)
-    if side >= 256:
-        return KernelConfig(16, 8 if views <= 16 else 4, 4 if beta else 8)
-    return KernelConfig(16, 16, 16) if views <= 32 else KernelConfig(16, 8, 16)
+    """A kernel configuration from a job's shape alone (``fused.config_for_shape``)."""
+    return KernelConfig(*fused.config_for_shape(n, views, height, width, beta))
 
 
 @dataclass
@@ -199,7 +185,9 @@ class BatchedFit:
     ``jobs``: sequence of dicts with means, quats, scales, densities, viewmats,
     Ks, target (float32, on one device) and optionally ``beta`` (Beta
     primitives), a ``config`` (KernelConfig) and ``capacity``. Gaussian and
-    Beta jobs mix freely in one batch.
+    Beta jobs mix freely in one batch. A job without a config takes the
+    tuner's (when a tuner is given), else the configuration of its shape
+    (``heuristic_config``).
     """
 
     def __init__(self, jobs, *, adam: Optional[AdamConfig] = AdamConfig(), n_streams: int = 4,
@@ -225,7 +213,9 @@ class BatchedFit:
                                  radius_clip=_clip(job, radius_clip), beta=job.get("beta"))
                 cfg = rec.config
                 self.tune_records.append(rec)
-            cfg = cfg or KernelConfig()
+            if cfg is None:  # the configuration of the job's shape
+                C, H, W = job["target"].shape
+                cfg = heuristic_config(int(job["means"].shape[0]), int(C), int(H), int(W), job.get("beta") is not None)
             s = min(range(len(self.streams)), key=lambda k: load[k])
             load[s] += self._cost(job)
             ws = Workspace(self.device) if self.packed else stream_ws[s]

@@ -204,8 +204,12 @@ def _raster_state(prims, viewmats, Ks, width, height, eps, radius_clip) -> dict:
     means, quats, scales, densities, beta, uncertainty = prims
     C = viewmats.shape[0]
     if _use_fused(means):
-        return {"fused": fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps,
-                                       radius_clip, beta=beta, uncertainty=uncertainty)}
+        f = fused.prepare(means, quats, scales, densities, viewmats, Ks, width, height, eps, radius_clip, beta=beta,
+                          uncertainty=uncertainty)
+        # the waves per tile of the configuration for this shape (the tile
+        # edge stays the operators' 16)
+        _, f.fwd_waves, f.bwd_waves = fused.config_for_shape(means.shape[0], C, height, width, beta is not None)
+        return {"fused": f}
     T = RASTER_TILE
     with stage("project"):
         covars, _ = ops.parameter_transform_fwd(quats, scales)
@@ -231,7 +235,7 @@ def _raster_render(b, prims, shape):
     means, _, _, _, beta, uncertainty = prims
     C, height, width = shape
     if "fused" in b:
-        return fused.render(b["fused"], C, means.device)
+        return fused.render(b["fused"], C, means.device, b["fused"].fwd_waves)
     T = RASTER_TILE
     prefix = "beta_" if beta is not None else ""
     args = (b["mapped"], b["conics"], b["opacity"], b["mus"]) + (() if beta is None else (b["beta2d"],))
@@ -308,7 +312,8 @@ def rasterize_bwd(context: _Context, v_image: Optional[Tensor], v_variance: Opti
         b.update(_raster_state(context.primitives, b["viewmats"], b["Ks"], width, height, *b.pop("recompute")))
     if "fused" in b:
         grads = fused.backward(b["fused"], means, quats, scales, densities, b["viewmats"], b["Ks"], v_image,
-                               v_variance=v_variance, detach_variance=context.detach_variance)
+                               waves=b["fused"].bwd_waves, v_variance=v_variance,
+                               detach_variance=context.detach_variance)
         v_means, v_quats, v_scales, v_densities = grads[:4]
         v_beta = grads[4] if beta is not None else None
         v_uncertainty = grads[-1] if uncertainty is not None else None

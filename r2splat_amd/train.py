@@ -91,12 +91,17 @@ class AdamConfig:
 
 class GaussianFit:
     """One job: N Gaussians (or Beta primitives, when ``beta`` [N] is given),
-    C views of H x W, MSE against ``target`` [C, H, W]."""
+    C views of H x W, MSE against ``target`` [C, H, W]. ``fwd_waves`` /
+    ``bwd_waves`` None take the waves of the job's shape
+    (``fused.config_for_shape``), as the explicit API does, so that a default
+    fit steps exactly as the explicit path computes; 0 is the kernels' default
+    (8)."""
 
     def __init__(self, means, quats, scales, densities, viewmats, Ks, target, *, eps: float = 1e-8,
                  radius_clip: Optional[float] = None, adam: Optional[AdamConfig] = None, capacity: Optional[int] = None,
                  headroom: float = 1.15, stream: Optional[torch.cuda.Stream] = None, tile_size: int = TILE,
-                 fwd_waves: int = 0, bwd_waves: int = 0, workspace: Optional[Workspace] = None, beta=None):
+                 fwd_waves: Optional[int] = None, bwd_waves: Optional[int] = None,
+                 workspace: Optional[Workspace] = None, beta=None):
         device = means.device
         if means.dtype != torch.float32:
             raise TypeError("GaussianFit runs the float32 fused pipeline")
@@ -111,7 +116,10 @@ class GaussianFit:
         self.eps, self.radius_clip = float(eps), float(radius_clip)
         self.adam = adam
         self.headroom = headroom
-        self.tile_size, self.fwd_waves, self.bwd_waves = int(tile_size), int(fwd_waves), int(bwd_waves)
+        _, shape_fwd, shape_bwd = fused.config_for_shape(self.N, self.C, self.H, self.W, beta is not None)
+        self.tile_size = int(tile_size)
+        self.fwd_waves = shape_fwd if fwd_waves is None else int(fwd_waves)
+        self.bwd_waves = shape_bwd if bwd_waves is None else int(bwd_waves)
         self.stream = stream
         N = self.N
         self.kind = int(beta is not None)

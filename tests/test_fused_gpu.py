@@ -149,6 +149,30 @@ def test_fused_tile_offsets_match_operator_binning(device):
     assert torch.equal(f.record_offsets[:CN + 1].long(), expect)
 
 
+@pytest.mark.parametrize("W,H,T", [(80, 96, 16), (520, 504, 16), (40, 24, 8)])
+def test_tile_order_is_banded_longest_first(device, W, H, T):
+    """The tile order (direct placement; 33 x 32 tiles per camera: sort and
+    pack): a permutation of the tiles in which XCD band x (per + (x < rem)
+    tiles of the camera-major index from x per + min(x, rem)) holds exactly
+    its own tiles, in non-increasing size class floor(8 log2(count + 1)):
+    each tile's count + 1 is below 2^(1/8) times its predecessor's."""
+    sc = workloads.scene(2500, 3, H, W, scale_range=(0.01, 0.08), seed=13, device=device)
+    f = fused.prepare(*sc.params(), sc.viewmats, sc.Ks, W, H, 1e-8, 3.0, tile_size=T)
+    n = f.tile_order.numel()
+    order = f.tile_order.long()
+    assert torch.equal(torch.sort(order).values, torch.arange(n, device=device))
+    off = f.offsets.reshape(-1).long()
+    ends = torch.cat([off[1:], torch.tensor([f.n_isects], device=device)])
+    size = (ends - off).double() + 1
+    per, rem = n // 8, n % 8
+    for x in range(8):
+        lo = x * per + min(x, rem)
+        hi = lo + per + (1 if x < rem else 0)
+        band = order[lo:hi]
+        assert bool(((band >= lo) & (band < hi)).all()), x
+        assert bool((size[band][1:] <= size[band][:-1] * 2 ** 0.125 * (1 + 1e-6)).all()), x
+
+
 @pytest.mark.parametrize("use_beta", [False, True])
 def test_packed_lists_hold_each_emission_once_in_its_tile(device, use_beta):
     """The packed tile lists are a permutation of the emission order: each
