@@ -46,7 +46,7 @@ PLAIN_CHECK_STEPS = 5
 # --no-graph`; profiles/raster_100k_64x256_occupancy_valu.md). The kernels
 # evaluate only the rows (backward) and row groups (forward) each record can
 # reach, so they issue far fewer operations than pairs x FLOP_PER_PAIR.
-VALU_INSTS = {"rasterize_fwd": 7.370e8, "rasterize_bwd": 8.121e8}
+VALU_INSTS = {"rasterize_fwd": 7.333e8, "rasterize_bwd": 8.121e8}
 SIMDS = 256 * 4          # compute units x SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
 # Bytes each launch of the raster kernels reads and writes beyond L2 on this
@@ -59,7 +59,7 @@ CLOCK_HZ = 2.4e9         # peak engine clock
 # counts exactly its 268,435,456 bytes each way this way; FETCH_SIZE counts
 # its 128-byte reads as 64 bytes (half). Summary and commands:
 # profiles/raster_100k_64x256_hbm_traffic.md.
-TRAFFIC_BYTES = {"rasterize_fwd": (2_061_623_776, 17_357_984), "rasterize_bwd": (2_386_446_336, 922_386_400)}
+TRAFFIC_BYTES = {"rasterize_fwd": (2_303_045_248, 17_313_472), "rasterize_bwd": (2_398_399_040, 921_594_144)}
 
 
 def stage_rooflines(stages_ms, N, CN, I, kept_fwd, kept_bwd):
