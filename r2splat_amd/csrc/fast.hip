@@ -1531,7 +1531,7 @@ fused_keep_mask_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR
 // (r2splat/explicit.py:227-231).
 // ---------------------------------------------------------------------------
 constexpr int kParamFields = 12;  // v_mean 3, v_covariance 6, v_density, v_beta, v_uncertainty
-constexpr int kVjpWaves = 16;
+constexpr int kVjpWaves = 8;  // 8 waves fit the kernel in registers without spills
 // partials one wave stages at a time (its share of the LDS buffer, 8 floats each)
 constexpr uint32_t kVjpStageParts = kParamFields * kWave / kPartialStride;
 
