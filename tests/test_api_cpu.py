@@ -155,3 +155,21 @@ def test_bench_refuses_more_than_one_gpu():
                           timeout=120, cwd=ROOT)
     assert proc.returncode == 2
     assert "one GPU" in json.loads(proc.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_shape_configurations_are_valid_kernel_configurations():
+    """fused.config_for_shape gives a tile edge the fused kernels support and
+    wave counts they are built for, for every shape of the serving mix and
+    beyond; Beta primitives and images up to 256^2 keep 16-pixel tiles."""
+    from r2splat_amd import fused
+    from r2splat_amd.batched import KernelConfig, heuristic_config
+
+    for n in (1, 1000, 10_000, 60_000, 400_000, 2_000_000):
+        for views in (1, 16, 32, 64, 128, 256):
+            for side in (32, 128, 256, 300, 512, 1024):
+                for beta in (False, True):
+                    tile, fw, bw = fused.config_for_shape(n, views, side, side, beta)
+                    assert tile in (8, 16, 32) and fw in (4, 8, 16) and bw in (4, 8, 16)
+                    if beta or side < 512:
+                        assert tile == 16
+                    assert heuristic_config(n, views, side, side, beta) == KernelConfig(tile, fw, bw)
