@@ -567,3 +567,37 @@ def test_fused_variance_matches_float64_on_kept_pairs(device, use_beta, detach):
     for name in names:
         assert torch.equal(getattr(c, name), getattr(a, name)), name
         assert torch.equal(getattr(d, name), 2.0 * getattr(a, name)), name
+
+
+@pytest.mark.parametrize("case", ["no_primitives", "all_off_screen", "single_tile_grid"])
+def test_fused_float32_degenerate_scenes(device, case):
+    """The float32 fused path through the explicit API on scenes with nothing
+    to draw (no primitives; primitives whose footprints miss every image) and
+    on an image of one tile: images and gradients match the float64 operator
+    path (zero where nothing is drawn)."""
+    H, W = (40, 48) if case != "single_tile_grid" else (12, 10)
+    sc = workloads.scene(64, 3, H, W, scale_range=(0.01, 0.08), seed=21, device=device)
+    means, quats, scales, dens = sc.params()
+    if case == "no_primitives":
+        means, quats, scales, dens = means[:0], quats[:0], scales[:0], dens[:0]
+    elif case == "all_off_screen":
+        means = means + torch.tensor([0.0, 50.0, 0.0], device=device)  # far above every orbit camera's view
+    img, ctx = r2splat_amd.rasterize_fwd(means, quats, scales, dens, sc.viewmats, sc.Ks, (H, W))
+    assert img.shape == (3, H, W) and img.dtype == torch.float32
+    v = torch.randn((3, H, W), device=device, generator=torch.Generator(device=device).manual_seed(2))
+    grads = r2splat_amd.rasterize_bwd(ctx, v)
+    ref_img, ref_ctx = r2splat_amd.rasterize_fwd(means.double(), quats.double(), scales.double(), dens.double(),
+                                                 sc.viewmats.double(), sc.Ks.double(), (H, W))
+    ref = r2splat_amd.rasterize_bwd(ref_ctx, v.double())
+    if case != "single_tile_grid":
+        assert torch.count_nonzero(img) == 0
+    assert (img.double() - ref_img).abs().max() <= 1e-5 * max(1.0, float(ref_img.abs().max()))
+    for name in ("means", "quats", "scales", "densities"):
+        got, exp = getattr(grads, name), getattr(ref, name)
+        assert got.shape == exp.shape
+        if exp.numel() == 0:
+            continue
+        if case == "single_tile_grid":
+            assert (got.double() - exp).abs().max() <= 1e-4 * float(exp.abs().max()), name
+        else:  # nothing drawn: exact zeros on both paths
+            assert torch.count_nonzero(got) == 0 and torch.count_nonzero(exp) == 0, name
