@@ -19,12 +19,16 @@ TypeError reports a wrong kind of argument, ValueError a wrong value or shape.
 
 from __future__ import annotations
 
+import ctypes
+import functools
 import math
 from dataclasses import dataclass
 from numbers import Integral, Real
 from typing import Callable, List, Optional, Sequence, Tuple
 
 import torch
+
+from .hip import _backend as be
 
 Tensor = torch.Tensor
 INT32_MAX = 2**31 - 1
@@ -40,30 +44,80 @@ def capturing() -> bool:
 # deferred value checks
 # ---------------------------------------------------------------------------
 class ValueChecks:
-    """Device-side predicates ("some element is bad") evaluated together.
+    """Predicates ("some element is bad") evaluated together.
 
-    ``run()`` stacks every predicate into one boolean vector, copies it to the
-    host once and raises ValueError with the message of the first predicate
-    that holds, in the order they were added.
+    Elementwise checks of a whole buffer (finiteness, sign, all-zero rows)
+    run as one native launch per 16 (r2s_value_checks); other predicates are
+    torch reductions; results already known on the host keep their place in
+    the order. ``run()`` copies every flag to the host once and raises
+    ValueError with the message of the first predicate that holds, in the
+    order they were added.
     """
 
     def __init__(self):
-        self._pending: List[Tuple[str, Callable[[], Tensor]]] = []
+        # (message, check): check is a callable (torch predicate), a bool
+        # (known on the host) or (tensor, kind, row) (native elementwise)
+        self._pending: List[Tuple[str, object]] = []
 
     def add(self, message: str, any_bad: Callable[[], Tensor]) -> None:
         self._pending.append((message, any_bad))
 
+    def known(self, message: str, bad: bool) -> None:
+        self._pending.append((message, bool(bad)))
+
+    def element(self, message: str, value: Tensor, kind: int, row: int = 1) -> None:
+        """Some element of `value` (or some row of `row` elements, kind
+        CHECK_ZERO_ROW) is bad; `kind` one of the CHECK_* codes."""
+        self._pending.append((message, (value.contiguous(), kind, row)))
+
     def finite(self, name: str, value: Tensor) -> None:
-        self.add(f"{name} must contain only finite values", lambda: ~torch.isfinite(value).all())
+        self.element(f"{name} must contain only finite values", value, be.CHECK_NONFINITE)
 
     def run(self) -> None:
         pending, self._pending = self._pending, []
         if not pending or capturing():
             return
-        flags = torch.stack([fn().reshape(()) for _, fn in pending]).tolist()
-        for (message, _), bad in zip(pending, flags):
-            if bad:
+        native = [i for i, (_, c) in enumerate(pending) if isinstance(c, tuple)]
+        device = pending[native[0]][1][0].device if native else None
+        for i in list(native):
+            value, kind, row = pending[i][1]
+            if value.device != device or value.dtype not in FLOATS:
+                native.remove(i)
+                pending[i] = (pending[i][0], functools.partial(_element_predicate, value, kind, row))
+        reduced = [i for i, (_, c) in enumerate(pending) if callable(c)]
+        parts = []
+        if native:
+            flags = torch.empty((len(native),), dtype=torch.int32, device=device)
+            with torch.cuda.device(device):
+                for lo in range(0, len(native), be.MAX_VALUE_CHECKS):
+                    chunk = native[lo: lo + be.MAX_VALUE_CHECKS]
+                    table = (be.ValueCheck * len(chunk))()
+                    for entry, i in zip(table, chunk):
+                        value, kind, row = pending[i][1]
+                        entry.data, entry.n, entry.dtype = value.data_ptr(), value.numel(), be.dtype_code(value.dtype)
+                        entry.kind, entry.row = kind, row
+                    be.call("r2s_value_checks", ctypes.addressof(table), len(chunk), flags[lo:].data_ptr(),
+                            be.stream_of(device))
+            parts.append(flags)
+        if reduced:
+            parts.append(torch.stack([pending[i][1]().reshape(()) for i in reduced]).to(torch.int32))
+        values = (torch.cat(parts) if len(parts) > 1 else parts[0]).tolist() if parts else []
+        bad = dict(zip(native + reduced, values))
+        for i, (message, c) in enumerate(pending):
+            if (c if isinstance(c, bool) else bad[i]):
                 raise ValueError(message)
+
+
+def _element_predicate(value: Tensor, kind: int, row: int) -> Tensor:
+    """The torch form of a native elementwise check (buffers the native launch
+    cannot take: another device or dtype)."""
+    if kind == be.CHECK_NONFINITE:
+        return ~torch.isfinite(value).all()
+    if kind == be.CHECK_NONPOSITIVE:
+        return (value <= 0).any()
+    if kind == be.CHECK_NEGATIVE:
+        return (value < 0).any()
+    return (value.reshape(-1, row) == 0).all(dim=-1).any()
 
 
 # ---------------------------------------------------------------------------
@@ -122,23 +176,23 @@ class _Field:
     name: str
     trailing: tuple  # dims after the primitive count
     required: bool = True
-    violation: Optional[Callable[[Tensor], Tensor]] = None  # elementwise "bad value"
+    violation: Optional[int] = None  # elementwise "bad value": a CHECK_* code of r2s_value_checks
     message: str = ""
 
 
 _FIELDS = (
     _Field("means", (3,)),
-    _Field("quats", (4,), True, lambda t: (t == 0).all(dim=-1), "quats must have nonzero norm"),
-    _Field("scales", (3,), True, lambda t: t <= 0, "scales must be strictly positive physical lengths"),
-    _Field("densities", (), True, lambda t: t < 0, "densities must be nonnegative"),
-    _Field("beta", (), False, lambda t: t <= 0, "beta must contain positive 3D exponents"),
-    _Field("uncertainty", (), False, lambda t: t < 0, "uncertainty must be nonnegative relative variance"),
+    _Field("quats", (4,), True, be.CHECK_ZERO_ROW, "quats must have nonzero norm"),
+    _Field("scales", (3,), True, be.CHECK_NONPOSITIVE, "scales must be strictly positive physical lengths"),
+    _Field("densities", (), True, be.CHECK_NEGATIVE, "densities must be nonnegative"),
+    _Field("beta", (), False, be.CHECK_NONPOSITIVE, "beta must contain positive 3D exponents"),
+    _Field("uncertainty", (), False, be.CHECK_NEGATIVE, "uncertainty must be nonnegative relative variance"),
 )
 
 
 def _domain(checks: ValueChecks, field: _Field, value: Tensor) -> None:
     if field.violation is not None:
-        checks.add(field.message, lambda: field.violation(value).any())
+        checks.element(field.message, value, field.violation, field.trailing[0] if field.trailing else 1)
 
 
 def primitives(values: Sequence, checks: ValueChecks) -> tuple:

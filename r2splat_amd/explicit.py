@@ -22,6 +22,7 @@ import torch
 from . import _validation as check
 from . import fused
 from . import hip as ops
+from .hip import _backend as be
 from .hip import _wrapper as _ops_impl
 from ._beta import beta_path_log_grad, beta_path_ratio
 from .profiling import stage
@@ -165,14 +166,49 @@ def _cameras(viewmats, Ks, means, height: int, width: int, checks: check.ValueCh
     return viewmats, Ks
 
 
+_FRAME_CACHE: dict = {}
+
+
+def _constant_vector(value, means):
+    """A volume-frame 3-vector given as Python numbers: (in the primitives'
+    dtype, in float64 as given, nonfinite?, nonpositive?) with the flags of
+    the rounded vector, built once per value, device and dtype (the tensors
+    are never written), or None for any other form of the argument."""
+    if not (isinstance(value, (tuple, list)) and len(value) == 3 and all(type(v) in (int, float) for v in value)):
+        return None
+    key = (tuple(value), means.device, means.dtype)
+    hit = _FRAME_CACHE.get(key)
+    if hit is None:
+        try:
+            rounded = torch.as_tensor(value, dtype=means.dtype)
+            exact = torch.as_tensor(value, dtype=torch.float64)
+        except (OverflowError, RuntimeError, TypeError):
+            return None
+        if len(_FRAME_CACHE) >= 64:
+            _FRAME_CACHE.clear()
+        hit = (rounded.to(means.device), exact.to(means.device), not bool(torch.isfinite(rounded).all()),
+               bool((rounded <= 0).any()))
+        _FRAME_CACHE[key] = hit
+    return hit
+
+
 def _volume_frame(extent, center, means, checks: check.ValueChecks):
     """volume_extent (positive) and volume_center: 3-vectors, constant. ->
     (extent, center) in the primitives' dtype, and both in float64 as given
-    (Python numbers are not rounded to float32 there)."""
-    out, exact = [], []
+    (Python numbers are not rounded to float32 there). Python-number vectors
+    are checked on the host (in the same order as device values)."""
+    out, exact, nonpositive = [], [], None
     for name, value in (("volume_extent", extent), ("volume_center", center)):
         check.constant(name, value)
         given = value
+        hit = _constant_vector(value, means)
+        if hit is not None:
+            out.append(hit[0])
+            exact.append(hit[1])
+            checks.known(f"{name} must contain only finite values", hit[2])
+            if name == "volume_extent":
+                nonpositive = hit[3]
+            continue
         if not isinstance(value, Tensor):
             value = torch.as_tensor(value, device=means.device, dtype=means.dtype)
         value = check.tensor(name, value, (3,), means)
@@ -182,8 +218,11 @@ def _volume_frame(extent, center, means, checks: check.ValueChecks):
             exact.append(value.double().contiguous())
         else:
             exact.append(torch.as_tensor(given, device=means.device, dtype=torch.float64).reshape(3).contiguous())
-    extent = out[0]
-    checks.add("volume_extent must contain positive physical lengths", lambda: (extent <= 0).any())
+    message = "volume_extent must contain positive physical lengths"
+    if nonpositive is None:
+        checks.element(message, out[0], be.CHECK_NONPOSITIVE)
+    else:
+        checks.known(message, nonpositive)
     return out[0], out[1], exact[0], exact[1]
 
 

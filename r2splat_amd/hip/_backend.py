@@ -26,6 +26,16 @@ D = ctypes.c_double
 SZ = ctypes.c_size_t
 
 
+class ValueCheck(ctypes.Structure):
+    """One entry of an r2s_value_checks batch (r2s_value_check)."""
+
+    _fields_ = [("data", P), ("n", U64), ("dtype", ctypes.c_int32), ("kind", ctypes.c_int32), ("row", U32),
+                ("reserved", U32)]
+
+
+CHECK_NONFINITE, CHECK_NONPOSITIVE, CHECK_NEGATIVE, CHECK_ZERO_ROW = 0, 1, 2, 3
+MAX_VALUE_CHECKS = 16
+
 
 class RasterJob(ctypes.Structure):
     """One entry of a packed render / raster-VJP table (r2s_raster_job)."""
@@ -40,6 +50,7 @@ class RasterJob(ctypes.Structure):
 SIGNATURES = {
     "r2s_abi_version": [],
     "r2s_last_error": [],
+    "r2s_value_checks": [P, U32, P, P],
     "r2s_parameter_transform_fwd": [I32, P, P, P, P, U32, P],
     "r2s_parameter_transform_bwd": [I32, P, P, P, P, P, U32, P],
     "r2s_world_to_image_projection_fwd": [I32, P, P, P, P, U32, U32, F, F, P, P, P, U32, U32, P],

@@ -75,7 +75,7 @@ def test_ctypes_table_covers_the_header():
 
 def test_library_version_and_workspace_queries_need_no_gpu():
     lib = _backend.lib(require_gpu=False)
-    assert lib.r2s_abi_version() == 4
+    assert lib.r2s_abi_version() == 5
     assert lib.r2s_scan_workspace_bytes(0) > 0
     big = lib.r2s_sort_workspace_bytes(10**7, 15)
     assert big >= 4 * 4 * 10**7
