@@ -38,12 +38,12 @@ typedef struct ihipStream_t* r2s_stream_t; /* identical to hipStream_t */
 #define R2S_F64 1
 
 /* Library identification and error reporting. Version 5: r2s_value_checks
- * (version 4: the fused voxel
- * entry points take the double-precision frame of r2s_voxel_frame (version 3:
- * they take uncertainties and the variance buffers; the TV and L1 loss
- * backward take lambda as a float; version 2: the fused raster entry
- * points take the record kind and the variance buffers, and `packed` holds
- * tile-list entries followed by the projected records)). */
+ * (version 4: the fused voxel entry points take the double-precision frame
+ * of r2s_voxel_frame; version 3: they take uncertainties and the variance
+ * buffers, the TV and L1 loss backward take lambda as a float; version 2:
+ * the fused raster entry points take the record kind and the variance
+ * buffers, and `packed` holds tile-list entries followed by the projected
+ * records). */
 int r2s_abi_version(void);
 const char* r2s_last_error(void);
 
@@ -52,19 +52,24 @@ const char* r2s_last_error(void);
  * for the whole batch replaces the reference's per-argument device
  * reductions (r2splat/_validation.py:14-96, the isfinite / domain checks of
  * r2splat/explicit.py:118-160); the caller copies the flags to the host once.
- * NaN is not <= 0 or < 0 (as in the reference's comparisons): the NONFINITE
- * check of the same buffer reports it. */
+ * Comparisons follow IEEE semantics as the reference's do: NaN is not <= 0
+ * or < 0 but differs from 0 and 1 (the NONFINITE check of the same buffer
+ * reports it first). */
 #define R2S_CHECK_NONFINITE 0   /* some element is NaN or infinite */
 #define R2S_CHECK_NONPOSITIVE 1 /* some element <= 0 */
 #define R2S_CHECK_NEGATIVE 2    /* some element < 0 */
 #define R2S_CHECK_ZERO_ROW 3    /* some row of `row` consecutive elements is all zero */
+#define R2S_CHECK_K_FOCAL 4     /* some 3x3 intrinsics (row 9) has fx <= 0 or fy <= 0 */
+#define R2S_CHECK_K_PINHOLE 5   /* some 3x3 intrinsics (row 9) has nonzero skew or last row != [0, 0, 1] */
+#define R2S_CHECK_VIEW_ROW 6    /* some 4x4 viewmat (row 16) has last row != [0, 0, 0, 1] */
 #define R2S_MAX_VALUE_CHECKS 16
 typedef struct r2s_value_check {
     const void* data; /* device buffer of n elements of dtype */
     uint64_t n;
     int32_t dtype;    /* R2S_F32 / R2S_F64 */
     int32_t kind;     /* R2S_CHECK_* */
-    uint32_t row;     /* R2S_CHECK_ZERO_ROW: elements per row (divides n) */
+    uint32_t row;     /* elements per row (divides n): R2S_CHECK_ZERO_ROW; 9 and 16 for the
+                         camera checks */
     uint32_t reserved;
 } r2s_value_check;
 int r2s_value_checks(const r2s_value_check* checks, uint32_t n_checks, uint32_t* flags,

@@ -117,7 +117,14 @@ def _element_predicate(value: Tensor, kind: int, row: int) -> Tensor:
         return (value <= 0).any()
     if kind == be.CHECK_NEGATIVE:
         return (value < 0).any()
-    return (value.reshape(-1, row) == 0).all(dim=-1).any()
+    m = value.reshape(-1, row)
+    if kind == be.CHECK_K_FOCAL:
+        return (m[:, (0, 4)] <= 0).any()
+    if kind == be.CHECK_K_PINHOLE:
+        return (m[:, (1, 3, 6, 7)] != 0).any() | (m[:, 8] != 1).any()
+    if kind == be.CHECK_VIEW_ROW:
+        return (m[:, 12:15] != 0).any() | (m[:, 15] != 1).any()
+    return (m == 0).all(dim=-1).any()
 
 
 # ---------------------------------------------------------------------------

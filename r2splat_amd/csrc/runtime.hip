@@ -56,12 +56,22 @@ __device__ __forceinline__ bool scan_check(const r2s_value_check& c) {
     const S* p = static_cast<const S*>(c.data);
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     bool bad = false;
-    if (c.kind == R2S_CHECK_ZERO_ROW) {
+    if (c.kind >= R2S_CHECK_ZERO_ROW) {
         const uint64_t rows = c.n / c.row;
         for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += stride) {
-            bool zero = true;
-            for (uint32_t k = 0; k < c.row; ++k) zero = zero && p[r * c.row + k] == S(0);
-            bad = bad || zero;
+            const S* m = p + r * c.row;
+            bool b;
+            if (c.kind == R2S_CHECK_K_FOCAL) {
+                b = m[0] <= S(0) || m[4] <= S(0);
+            } else if (c.kind == R2S_CHECK_K_PINHOLE) {
+                b = m[1] != S(0) || m[3] != S(0) || m[6] != S(0) || m[7] != S(0) || m[8] != S(1);
+            } else if (c.kind == R2S_CHECK_VIEW_ROW) {
+                b = m[12] != S(0) || m[13] != S(0) || m[14] != S(0) || m[15] != S(1);
+            } else {
+                b = true;
+                for (uint32_t k = 0; k < c.row; ++k) b = b && m[k] == S(0);
+            }
+            bad = bad || b;
         }
     } else {
         for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < c.n; i += stride)
@@ -94,13 +104,16 @@ extern "C" int r2s_value_checks(const r2s_value_check* checks, uint32_t n_checks
     for (uint32_t i = 0; i < n_checks; ++i) {
         const r2s_value_check& c = checks[i];
         R2S_REQUIRE(c.dtype == R2S_F32 || c.dtype == R2S_F64, "check %u: dtype must be R2S_F32 or R2S_F64", i);
-        R2S_REQUIRE(c.kind >= R2S_CHECK_NONFINITE && c.kind <= R2S_CHECK_ZERO_ROW, "check %u: unknown kind %d", i,
+        R2S_REQUIRE(c.kind >= R2S_CHECK_NONFINITE && c.kind <= R2S_CHECK_VIEW_ROW, "check %u: unknown kind %d", i,
                     c.kind);
         R2S_REQUIRE(c.n == 0 || c.data, "check %u: data must be non-null", i);
-        R2S_REQUIRE(c.kind != R2S_CHECK_ZERO_ROW || (c.row > 0 && c.n % c.row == 0),
+        R2S_REQUIRE(c.kind < R2S_CHECK_ZERO_ROW || (c.row > 0 && c.n % c.row == 0),
                     "check %u: row must be positive and divide n", i);
+        R2S_REQUIRE((c.kind != R2S_CHECK_K_FOCAL && c.kind != R2S_CHECK_K_PINHOLE) || c.row == 9,
+                    "check %u: intrinsics rows hold 9 elements", i);
+        R2S_REQUIRE(c.kind != R2S_CHECK_VIEW_ROW || c.row == 16, "check %u: viewmat rows hold 16 elements", i);
         table.c[i] = c;
-        most = std::max<uint64_t>(most, c.kind == R2S_CHECK_ZERO_ROW ? c.n / c.row : c.n);
+        most = std::max<uint64_t>(most, c.kind >= R2S_CHECK_ZERO_ROW ? c.n / c.row : c.n);
     }
     value_checks_clear_kernel<<<1, kWave, 0, stream>>>(n_checks, flags);
     if (int rc = check_launch("value_checks_clear")) return rc;

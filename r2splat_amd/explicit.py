@@ -158,11 +158,9 @@ def _cameras(viewmats, Ks, means, height: int, width: int, checks: check.ValueCh
     Ks = check.tensor("Ks", Ks, (C, 3, 3), means)
     checks.finite("viewmats", viewmats)
     checks.finite("Ks", Ks)
-    checks.add("Ks must have positive fx and fy", lambda: (Ks[:, (0, 1), (0, 1)] <= 0).any())
-    checks.add("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]",
-               lambda: (Ks[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any() | (Ks[:, 2, 2] != 1).any())
-    checks.add("viewmats must have homogeneous last row [0, 0, 0, 1]",
-               lambda: (viewmats[:, 3] != means.new_tensor([0, 0, 0, 1])).any())
+    checks.element("Ks must have positive fx and fy", Ks, be.CHECK_K_FOCAL, 9)
+    checks.element("Ks must be pinhole intrinsics with zero skew and last row [0, 0, 1]", Ks, be.CHECK_K_PINHOLE, 9)
+    checks.element("viewmats must have homogeneous last row [0, 0, 0, 1]", viewmats, be.CHECK_VIEW_ROW, 16)
     return viewmats, Ks
 
 

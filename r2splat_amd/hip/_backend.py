@@ -34,6 +34,7 @@ class ValueCheck(ctypes.Structure):
 
 
 CHECK_NONFINITE, CHECK_NONPOSITIVE, CHECK_NEGATIVE, CHECK_ZERO_ROW = 0, 1, 2, 3
+CHECK_K_FOCAL, CHECK_K_PINHOLE, CHECK_VIEW_ROW = 4, 5, 6
 MAX_VALUE_CHECKS = 16
 
 

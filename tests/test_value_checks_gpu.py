@@ -74,12 +74,54 @@ def test_native_checks_equal_torch_predicates(device, dtype):
     assert 0 < sum(want) < len(want)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_native_camera_checks_equal_the_matrix_predicates(device, dtype):
+    """Intrinsics [C, 3, 3] and viewmats [C, 4, 4] with one entry changed per
+    case (NaN, infinities, signed zeros, a tiny skew), against the predicates
+    written on the matrices' entries."""
+    C = 40
+    K0 = torch.tensor([[20.0, 0, 8], [0, 30, 9], [0, 0, 1]], dtype=dtype).repeat(C, 1, 1)
+    V0 = torch.eye(4, dtype=dtype).repeat(C, 1, 1)
+    V0[:, :3, 3] = torch.rand(C, 3, dtype=dtype)
+    cases = [("K", K0.clone()), ("V", V0.clone())]
+    for (r, c) in ((0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1), (2, 2), (0, 2)):
+        for v in (0.0, -0.0, -1.0, 1e-30, 1.0, float("nan"), float("inf")):
+            K = K0.clone()
+            K[C - 1 if (r + c) % 2 else 0, r, c] = v
+            cases.append(("K", K))
+    for c in range(4):
+        for v in (0.0, -0.0, 1.0, 1e-30, float("nan")):
+            V = V0.clone()
+            V[17, 3, c] = v
+            cases.append(("V", V))
+    entries, want = [], []
+    for kind_of, m in cases:
+        m = m.to(device)
+        if kind_of == "K":
+            entries += [(m, be.CHECK_K_FOCAL, 9), (m, be.CHECK_K_PINHOLE, 9)]
+            want += [bool((m[:, (0, 1), (0, 1)] <= 0).any()),
+                     bool((m[:, (0, 1, 2, 2), (1, 0, 0, 1)] != 0).any() | (m[:, 2, 2] != 1).any())]
+        else:
+            entries.append((m, be.CHECK_VIEW_ROW, 16))
+            want.append(bool((m[:, 3] != m.new_tensor([0, 0, 0, 1])).any()))
+    got = _native_flags(entries, device)
+    assert got == [int(w) for w in want]
+    assert [int(bool(check._element_predicate(t, k, r))) for t, k, r in entries] == got
+    assert 0 < sum(want) < len(want)
+
+
 def test_native_checks_reject_bad_tables(device):
     t = torch.ones(8, device=device)
     table = (be.ValueCheck * 1)()
     table[0].data, table[0].n, table[0].dtype, table[0].kind, table[0].row = t.data_ptr(), 8, 0, be.CHECK_ZERO_ROW, 3
     flags = torch.zeros(1, dtype=torch.int32, device=device)
     with pytest.raises(RuntimeError, match="row must be positive and divide n"):
+        be.call("r2s_value_checks", ctypes.addressof(table), 1, flags.data_ptr(), be.stream_of(device))
+    table[0].kind, table[0].row = be.CHECK_K_FOCAL, 4
+    with pytest.raises(RuntimeError, match="intrinsics rows hold 9 elements"):
+        be.call("r2s_value_checks", ctypes.addressof(table), 1, flags.data_ptr(), be.stream_of(device))
+    table[0].kind, table[0].row = be.CHECK_VIEW_ROW, 8
+    with pytest.raises(RuntimeError, match="viewmat rows hold 16 elements"):
         be.call("r2s_value_checks", ctypes.addressof(table), 1, flags.data_ptr(), be.stream_of(device))
     table[0].kind = 9
     with pytest.raises(RuntimeError, match="unknown kind"):
@@ -215,3 +257,5 @@ def test_camera_checks_still_run(device):
     bad["means"][0, 0] = float("nan")
     with pytest.raises(ValueError, match="^means must contain only finite values$"):
         _rasterize(bad, device, torch.float32, fx=0.0)
+    with pytest.raises(ValueError, match="^Ks must contain only finite values$"):
+        _rasterize(p, device, torch.float32, fx=float("nan"))
