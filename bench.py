@@ -175,18 +175,22 @@ def main():
         return torch.autograd.grad(loss, params)
 
     def timed(backend, steps, warmup):
+        """(ms per step without stage events, stage times from a second pass
+        of the same steps with them)."""
         explicit.BACKEND = backend
         for _ in range(warmup):
             step()
-        torch.cuda.synchronize()
-        profiling.reset()
-        profiling.enable(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        profiling.reset()
+        profiling.enable(True)
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
         profiling.enable(False)
         return (t1 - t0) * 1e3 / steps, profiling.collect()
 
@@ -666,13 +670,17 @@ def voxel_timing(sc, steps):
 
     vstep()
     torch.cuda.synchronize()
-    profiling.reset()
-    profiling.enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         vstep()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    # stage times from a second pass with stage events (which add host work)
+    profiling.reset()
+    profiling.enable(True)
+    for _ in range(steps):
+        vstep()
+    torch.cuda.synchronize()
     profiling.enable(False)
     return {"ms_per_step": round(ms, 3), "iters_per_s": round(1e3 / ms, 3),
             "stages_ms": {k: round(v[0], 4) for k, v in profiling.collect().items()}}

@@ -38,7 +38,7 @@ extern "C" const char* r2s_last_error(void) { return r2s::g_error; }
 namespace r2s {
 
 constexpr int kCheckThreads = 256;
-constexpr uint32_t kCheckBlocks = 256;
+constexpr uint32_t kCheckBlocks = 1024;  // enough workgroups per check to stream a buffer at HBM rate
 
 struct ValueCheckTable {
     r2s_value_check c[R2S_MAX_VALUE_CHECKS];
@@ -117,7 +117,7 @@ extern "C" int r2s_value_checks(const r2s_value_check* checks, uint32_t n_checks
     }
     value_checks_clear_kernel<<<1, kWave, 0, stream>>>(n_checks, flags);
     if (int rc = check_launch("value_checks_clear")) return rc;
-    constexpr uint64_t kPerBlock = kCheckThreads * 8;
+    constexpr uint64_t kPerBlock = kCheckThreads * 4;
     const uint64_t want = std::max<uint64_t>(1, (most + kPerBlock - 1) / kPerBlock);
     const uint32_t blocks = uint32_t(std::min<uint64_t>(kCheckBlocks, want));
     value_checks_kernel<<<dim3(blocks, n_checks), kCheckThreads, 0, stream>>>(table, flags);
