@@ -174,7 +174,11 @@ def _constant_vector(value, means):
     are never written), or None for any other form of the argument."""
     if not (isinstance(value, (tuple, list)) and len(value) == 3 and all(type(v) in (int, float) for v in value)):
         return None
-    key = (tuple(value), means.device, means.dtype)
+    try:
+        # hex keys tell -0.0 from 0.0 and match NaN
+        key = (tuple(float(v).hex() for v in value), means.device, means.dtype)
+    except OverflowError:
+        return None
     hit = _FRAME_CACHE.get(key)
     if hit is None:
         try:

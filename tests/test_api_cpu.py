@@ -173,3 +173,25 @@ def test_shape_configurations_are_valid_kernel_configurations():
                     if beta or side < 512:
                         assert tile == 16
                     assert heuristic_config(n, views, side, side, beta) == KernelConfig(tile, fw, bw)
+
+
+def test_python_volume_frames_are_checked_on_the_host():
+    """Volume-frame vectors given as Python numbers: rounded to the
+    primitives' precision before the finiteness and sign checks, cached per
+    value (signed zeros apart), other forms left to the tensor checks."""
+    from r2splat_amd import explicit
+
+    like = torch.zeros(1, 3)
+    a = explicit._constant_vector((0.0, -0.0, 2), like)
+    b = explicit._constant_vector((0.0, 0.0, 2), like)
+    assert a is not b and torch.equal(a[0], b[0]) and a[0][1].item() == 0.0
+    assert torch.signbit(a[0][1]) and not torch.signbit(b[0][1])
+    assert explicit._constant_vector((0.0, -0.0, 2), like) is a
+    assert a[2:] == (False, True)  # finite; not all positive
+    assert explicit._constant_vector((1.0, 2.0, 3.0), like)[2:] == (False, False)
+    assert explicit._constant_vector((float("nan"), 1.0, 1.0), like)[2:] == (True, False)
+    assert explicit._constant_vector((1e39, 1.0, 1.0), like)[2] is True  # overflows float32
+    assert explicit._constant_vector((1e39, 1.0, 1.0), like.double())[2] is False
+    assert explicit._constant_vector((1e39, 1.0, 1.0), like.double())[1].dtype == torch.float64
+    for other in ((1.0, 1.0), [1.0, 1.0, 1.0, 1.0], (10**400, 1, 1), (True, 1, 1), "abc", torch.ones(3)):
+        assert explicit._constant_vector(other, like) is None
