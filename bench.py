@@ -682,8 +682,20 @@ def voxel_timing(sc, steps):
         vstep()
     torch.cuda.synchronize()
     profiling.enable(False)
-    return {"ms_per_step": round(ms, 3), "iters_per_s": round(1e3 / ms, 3),
-            "stages_ms": {k: round(v[0], 4) for k, v in profiling.collect().items()}}
+    stages = {k: round(v[0], 4) for k, v in profiling.collect().items()}
+    # intersections (primitive, 8^3 tile) of this volume: each kernel evaluates
+    # every voxel of each intersected tile (512 voxel-primitive pairs)
+    from r2splat_amd import explicit
+
+    with torch.no_grad():
+        _, ctx = explicit.voxelize_fwd(*sc.params(), shape, volume_extent=(1.0, 1.0, 1.0))
+    n = int(ctx.buffers["isect_ids"].numel())
+    del ctx
+    pairs = n * 512
+    rates = {name: round(pairs / (stages[name] * 1e-3) / 1e9, 1) for name in ("voxelize_fwd", "voxelize_bwd")
+             if stages.get(name)}
+    return {"ms_per_step": round(ms, 3), "iters_per_s": round(1e3 / ms, 3), "stages_ms": stages,
+            "intersections": n, "tile_pairs": pairs, "tile_pairs_per_s_e9": rates}
 
 
 def _popcount(words):
