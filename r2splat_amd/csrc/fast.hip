@@ -1532,8 +1532,14 @@ fused_keep_mask_kernel(const uint2* __restrict__ entries, const Stored<BETA, VAR
 // ---------------------------------------------------------------------------
 constexpr int kParamFields = 12;  // v_mean 3, v_covariance 6, v_density, v_beta, v_uncertainty
 constexpr int kVjpWaves = 8;  // 8 waves fit the kernel in registers without spills
-// partials one wave stages at a time (its share of the LDS buffer, 8 floats each)
-constexpr uint32_t kVjpStageParts = kParamFields * kWave / kPartialStride;
+// partials one wave stages at a time (8 floats each): 192 (a camera's share of
+// a 64-primitive group is about 275 partials at 100k Gaussians x 64 views)
+// takes 0.30 ms where 96 took 0.33 ms per parameter VJP; the LDS stays within
+// the two workgroups per CU the registers allow
+constexpr uint32_t kVjpStageParts = 192;
+// one wave's share of the LDS buffer: its staged partials, later its sums
+constexpr uint32_t kVjpWaveFloats = kVjpStageParts * kPartialStride;
+static_assert(kVjpWaveFloats >= uint32_t(kParamFields * kWave), "a wave's sums fit its staging area");
 
 template <bool BETA, bool VAR>
 __global__ void __launch_bounds__(kVjpWaves * kWave)
@@ -1548,14 +1554,14 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
                        const uint32_t* __restrict__ perm) {
     // one LDS buffer: each wave's staging area during the camera loop, then the
     // same wave's partial sums (part[wave][field][lane]) for the final reduction
-    __shared__ __align__(16) float lds[kVjpWaves * kParamFields * kWave];
+    __shared__ __align__(16) float lds[kVjpWaves * kVjpWaveFloats];
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     // lane p of the primitive order: records (c, p), parameters and gradients of input primitive n
     const uint32_t pr = blockIdx.x * kWave + lane;
     const bool valid = pr < N;
     const uint32_t n = valid ? perm[pr] : pr;
     const uint32_t last = min(N - blockIdx.x * kWave, uint32_t(kWave)) - 1u;  // the last valid lane
-    float* stage = lds + wave * (kParamFields * kWave);
+    float* stage = lds + wave * kVjpWaveFloats;
     float acc[kParamFields] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, p[3] = {0.f, 0.f, 0.f};
     const uint32_t n_live = live(d_n, capacity);
@@ -1656,7 +1662,7 @@ fused_param_vjp_kernel(const float* __restrict__ means, const float* __restrict_
 #pragma unroll
     for (int f = 0; f < kParamFields; ++f) {
         float t = lds[f * kWave + lane];
-        for (int w = 1; w < kVjpWaves; ++w) t += lds[(w * kParamFields + f) * kWave + lane];
+        for (int w = 1; w < kVjpWaves; ++w) t += lds[w * kVjpWaveFloats + f * kWave + lane];
         acc[f] = t;
     }
     float q[4], mag, norm, R[9];
