@@ -254,7 +254,9 @@ def main():
         "pairs": pc,
         "stages_ms": {k: round(v[0], 4) for k, v in stages.items()},
         "eager_path": {"value": round(1e3 / eager_ms, 3), "unit": "iters/s", "ms_per_step": round(eager_ms, 3),
-                       "description": "the same fused step issued eagerly through the autograd wrapper (per step: the input and adjoint value checks and the overflow check read back to the host)"},
+                       "description": "the same fused step issued eagerly through the autograd wrapper (per "
+                                      "step: the input and adjoint value checks and the overflow check read "
+                                      "back to the host)"},
         "plain_path": None if plain is None else {
             "value": round(1e3 / plain[0], 3), "unit": "iters/s", "ms_per_step": round(plain[0], 3),
             "speedup_of_value": round(plain[0] / ms, 3),
