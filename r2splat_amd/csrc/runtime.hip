@@ -38,7 +38,7 @@ extern "C" const char* r2s_last_error(void) { return r2s::g_error; }
 namespace r2s {
 
 constexpr int kCheckThreads = 256;
-constexpr uint32_t kCheckBlocks = 1024;  // enough workgroups per check to stream a buffer at HBM rate
+constexpr uint32_t kCheckBlocks = 4096;  // enough workgroups per check to stream a large buffer at HBM rate
 
 struct ValueCheckTable {
     r2s_value_check c[R2S_MAX_VALUE_CHECKS];
